@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Headline benchmark: the 1024-bit resident multiplication chain on one MI355X.
+
+Workload (one "step"): 65,536 independent 1024-bit (32 x u32 limb) values, each multiplied 16 times by
+its own factor and rounded to nearest-even after every multiplication (the reference's resident
+`mul_chain` benchmark, benchmarks/benchmark.cpp:67-94).  Operands: random sign, uniform random limbs
+with the top bit set, exponent uniform in [-64, 64], seeded.  Inputs are resident in HBM (limb
+planes) before the timed region; one step is one lf_mul_chain launch over the whole batch.
+
+Prints ONE JSON line on stdout (diagnostics go to stderr).  Besides the contract fields it carries:
+  roofline      the chain kernel against the measured v_mad_u64_u32 rate of the device (VALU-bound),
+  cpu_baseline  the reference's own CPU arithmetic (core.hpp, compiled from the reference into
+                oracle/_ref/libref.so) on all the host cores it may use, plus MPFR serial/all-cores,
+  parity        the GPU output compared bit for bit with the CPU oracle on the same inputs.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+import threading
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+BITS, COUNT, CHAIN = 1024, 65536, 16
+# Reference (Apple M5 Max) device time for this exact workload: 3.192 ms per chain batch
+# (benchmarks/results/final.csv:145) -> 65,536 / 3.192 ms.
+REFERENCE_VALUES_PER_S = COUNT / 3.192e-3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_inputs(seed: int):
+    n = BITS // 32
+    rng = np.random.default_rng(seed)
+
+    def rec():
+        r = np.zeros((COUNT, n + 3), dtype=np.uint32)
+        r[:, :n] = rng.integers(0, 1 << 32, size=(COUNT, n), dtype=np.uint64).astype(np.uint32)
+        r[:, n - 1] |= np.uint32(0x80000000)
+        r[:, n] = (rng.integers(-64, 65, size=COUNT) & 0xFFFFFFFF).astype(np.uint32)
+        r[:, n + 1] = (np.where(rng.integers(0, 2, size=COUNT) == 1, 1, -1) & 0xFFFFFFFF).astype(np.uint32)
+        return r
+
+    return rec(), rec()
+
+
+def host_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(x, f, budget_s: float):
+    """The reference's CPU core (oracle/_ref/libref.so) -- or the oracle port when that library is
+    absent -- on a bounded sample of the same workload, serial and on all allowed host cores; then MPFR."""
+    ref = ROOT / "oracle" / "_ref" / "libref.so"
+    if ref.exists():
+        L = C.CDLL(str(ref))
+        fn, kind = L.ref_mul_chain, "reference"
+    else:
+        from oracle import lf_oracle
+
+        L = lf_oracle.lib()
+        fn, kind = L.lfo_mul_chain, "port"
+    fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    fn.restype = C.c_int
+    out = np.zeros_like(x)
+    # size the serial sample to the budget from a short probe
+    probe = 256
+    t0 = time.perf_counter()
+    fn(BITS, x.ctypes.data, f.ctypes.data, out.ctypes.data, probe, CHAIN)
+    per_value = (time.perf_counter() - t0) / probe
+    sample = int(min(COUNT, max(1024, budget_s / 2 / max(per_value, 1e-9))))
+    t0 = time.perf_counter()
+    fn(BITS, x.ctypes.data, f.ctypes.data, out.ctypes.data, sample, CHAIN)
+    serial = time.perf_counter() - t0
+    threads = host_threads()
+    chunks = np.array_split(np.arange(sample), threads)
+
+    def part(ix):
+        lo = int(ix[0]) if len(ix) else 0
+        k = len(ix)
+        if k:
+            stride = x.shape[1] * 4
+            fn(BITS, x.ctypes.data + lo * stride, f.ctypes.data + lo * stride, out.ctypes.data + lo * stride, k, CHAIN)
+
+    ths = [threading.Thread(target=part, args=(c,)) for c in chunks]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    parallel = time.perf_counter() - t0
+    res = {
+        "value": sample / parallel,
+        "unit": "values/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": f"{sample} of the {COUNT} 1024-bit values, 16-step chain each, oracle/_ref libref.so ref_mul_chain",
+        "serial_values_per_s": sample / serial,
+    }
+    mpfr = ROOT / "oracle" / "mpfr_chain_bench"
+    if mpfr.exists():
+        try:
+            p = subprocess.run([str(mpfr), str(BITS), str(COUNT), str(CHAIN), str(threads), "3"], capture_output=True,
+                               text=True, timeout=300)
+            m = json.loads(p.stdout)
+            res["mpfr"] = {"version": m["mpfr"], "threads": threads, "serial_values_per_s": COUNT / m["serial_s"],
+                           "parallel_values_per_s": COUNT / m["parallel_s"], "sample": f"all {COUNT} values, median of 3"}
+        except Exception as e:  # the MPFR leg is informative only
+            res["mpfr"] = {"error": str(e)}
+    return res
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU work for the baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})", file=sys.stderr)
+        return 2
+
+    from limbforge_amd import capi
+
+    log(f"[bench] device: {capi.device_name()}")
+    x, f = make_inputs(20261007 + BITS)
+    X = capi.Buffer(BITS, COUNT)
+    F = capi.Buffer(BITS, COUNT)
+    O = capi.Buffer(BITS, COUNT)
+    X.upload(x)
+    F.upload(f)
+    stream = capi.Stream()
+    e0, e1 = capi.Event(), capi.Event()
+    for _ in range(a.warmup):
+        capi.mul_chain(BITS, X, F, O, CHAIN, stream=stream.handle)
+    stream.synchronize()
+    log(f"[bench] warmup done ({a.warmup} steps); timing {a.steps} steps")
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(a.steps):
+        capi.mul_chain(BITS, X, F, O, CHAIN, stream=stream.handle)
+    e1.record(stream)
+    stream.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / a.steps
+    kernel_ms = e0.elapsed_ms(e1) / a.steps
+    value = COUNT / (ms_per_step * 1e-3)
+    log(f"[bench] {ms_per_step:.4f} ms/step wall, {kernel_ms:.4f} ms/launch (events)")
+
+    # roofline: algorithmic 32x32->64 word products per launch / launch time, against the measured
+    # sustained v_mad_u64_u32 rate (one such instruction per word product is the minimum)
+    products = COUNT * CHAIN * (BITS // 32) ** 2
+    achieved = products / (kernel_ms * 1e-3) / 1e12
+    rate = C.c_double(0)
+    capi.check(capi.lib().lf_calibrate_mad64(256 * 64, 4096, C.byref(rate)))
+    peak = rate.value / 1e12
+    roofline = {
+        "bound": "valu",
+        "achieved": round(achieved, 4),
+        "peak": round(peak, 4),
+        "unit": "Tword-products/s",
+        "frac": round(achieved / peak, 4),
+        "traffic": None,
+        "kernel": "mul_chain_kernel<32>",
+        "kernel_ms": round(kernel_ms, 5),
+        "peak_source": "measured: lf_calibrate_mad64 (8 independent v_mad_u64_u32 chains per lane, 16384 workgroups)",
+        "algorithmic_bytes": COUNT * 3 * (4 * 32 + 12),
+    }
+
+    parity = None
+    if not a.no_parity:
+        from oracle import lf_oracle
+
+        got = O.download()
+        want = lf_oracle.mul_chain(BITS, x, f, CHAIN)
+        mism = int((got != want).any(axis=1).sum())
+        parity = {"checked_values": COUNT, "mismatches": mism, "against": "CPU oracle (MPFR-pinned)"}
+        log(f"[bench] parity: {mism} mismatching values of {COUNT}")
+
+    base = None
+    if not a.no_cpu_baseline:
+        log("[bench] timing the CPU baseline")
+        base = cpu_baseline(x, f, a.cpu_budget)
+
+    line = {
+        "metric": "1024-bit mul-chain values/s",
+        "value": round(value, 1),
+        "unit": "values/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(value / REFERENCE_VALUES_PER_S, 3),
+        "dtype": "u32",
+        "data": "synthetic (seeded random limbs, exponents in [-64, 64])",
+        "config": {"workload": "1024-bit mul chain: 65,536 values x 16 steps, resident (config 0)", "bits": BITS,
+                   "values": COUNT, "chain_steps": CHAIN, "parallelism": "single GPU"},
+        "baseline_note": "vs Apple M5 Max device time 3.192 ms per batch (benchmarks/results/final.csv:145)",
+        "roofline": roofline,
+        "cpu_baseline": base,
+        "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

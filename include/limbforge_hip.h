@@ -1,0 +1,130 @@
+/*
+ * limbforge_hip.h — the C ABI of liblimbforge_hip.so, the MI355X (gfx950) implementation of the
+ * WolfNum/LimbForge data-parallel core.
+ *
+ * The reference exposes this path only as a C++17 API (include/limbforge/engine.hpp and linalg.hpp);
+ * it has no C ABI of its own.  Each entry point below names the reference interface it replaces, so a
+ * C++ facade (this repository's include/limbforge/*.hpp) or any FFI (ctypes, cgo, JNI) can drive the
+ * same kernels.  The ABI uses plain pointers, sizes and integer status codes: no C++ types, no
+ * exceptions.  Arithmetic errors are never reported here: they are status bits inside the results
+ * (division by zero 1, exponent overflow 2, invalid 4), as in the reference (docs/numerics.md:14-31).
+ *
+ * Element formats (engine.hpp:195): a real value of `bits` (multiple of 32 in [64, 1024]) is
+ * n = bits/32 little-endian uint32 limbs, int32 exponent, int32 sign, uint32 status (4n+12 bytes);
+ * a complex value is two of them (re, im).  Host arrays use that layout.  Device operands are
+ * "limb planes": n limb planes, then exponent, sign and status planes (a complex operand: the real
+ * planes, then the imaginary planes), each plane `pitch` uint32 elements long.
+ *
+ * Return codes: LF_OK, or one of the error classes below with a message in lf_last_error() (per
+ * host thread).  The C++ facade maps them to std::invalid_argument / std::logic_error /
+ * std::runtime_error as the reference throws them (engine.mm:20-28, linalg.mm:286,307-308).
+ */
+#ifndef LIMBFORGE_HIP_H
+#define LIMBFORGE_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LF_OK 0
+#define LF_INVALID_ARGUMENT 1
+#define LF_LOGIC_ERROR 2
+#define LF_RUNTIME_ERROR 3
+
+/* Operation numbering of limbforge::Operation (engine.hpp:13). complex_fma/complex_fms (11, 12)
+ * are not provided by this build. */
+enum {
+    LF_OP_ADD = 0, LF_OP_SUB = 1, LF_OP_MUL = 2, LF_OP_DIV = 3, LF_OP_COMPLEX_ADD = 4, LF_OP_COMPLEX_MUL = 5,
+    LF_OP_COMPLEX_DIV = 6, LF_OP_SQUARE = 7, LF_OP_SQRT = 8, LF_OP_FMA = 9, LF_OP_FMS = 10
+};
+
+/* A device operand: limb planes and their pitch (uint32 elements per plane). */
+typedef struct lf_operand {
+    void* planes;
+    uint64_t pitch;
+} lf_operand;
+
+/* Broadcast read: element i uses index (i / stride) % period; period 0 = no wrap (engine.hpp:19-25). */
+typedef struct lf_broadcast {
+    uint64_t stride;
+    uint64_t period;
+} lf_broadcast;
+
+/* Message of the last failing call on this host thread (empty string if none). */
+const char* lf_last_error(void);
+
+/* Library version; replaces limbforge::library_version (version.hpp:31, src/version.cpp). */
+void lf_library_version(unsigned* major, unsigned* minor, unsigned* patch);
+
+/* ---- device, memory, streams: the HIP runtime behind Engine (engine.hpp:184-227), Buffer<T>
+ * (engine.hpp:74-86), CommandBatch and Submission (engine.hpp:93-182) ---- */
+int lf_device_count(int* count);
+int lf_device_set(int device);
+/* Engine::device_name (engine.hpp:188) */
+int lf_device_name(char* out, size_t capacity);
+/* Engine::allocate (engine.hpp:225): device memory for resident operands */
+int lf_malloc(void** ptr, size_t bytes);
+int lf_free(void* ptr);
+/* pinned host staging */
+int lf_host_alloc(void** ptr, size_t bytes);
+int lf_host_free(void* ptr);
+/* direction: 0 host->device, 1 device->host, 2 device->device */
+int lf_memcpy_async(void* dst, const void* src, size_t bytes, int direction, void* stream);
+int lf_stream_create(void** stream);
+int lf_stream_destroy(void* stream);
+int lf_stream_synchronize(void* stream);
+int lf_event_create(void** event);
+int lf_event_destroy(void* event);
+int lf_event_record(void* event, void* stream);
+int lf_event_synchronize(void* event);
+/* 1 when the event has completed (Submission::ready, engine.hpp:101) */
+int lf_event_query(void* event, int* done);
+int lf_event_elapsed_ms(float* ms, void* start, void* end);
+/* Capture the launches issued on `stream` into a graph (a dependent op batch replayed as one
+ * submission; CommandBatch::submit, engine.hpp:180). */
+int lf_graph_begin_capture(void* stream);
+int lf_graph_end_capture(void* stream, void** graph_exec);
+int lf_graph_launch(void* graph_exec, void* stream);
+int lf_graph_destroy(void* graph_exec);
+
+/* ---- layout: Buffer<T>::upload / download (engine.hpp:84-85) between the public AoS element and
+ * limb planes; both arrays are device memory ---- */
+uint64_t lf_plane_pitch(uint64_t count);
+uint64_t lf_planes_bytes(int bits, int complex_values, uint64_t count);
+int lf_aos_to_planes(int bits, int complex_values, const void* aos, const lf_operand* planes, uint64_t count, void* stream);
+int lf_planes_to_aos(int bits, int complex_values, const lf_operand* planes, void* aos, uint64_t count, void* stream);
+
+/* ---- compute ---- */
+/* One element-wise operation over resident operands: the reference's CommandBatch::run overloads
+ * (engine.hpp:120-147) through CommandBatch::encode (engine.mm:332-346) and the kernels
+ * arithmetic / complex_arithmetic / fused_arithmetic (kernels.metal:8-33).
+ * out[i] = op(a[i], b[(i/bs)%bp], c[(i/cs)%cp]); b is ignored by unary ops, c by non-fused ops;
+ * b_index / c_index may be NULL (no broadcast).  Operands may alias out (element-wise). */
+int lf_elementwise(int bits, int op, const lf_operand* a, const lf_operand* b, const lf_operand* c,
+                   const lf_operand* out, uint64_t count, const lf_broadcast* b_index,
+                   const lf_broadcast* c_index, void* stream);
+
+/* The resident multiplication chain x <- RN(x * f[(i/fs)%fp]) repeated `steps` times in one launch:
+ * replaces `steps` dependent CommandBatch::run(Operation::mul, x, y, x) calls (engine.hpp:120-124,
+ * as benchmarks/benchmark.cpp:84-92 issues them).  out may alias x. */
+int lf_mul_chain(int bits, const lf_operand* x, const lf_operand* f, const lf_operand* out, uint64_t count,
+                 const lf_broadcast* f_index, int steps, void* stream);
+
+/* Host-array element-wise call: Engine::run / run_unary / run_ternary (engine.hpp:202-204;
+ * engine.mm:254-280, 479-499).  a, b, c, out are host arrays of AoS elements; b holds
+ * broadcast_elements(count, b_index) elements (engine.hpp:22-25), likewise c.  Returns after the
+ * device has finished; gpu_seconds (device time of the kernels) and wall_seconds may be NULL. */
+int lf_run_host(int bits, int op, const void* a, const void* b, const void* c, void* out, uint64_t count,
+                const lf_broadcast* b_index, const lf_broadcast* c_index, double* gpu_seconds, double* wall_seconds);
+
+/* ---- measurement support (no reference counterpart) ---- */
+/* Sustained v_mad_u64_u32 rate of the device (32x32->64 word products per second), measured with
+ * `blocks` workgroups of 256 lanes, each lane running `iters` rounds of eight independent chains. */
+int lf_calibrate_mad64(uint64_t blocks, int iters, double* products_per_second);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,138 @@
+"""Build recipe for liblimbforge_hip.so (HIP kernels + C ABI, gfx950) and the C++17 facade.
+
+Usage: python -m limbforge_amd.build [--jobs N] [--only hip|cxx]
+
+Everything is compiled in-tree: objects under limbforge_amd/_build/, libraries under
+limbforge_amd/lib/.  The element-wise kernels are compiled once per limb count (2..32) so that the 31
+widths build in parallel.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+HOST = PKG / "host"
+BUILD = PKG / "_build"
+LIB = PKG / "lib"
+INCLUDE = ROOT / "include"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    # no packed FP32 instructions anywhere in device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the limb loops are fully unrolled (1024 products at 1024 bits) so that every limb stays in a VGPR
+    "-mllvm", "-pragma-unroll-threshold=1000000",
+    f"-I{CSRC}",
+    f"-I{INCLUDE}",
+]
+WIDTHS = range(2, 33)
+
+
+def _run(cmd: list[str]) -> tuple[list[str], int, str]:
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    return cmd, p.returncode, p.stdout
+
+
+def _hip_jobs() -> list[tuple[Path, list[str]]]:
+    jobs = []
+    for n in WIDTHS:
+        out = BUILD / f"elementwise_w{n}.o"
+        jobs.append((out, [HIPCC, *HIP_FLAGS, f"-DLF_WORDS={n}", "-c", str(CSRC / "elementwise_w.hip"), "-o", str(out)]))
+    for name in ("layout", "capi", "calibrate"):
+        out = BUILD / f"{name}.o"
+        jobs.append((out, [HIPCC, *HIP_FLAGS, "-c", str(CSRC / f"{name}.hip"), "-o", str(out)]))
+    for src in sorted(CSRC.glob("gemm*.hip")) + sorted(CSRC.glob("mixed*.hip")):
+        out = BUILD / f"{src.stem}.o"
+        jobs.append((out, [HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(out)]))
+    return jobs
+
+
+def build_hip(jobs: int) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    LIB.mkdir(exist_ok=True)
+    work = _hip_jobs()
+    # the heaviest translation units first
+    work.sort(key=lambda j: -int(j[0].stem.split("_w")[-1]) if "_w" in j[0].stem else -100)
+    failures = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for cmd, rc, log in ex.map(lambda j: _run(j[1]), work):
+            if rc != 0:
+                failures.append((cmd, log))
+    if failures:
+        for cmd, log in failures:
+            sys.stderr.write(" ".join(cmd) + "\n" + log + "\n")
+        raise RuntimeError(f"{len(failures)} HIP translation unit(s) failed to compile")
+    objs = [str(o) for o, _ in work]
+    lib = LIB / "liblimbforge_hip.so"
+    tmp = LIB / "liblimbforge_hip.so.tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs]
+    _, rc, log = _run(cmd)
+    if rc != 0:
+        raise RuntimeError("link failed:\n" + log)
+    os.replace(tmp, lib)
+    return lib
+
+
+def build_cxx(jobs: int) -> Path | None:
+    """The C++17 facade (include/limbforge/*.hpp) over the C ABI: liblimbforge.so."""
+    srcs = sorted(HOST.glob("*.cpp"))
+    if not srcs:
+        return None
+    BUILD.mkdir(exist_ok=True)
+    LIB.mkdir(exist_ok=True)
+    cxx = os.environ.get("CXX", "g++")
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", "-pthread"]
+    work = []
+    for s in srcs:
+        out = BUILD / f"host_{s.stem}.o"
+        work.append((out, [cxx, *flags, "-c", str(s), "-o", str(out)]))
+    failures = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for cmd, rc, log in ex.map(lambda j: _run(j[1]), work):
+            if rc != 0:
+                failures.append((cmd, log))
+    if failures:
+        for cmd, log in failures:
+            sys.stderr.write(" ".join(cmd) + "\n" + log + "\n")
+        raise RuntimeError("C++ facade failed to compile")
+    lib = LIB / "liblimbforge.so"
+    cmd = [cxx, "-shared", "-o", str(lib), *[str(o) for o, _ in work], f"-L{LIB}", "-llimbforge_hip",
+           "-Wl,-rpath,$ORIGIN", "-pthread"]
+    _, rc, log = _run(cmd)
+    if rc != 0:
+        raise RuntimeError("C++ facade link failed:\n" + log)
+    return lib
+
+
+def main(argv: list[str] | None = None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=min(16, os.cpu_count() or 4))
+    ap.add_argument("--only", choices=["hip", "cxx"], default=None)
+    ap.add_argument("--clean", action="store_true")
+    a = ap.parse_args(argv)
+    if a.clean and BUILD.exists():
+        shutil.rmtree(BUILD)
+    if a.only in (None, "hip"):
+        print("built", build_hip(a.jobs))
+    if a.only in (None, "cxx"):
+        lib = build_cxx(a.jobs)
+        if lib:
+            print("built", lib)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,279 @@
+"""ctypes binding of liblimbforge_hip.so (include/limbforge_hip.h) for Python callers.
+
+This is a thin host-side mirror of the reference's Engine interface (include/limbforge/engine.hpp):
+`Engine.run / run_unary / run_ternary` on host arrays and resident `Buffer` operands with
+`Batch.run` / `Batch.mul_chain` / `Batch.submit`, with the same argument meaning and error classes
+(ValueError for std::invalid_argument, RuntimeError for logic/runtime errors).  Values are numpy
+uint32 arrays of shape (count, n+3) (real) or (count, 2*(n+3)) (complex): each row is the public
+element record (n limbs, exponent, sign, status) of engine.hpp:195.
+
+The library must be built (python -m limbforge_amd.build); loading fails loudly otherwise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "liblimbforge_hip.so"
+
+OPS = {
+    "add": 0, "sub": 1, "mul": 2, "div": 3, "complex_add": 4, "complex_mul": 5, "complex_div": 6,
+    "square": 7, "sqrt": 8, "fma": 9, "fms": 10,
+}
+COMPLEX_OPS = {4, 5, 6}
+UNARY_OPS = {7, 8}
+TERNARY_OPS = {9, 10}
+
+
+class Operand(C.Structure):
+    _fields_ = [("planes", C.c_void_p), ("pitch", C.c_uint64)]
+
+
+class Broadcast(C.Structure):
+    _fields_ = [("stride", C.c_uint64), ("period", C.c_uint64)]
+
+
+class LimbForgeLogicError(RuntimeError):
+    """std::logic_error of the reference API."""
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m limbforge_amd.build`")
+    L = C.CDLL(str(LIB_PATH))
+    vp, u64, i32 = C.c_void_p, C.c_uint64, C.c_int
+    P = C.POINTER
+    sig = {
+        "lf_last_error": (C.c_char_p, []),
+        "lf_library_version": (None, [P(C.c_uint), P(C.c_uint), P(C.c_uint)]),
+        "lf_device_count": (i32, [P(i32)]),
+        "lf_device_set": (i32, [i32]),
+        "lf_device_name": (i32, [C.c_char_p, C.c_size_t]),
+        "lf_malloc": (i32, [P(vp), C.c_size_t]),
+        "lf_free": (i32, [vp]),
+        "lf_host_alloc": (i32, [P(vp), C.c_size_t]),
+        "lf_host_free": (i32, [vp]),
+        "lf_memcpy_async": (i32, [vp, vp, C.c_size_t, i32, vp]),
+        "lf_stream_create": (i32, [P(vp)]),
+        "lf_stream_destroy": (i32, [vp]),
+        "lf_stream_synchronize": (i32, [vp]),
+        "lf_event_create": (i32, [P(vp)]),
+        "lf_event_destroy": (i32, [vp]),
+        "lf_event_record": (i32, [vp, vp]),
+        "lf_event_synchronize": (i32, [vp]),
+        "lf_event_query": (i32, [vp, P(i32)]),
+        "lf_event_elapsed_ms": (i32, [P(C.c_float), vp, vp]),
+        "lf_graph_begin_capture": (i32, [vp]),
+        "lf_graph_end_capture": (i32, [vp, P(vp)]),
+        "lf_graph_launch": (i32, [vp, vp]),
+        "lf_graph_destroy": (i32, [vp]),
+        "lf_plane_pitch": (u64, [u64]),
+        "lf_planes_bytes": (u64, [i32, i32, u64]),
+        "lf_aos_to_planes": (i32, [i32, i32, vp, P(Operand), u64, vp]),
+        "lf_planes_to_aos": (i32, [i32, i32, P(Operand), vp, u64, vp]),
+        "lf_elementwise": (i32, [i32, i32, P(Operand), P(Operand), P(Operand), P(Operand), u64, P(Broadcast), P(Broadcast), vp]),
+        "lf_mul_chain": (i32, [i32, P(Operand), P(Operand), P(Operand), u64, P(Broadcast), i32, vp]),
+        "lf_calibrate_mad64": (i32, [u64, i32, P(C.c_double)]),
+        "lf_run_host": (i32, [i32, i32, vp, vp, vp, vp, u64, P(Broadcast), P(Broadcast), P(C.c_double), P(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc == 0:
+        return
+    msg = lib().lf_last_error().decode()
+    if rc == 1:
+        raise ValueError(msg)
+    if rc == 2:
+        raise LimbForgeLogicError(msg)
+    raise RuntimeError(msg)
+
+
+def words(bits: int) -> int:
+    return bits // 32
+
+
+def record_words(bits: int, complex_values: bool = False) -> int:
+    return (bits // 32 + 3) * (2 if complex_values else 1)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().lf_device_count(C.byref(n)))
+    return n.value
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    check(lib().lf_device_name(buf, 256))
+    return buf.value.decode()
+
+
+class DeviceMemory:
+    """Owned device allocation (lf_malloc / lf_free)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().lf_malloc(C.byref(p), max(self.nbytes, 4)))
+        self.ptr = p.value
+
+    def free(self) -> None:
+        if self.ptr:
+            check(lib().lf_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def upload(self, host: np.ndarray, stream=None) -> None:
+        host = np.ascontiguousarray(host)
+        assert host.nbytes <= self.nbytes
+        check(lib().lf_memcpy_async(self.ptr, host.ctypes.data, host.nbytes, 0, stream))
+        check(lib().lf_stream_synchronize(stream))
+
+    def download(self, host: np.ndarray, stream=None) -> np.ndarray:
+        assert host.flags.c_contiguous and host.nbytes <= self.nbytes
+        check(lib().lf_memcpy_async(host.ctypes.data, self.ptr, host.nbytes, 1, stream))
+        check(lib().lf_stream_synchronize(stream))
+        return host
+
+
+class Buffer:
+    """A resident operand in limb planes (the reference's Buffer<Float<bits>> / Buffer<Complex<n>>)."""
+
+    def __init__(self, bits: int, count: int, complex_values: bool = False):
+        self.bits, self.count, self.complex = bits, int(count), bool(complex_values)
+        self.pitch = int(lib().lf_plane_pitch(self.count))
+        self.mem = DeviceMemory(int(lib().lf_planes_bytes(bits, int(self.complex), self.count)))
+
+    @property
+    def operand(self) -> Operand:
+        return Operand(self.mem.ptr, self.pitch)
+
+    def upload(self, records: np.ndarray, stream=None) -> None:
+        """Buffer::upload (engine.hpp:84): AoS records -> device -> limb planes."""
+        rec = np.ascontiguousarray(records, dtype=np.uint32)
+        assert rec.shape == (self.count, record_words(self.bits, self.complex)), rec.shape
+        if not self.count:
+            return
+        stage = DeviceMemory(rec.nbytes)
+        stage.upload(rec, stream)
+        op = self.operand
+        check(lib().lf_aos_to_planes(self.bits, int(self.complex), stage.ptr, C.byref(op), self.count, stream))
+        check(lib().lf_stream_synchronize(stream))
+        stage.free()
+
+    def download(self, stream=None) -> np.ndarray:
+        """Buffer::download (engine.hpp:85): limb planes -> AoS records on the host."""
+        out = np.zeros((self.count, record_words(self.bits, self.complex)), dtype=np.uint32)
+        if not self.count:
+            return out
+        stage = DeviceMemory(out.nbytes)
+        op = self.operand
+        check(lib().lf_planes_to_aos(self.bits, int(self.complex), C.byref(op), stage.ptr, self.count, stream))
+        stage.download(out, stream)
+        stage.free()
+        return out
+
+
+def _bcast(b):
+    if b is None:
+        return None
+    stride, period = b
+    return C.byref(Broadcast(stride, period))
+
+
+def elementwise(bits: int, op, a: Buffer, b: Buffer | None, c: Buffer | None, out: Buffer,
+                b_index=None, c_index=None, stream=None) -> None:
+    """CommandBatch::run (engine.hpp:120-147) on resident buffers, enqueued on `stream`."""
+    code = OPS[op] if isinstance(op, str) else int(op)
+    oa, oo = a.operand, out.operand
+    ob = b.operand if b is not None else oa
+    oc = c.operand if c is not None else oa
+    check(lib().lf_elementwise(bits, code, C.byref(oa), C.byref(ob), C.byref(oc), C.byref(oo), a.count,
+                               _bcast(b_index), _bcast(c_index), stream))
+
+
+def mul_chain(bits: int, x: Buffer, f: Buffer, out: Buffer, steps: int, f_index=None, stream=None) -> None:
+    ox, of, oo = x.operand, f.operand, out.operand
+    check(lib().lf_mul_chain(bits, C.byref(ox), C.byref(of), C.byref(oo), x.count, _bcast(f_index), int(steps), stream))
+
+
+def run_host(bits: int, op, a: np.ndarray, b: np.ndarray | None = None, c: np.ndarray | None = None,
+             b_index=None, c_index=None):
+    """Engine::run / run_unary / run_ternary (engine.hpp:202-204): host arrays in, host array out.
+    Returns (out, gpu_seconds, wall_seconds)."""
+    code = OPS[op] if isinstance(op, str) else int(op)
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    count = a.shape[0]
+    out = np.zeros_like(a)
+    pb = np.ascontiguousarray(b, dtype=np.uint32).ctypes.data if b is not None else None
+    bb = np.ascontiguousarray(b, dtype=np.uint32) if b is not None else None
+    cc = np.ascontiguousarray(c, dtype=np.uint32) if c is not None else None
+    g, w = C.c_double(0), C.c_double(0)
+    check(lib().lf_run_host(bits, code, a.ctypes.data, bb.ctypes.data if bb is not None else None,
+                            cc.ctypes.data if cc is not None else None, out.ctypes.data, count,
+                            _bcast(b_index), _bcast(c_index), C.byref(g), C.byref(w)))
+    del pb
+    return out, g.value, w.value
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        check(lib().lf_stream_create(C.byref(s)))
+        self.handle = s.value
+
+    def synchronize(self) -> None:
+        check(lib().lf_stream_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().lf_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        check(lib().lf_event_create(C.byref(e)))
+        self.handle = e.value
+
+    def record(self, stream=None) -> None:
+        check(lib().lf_event_record(self.handle, stream.handle if isinstance(stream, Stream) else stream))
+
+    def synchronize(self) -> None:
+        check(lib().lf_event_synchronize(self.handle))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float(0)
+        check(lib().lf_event_elapsed_ms(C.byref(ms), self.handle, end.handle))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().lf_event_destroy(self.handle)
+        except Exception:
+            pass

@@ -1,0 +1,445 @@
+// capi.hip — the C ABI of liblimbforge_hip.so (declared in include/limbforge_hip.h): argument checks with
+// the reference's error classes, width dispatch to the per-width kernel instantiations, and the
+// host-array entry that stages AoS arrays through device limb planes.
+#include "limbforge_hip.h"
+#include "lf_internal.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+
+namespace lf {
+hipError_t launch_aos_to_soa(const uint32_t* aos, uint32_t* soa, uint64_t pitch, uint32_t count, int record, hipStream_t s);
+hipError_t launch_soa_to_aos(const uint32_t* soa, uint32_t* aos, uint64_t pitch, uint32_t count, int record, hipStream_t s);
+} // namespace lf
+
+namespace {
+thread_local std::string last_error;
+
+int fail(int code, const std::string& message)
+{
+    last_error = message;
+    return code;
+}
+
+int device_error(hipError_t e, const char* where)
+{
+    return fail(LF_RUNTIME_ERROR, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define LF_HIP(call, where)                                                                                            \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) return device_error(e_, where);                                                          \
+    } while (0)
+
+bool valid_bits(int bits) { return bits >= 64 && bits <= 1024 && bits % 32 == 0; }
+
+bool is_complex(int op) { return op == LF_OP_COMPLEX_ADD || op == LF_OP_COMPLEX_MUL || op == LF_OP_COMPLEX_DIV; }
+bool is_unary(int op) { return op == LF_OP_SQUARE || op == LF_OP_SQRT; }
+bool is_ternary(int op) { return op == LF_OP_FMA || op == LF_OP_FMS; }
+
+// broadcast_elements (engine.hpp:22-25)
+uint64_t broadcast_elements(uint64_t count, const lf_broadcast* b)
+{
+    if (!b) return count;
+    uint64_t n = count / b->stride + (count % b->stride != 0);
+    return b->period && b->period < n ? b->period : n;
+}
+
+typedef hipError_t (*elementwise_fn)(int, const lf::ElemArgs&, hipStream_t);
+typedef hipError_t (*chain_fn)(const lf::ChainArgs&, hipStream_t);
+
+template <int N> void fill_tables(elementwise_fn* ew, chain_fn* ch)
+{
+    ew[N] = lf::launch_elementwise<N>;
+    ch[N] = lf::launch_mul_chain<N>;
+    if constexpr (N < 32) fill_tables<N + 1>(ew, ch);
+}
+
+struct Tables {
+    elementwise_fn ew[33] = {};
+    chain_fn chain[33] = {};
+    Tables() { fill_tables<2>(ew, chain); }
+};
+const Tables& tables()
+{
+    static const Tables t;
+    return t;
+}
+
+int check_broadcast(const lf_broadcast* b)
+{
+    if (!b) return LF_OK;
+    if (!b->stride) return fail(LF_INVALID_ARGUMENT, "broadcast stride must be positive");
+    if (b->stride > std::numeric_limits<uint32_t>::max() || b->period > std::numeric_limits<uint32_t>::max())
+        return fail(LF_INVALID_ARGUMENT, "broadcast exceeds 32-bit indexing");
+    return LF_OK;
+}
+
+int check_operand(const lf_operand* o, uint64_t elements, const char* name)
+{
+    if (!elements) return LF_OK;
+    if (!o || !o->planes) return fail(LF_INVALID_ARGUMENT, std::string("null operand ") + name);
+    if (o->pitch < elements) return fail(LF_INVALID_ARGUMENT, std::string("operand ") + name + " pitch is smaller than its element count");
+    return LF_OK;
+}
+
+// Per-host-thread device staging for lf_run_host (the reference keeps four cached staging buffers,
+// engine.mm:254-262).
+struct Staging {
+    void* dev[5] = {};
+    size_t bytes[5] = {};
+    hipEvent_t ev[2] = {};
+    hipStream_t stream = nullptr;
+    ~Staging()
+    {
+        for (auto* p : dev)
+            if (p) (void)hipFree(p);
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    hipError_t reserve(int k, size_t n)
+    {
+        if (bytes[k] >= n) return hipSuccess;
+        if (dev[k]) (void)hipFree(dev[k]);
+        dev[k] = nullptr;
+        bytes[k] = 0;
+        hipError_t e = hipMalloc(&dev[k], n);
+        if (e == hipSuccess) bytes[k] = n;
+        return e;
+    }
+};
+thread_local Staging staging;
+} // namespace
+
+extern "C" {
+
+const char* lf_last_error(void) { return last_error.c_str(); }
+
+void lf_library_version(unsigned* major, unsigned* minor, unsigned* patch)
+{
+    if (major) *major = 1;
+    if (minor) *minor = 3;
+    if (patch) *patch = 1;
+}
+
+int lf_device_count(int* count)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    if (count) *count = n;
+    return LF_OK;
+}
+
+int lf_device_set(int device)
+{
+    LF_HIP(hipSetDevice(device), "hipSetDevice");
+    return LF_OK;
+}
+
+int lf_device_name(char* out, size_t capacity)
+{
+    int dev = 0;
+    LF_HIP(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t prop;
+    LF_HIP(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    if (out && capacity) {
+        std::snprintf(out, capacity, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    return LF_OK;
+}
+
+int lf_malloc(void** ptr, size_t bytes)
+{
+    if (!ptr) return fail(LF_INVALID_ARGUMENT, "null pointer");
+    *ptr = nullptr;
+    if (!bytes) return LF_OK;
+    LF_HIP(hipMalloc(ptr, bytes), "hipMalloc");
+    return LF_OK;
+}
+
+int lf_free(void* ptr)
+{
+    if (ptr) LF_HIP(hipFree(ptr), "hipFree");
+    return LF_OK;
+}
+
+int lf_host_alloc(void** ptr, size_t bytes)
+{
+    if (!ptr) return fail(LF_INVALID_ARGUMENT, "null pointer");
+    *ptr = nullptr;
+    if (!bytes) return LF_OK;
+    LF_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault), "hipHostMalloc");
+    return LF_OK;
+}
+
+int lf_host_free(void* ptr)
+{
+    if (ptr) LF_HIP(hipHostFree(ptr), "hipHostFree");
+    return LF_OK;
+}
+
+int lf_memcpy_async(void* dst, const void* src, size_t bytes, int direction, void* stream)
+{
+    if (!bytes) return LF_OK;
+    if (!dst || !src) return fail(LF_INVALID_ARGUMENT, "null copy pointer");
+    hipMemcpyKind kind = direction == 0 ? hipMemcpyHostToDevice : direction == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    LF_HIP(hipMemcpyAsync(dst, src, bytes, kind, (hipStream_t)stream), "hipMemcpyAsync");
+    return LF_OK;
+}
+
+int lf_stream_create(void** stream)
+{
+    if (!stream) return fail(LF_INVALID_ARGUMENT, "null pointer");
+    hipStream_t s = nullptr;
+    LF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    *stream = s;
+    return LF_OK;
+}
+
+int lf_stream_destroy(void* stream)
+{
+    if (stream) LF_HIP(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy");
+    return LF_OK;
+}
+
+int lf_stream_synchronize(void* stream)
+{
+    LF_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+    return LF_OK;
+}
+
+int lf_event_create(void** event)
+{
+    if (!event) return fail(LF_INVALID_ARGUMENT, "null pointer");
+    hipEvent_t e = nullptr;
+    LF_HIP(hipEventCreate(&e), "hipEventCreate");
+    *event = e;
+    return LF_OK;
+}
+
+int lf_event_destroy(void* event)
+{
+    if (event) LF_HIP(hipEventDestroy((hipEvent_t)event), "hipEventDestroy");
+    return LF_OK;
+}
+
+int lf_event_record(void* event, void* stream)
+{
+    LF_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), "hipEventRecord");
+    return LF_OK;
+}
+
+int lf_event_synchronize(void* event)
+{
+    LF_HIP(hipEventSynchronize((hipEvent_t)event), "hipEventSynchronize");
+    return LF_OK;
+}
+
+int lf_event_query(void* event, int* done)
+{
+    hipError_t e = hipEventQuery((hipEvent_t)event);
+    if (e == hipErrorNotReady) {
+        if (done) *done = 0;
+        return LF_OK;
+    }
+    LF_HIP(e, "hipEventQuery");
+    if (done) *done = 1;
+    return LF_OK;
+}
+
+int lf_event_elapsed_ms(float* ms, void* start, void* end)
+{
+    LF_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end), "hipEventElapsedTime");
+    return LF_OK;
+}
+
+int lf_graph_begin_capture(void* stream)
+{
+    if (!stream) return fail(LF_INVALID_ARGUMENT, "graph capture needs a created stream");
+    LF_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    return LF_OK;
+}
+
+int lf_graph_end_capture(void* stream, void** graph_exec)
+{
+    if (!graph_exec) return fail(LF_INVALID_ARGUMENT, "null pointer");
+    hipGraph_t g = nullptr;
+    LF_HIP(hipStreamEndCapture((hipStream_t)stream, &g), "hipStreamEndCapture");
+    hipGraphExec_t x = nullptr;
+    hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return device_error(e, "hipGraphInstantiate");
+    *graph_exec = x;
+    return LF_OK;
+}
+
+int lf_graph_launch(void* graph_exec, void* stream)
+{
+    LF_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream), "hipGraphLaunch");
+    return LF_OK;
+}
+
+int lf_graph_destroy(void* graph_exec)
+{
+    if (graph_exec) LF_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec), "hipGraphExecDestroy");
+    return LF_OK;
+}
+
+uint64_t lf_plane_pitch(uint64_t count) { return (count + 63) / 64 * 64; }
+
+uint64_t lf_planes_bytes(int bits, int complex_values, uint64_t count)
+{
+    if (!valid_bits(bits)) return 0;
+    return (uint64_t)(bits / 32 + 3) * (complex_values ? 2 : 1) * lf_plane_pitch(count) * 4;
+}
+
+int lf_aos_to_planes(int bits, int complex_values, const void* aos, const lf_operand* planes, uint64_t count, void* stream)
+{
+    if (!valid_bits(bits)) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    if (count > std::numeric_limits<uint32_t>::max()) return fail(LF_INVALID_ARGUMENT, "batch exceeds 32-bit indexing");
+    if (!count) return LF_OK;
+    if (!aos) return fail(LF_INVALID_ARGUMENT, "null AoS array");
+    if (int r = check_operand(planes, count, "planes")) return r;
+    const int record = (bits / 32 + 3) * (complex_values ? 2 : 1);
+    LF_HIP(lf::launch_aos_to_soa((const uint32_t*)aos, (uint32_t*)planes->planes, planes->pitch, (uint32_t)count, record, (hipStream_t)stream),
+           "aos_to_planes launch");
+    return LF_OK;
+}
+
+int lf_planes_to_aos(int bits, int complex_values, const lf_operand* planes, void* aos, uint64_t count, void* stream)
+{
+    if (!valid_bits(bits)) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    if (count > std::numeric_limits<uint32_t>::max()) return fail(LF_INVALID_ARGUMENT, "batch exceeds 32-bit indexing");
+    if (!count) return LF_OK;
+    if (!aos) return fail(LF_INVALID_ARGUMENT, "null AoS array");
+    if (int r = check_operand(planes, count, "planes")) return r;
+    const int record = (bits / 32 + 3) * (complex_values ? 2 : 1);
+    LF_HIP(lf::launch_soa_to_aos((const uint32_t*)planes->planes, (uint32_t*)aos, planes->pitch, (uint32_t)count, record, (hipStream_t)stream),
+           "planes_to_aos launch");
+    return LF_OK;
+}
+
+int lf_elementwise(int bits, int op, const lf_operand* a, const lf_operand* b, const lf_operand* c, const lf_operand* out,
+                   uint64_t count, const lf_broadcast* b_index, const lf_broadcast* c_index, void* stream)
+{
+    // engine.mm:24-28, 332-336
+    if (!valid_bits(bits)) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    if (count > std::numeric_limits<uint32_t>::max()) return fail(LF_INVALID_ARGUMENT, "batch exceeds 32-bit indexing");
+    if (op < 0 || op > 12) return fail(LF_INVALID_ARGUMENT, "invalid operation");
+    if (op > LF_OP_FMS) return fail(LF_INVALID_ARGUMENT, "complex_fma / complex_fms are not provided by this build");
+    if (int r = check_broadcast(b_index)) return r;
+    if (int r = check_broadcast(c_index)) return r;
+    if (!count) return LF_OK;
+    if (int r = check_operand(a, count, "a")) return r;
+    if (int r = check_operand(out, count, "out")) return r;
+    if (!is_unary(op))
+        if (int r = check_operand(b, broadcast_elements(count, b_index), "b")) return r;
+    if (is_ternary(op))
+        if (int r = check_operand(c, broadcast_elements(count, c_index), "c")) return r;
+    lf::ElemArgs g;
+    g.a = (const uint32_t*)a->planes;
+    g.pa = a->pitch;
+    g.b = is_unary(op) ? g.a : (const uint32_t*)b->planes;
+    g.pb = is_unary(op) ? g.pa : b->pitch;
+    g.c = is_ternary(op) ? (const uint32_t*)c->planes : g.a;
+    g.pc = is_ternary(op) ? c->pitch : g.pa;
+    g.out = (uint32_t*)out->planes;
+    g.po = out->pitch;
+    g.count = (uint32_t)count;
+    g.b_stride = b_index ? (uint32_t)b_index->stride : 1;
+    g.b_period = b_index ? (uint32_t)b_index->period : 0;
+    g.c_stride = c_index ? (uint32_t)c_index->stride : 1;
+    g.c_period = c_index ? (uint32_t)c_index->period : 0;
+    LF_HIP(tables().ew[bits / 32](op, g, (hipStream_t)stream), "elementwise launch");
+    return LF_OK;
+}
+
+int lf_mul_chain(int bits, const lf_operand* x, const lf_operand* f, const lf_operand* out, uint64_t count,
+                 const lf_broadcast* f_index, int steps, void* stream)
+{
+    if (!valid_bits(bits)) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    if (count > std::numeric_limits<uint32_t>::max()) return fail(LF_INVALID_ARGUMENT, "batch exceeds 32-bit indexing");
+    if (steps < 0) return fail(LF_INVALID_ARGUMENT, "steps must be nonnegative");
+    if (int r = check_broadcast(f_index)) return r;
+    if (!count) return LF_OK;
+    if (int r = check_operand(x, count, "x")) return r;
+    if (int r = check_operand(out, count, "out")) return r;
+    if (int r = check_operand(f, broadcast_elements(count, f_index), "f")) return r;
+    lf::ChainArgs g;
+    g.x = (const uint32_t*)x->planes;
+    g.px = x->pitch;
+    g.f = (const uint32_t*)f->planes;
+    g.pf = f->pitch;
+    g.out = (uint32_t*)out->planes;
+    g.po = out->pitch;
+    g.count = (uint32_t)count;
+    g.f_stride = f_index ? (uint32_t)f_index->stride : 1;
+    g.f_period = f_index ? (uint32_t)f_index->period : 0;
+    g.steps = steps;
+    LF_HIP(tables().chain[bits / 32](g, (hipStream_t)stream), "mul_chain launch");
+    return LF_OK;
+}
+
+int lf_run_host(int bits, int op, const void* a, const void* b, const void* c, void* out, uint64_t count,
+                const lf_broadcast* b_index, const lf_broadcast* c_index, double* gpu_seconds, double* wall_seconds)
+{
+    const auto start = std::chrono::steady_clock::now();
+    if (gpu_seconds) *gpu_seconds = 0;
+    if (wall_seconds) *wall_seconds = 0;
+    if (!valid_bits(bits)) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    if (count > std::numeric_limits<uint32_t>::max()) return fail(LF_INVALID_ARGUMENT, "batch exceeds 32-bit indexing");
+    if (op < 0 || op > 12) return fail(LF_INVALID_ARGUMENT, "invalid operation");
+    if (op > LF_OP_FMS) return fail(LF_INVALID_ARGUMENT, "complex_fma / complex_fms are not provided by this build");
+    if (int r = check_broadcast(b_index)) return r;
+    if (int r = check_broadcast(c_index)) return r;
+    if (!count) return LF_OK;
+    if (!a || !out || (!is_unary(op) && !b) || (is_ternary(op) && !c)) return fail(LF_INVALID_ARGUMENT, "null host array");
+    const int parts = is_complex(op) ? 2 : 1;
+    const uint64_t E = (uint64_t)(bits / 8 + 12) * parts;
+    const uint64_t nb = is_unary(op) ? 0 : broadcast_elements(count, b_index);
+    const uint64_t nc = is_ternary(op) ? broadcast_elements(count, c_index) : 0;
+    Staging& st = staging;
+    if (!st.stream) LF_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& e : st.ev)
+        if (!e) LF_HIP(hipEventCreate(&e), "hipEventCreate");
+    const uint64_t pa = lf_plane_pitch(count), pb = lf_plane_pitch(nb), pc = lf_plane_pitch(nc);
+    const uint64_t aos_bytes = count * E;
+    LF_HIP(st.reserve(0, aos_bytes), "hipMalloc (staging)");
+    LF_HIP(st.reserve(1, lf_planes_bytes(bits, parts == 2, count)), "hipMalloc (staging)");
+    LF_HIP(st.reserve(2, nb ? lf_planes_bytes(bits, parts == 2, nb) : 4), "hipMalloc (staging)");
+    LF_HIP(st.reserve(3, nc ? lf_planes_bytes(bits, parts == 2, nc) : 4), "hipMalloc (staging)");
+    LF_HIP(st.reserve(4, lf_planes_bytes(bits, parts == 2, count)), "hipMalloc (staging)");
+    hipStream_t s = st.stream;
+    lf_operand A{st.dev[1], pa}, B{st.dev[2], pb}, C{st.dev[3], pc}, O{st.dev[4], pa};
+    const int cplx = parts == 2;
+    // stage in: AoS bytes to the device, then transposed into planes
+    LF_HIP(hipMemcpyAsync(st.dev[0], a, aos_bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    if (int r = lf_aos_to_planes(bits, cplx, st.dev[0], &A, count, s)) return r;
+    if (nb) {
+        LF_HIP(hipMemcpyAsync(st.dev[0], b, nb * E, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+        if (int r = lf_aos_to_planes(bits, cplx, st.dev[0], &B, nb, s)) return r;
+    }
+    if (nc) {
+        LF_HIP(hipMemcpyAsync(st.dev[0], c, nc * E, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+        if (int r = lf_aos_to_planes(bits, cplx, st.dev[0], &C, nc, s)) return r;
+    }
+    LF_HIP(hipEventRecord(st.ev[0], s), "hipEventRecord");
+    if (int r = lf_elementwise(bits, op, &A, nb ? &B : &A, nc ? &C : &A, &O, count, b_index, c_index, s)) return r;
+    LF_HIP(hipEventRecord(st.ev[1], s), "hipEventRecord");
+    if (int r = lf_planes_to_aos(bits, cplx, &O, st.dev[0], count, s)) return r;
+    LF_HIP(hipMemcpyAsync(out, st.dev[0], aos_bytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    LF_HIP(hipStreamSynchronize(s), "device execution");
+    float ms = 0;
+    LF_HIP(hipEventElapsedTime(&ms, st.ev[0], st.ev[1]), "hipEventElapsedTime");
+    if (gpu_seconds) *gpu_seconds = ms * 1e-3;
+    if (wall_seconds) *wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    return LF_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,39 @@
+// lf_internal.hpp — declarations shared by the HIP translation units of liblimbforge_hip.so.
+//
+// Device layout of a resident operand ("limb planes", structure of arrays): for `count` values of N
+// words, plane k (k < N) holds limb k of every value, then one plane each of exponents, signs and
+// statuses; a complex operand stores the real planes and then the imaginary planes.  Every plane has
+// `pitch` 32-bit elements (count rounded up to 64, so each plane starts on a 256-byte boundary) and a
+// wave's 64 lanes read one plane word each: every limb load is one coalesced 256-byte transaction.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lf {
+
+struct ElemArgs {
+    const uint32_t* a;
+    const uint32_t* b;
+    const uint32_t* c;
+    uint32_t* out;
+    uint64_t pa, pb, pc, po; // plane pitches in elements
+    uint32_t count;
+    uint32_t b_stride, b_period, c_stride, c_period;
+};
+
+struct ChainArgs {
+    const uint32_t* x;
+    const uint32_t* f;
+    uint32_t* out;
+    uint64_t px, pf, po;
+    uint32_t count;
+    uint32_t f_stride, f_period;
+    int steps;
+};
+
+// Element-wise operation `op` (numbering of limbforge::Operation) on N-word values.
+template <int N> hipError_t launch_elementwise(int op, const ElemArgs& g, hipStream_t s);
+// x <- RN(x * f) `steps` times with x held in registers (one launch for the whole chain).
+template <int N> hipError_t launch_mul_chain(const ChainArgs& g, hipStream_t s);
+
+} // namespace lf

@@ -1,0 +1,663 @@
+// limb_core.hpp — register-resident multiprecision arithmetic for gfx950 (CDNA4), one value per lane.
+//
+// A value is sign * integer(w[0..N)) * 2^(e - (32N-1)) with the top bit of w[N-1] set when nonzero,
+// exponent e in [-1e9, 1e9], sign in {-1, 0, 1} and a status bitmask {1 division by zero, 2 exponent
+// overflow, 4 invalid} (the reference's number contract, include/limbforge/core.hpp:22-28).  Every
+// operation returns the round-to-nearest-even result of the exact value (bit-identical to MPFR).
+//
+// Design for the wave64 VALU: all limb arrays are indexed at compile time only (every loop over words
+// is unrolled), so limbs stay in VGPRs; variable shifts are barrel shifters (log2(W) select stages of
+// v_cndmask plus one v_alignbit funnel pass) instead of runtime-indexed scratch; 32x32->64 products are
+// v_mad_u64_u32 carry chains.  Loops whose trip count grows with N (division digits, square-root bits)
+// stay rolled and move their windows as shift registers, which keeps code size within the instruction
+// cache at 1024 bits.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lf {
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef int64_t i64;
+
+constexpr int EMAX = 1000000000;
+enum : u32 { ST_OK = 0, ST_DIV0 = 1, ST_OVERFLOW = 2, ST_INVALID = 4 };
+
+template <int N> struct Num {
+    u32 w[N];
+    int e;
+    int s;
+    u32 st;
+};
+
+template <int N> __device__ __forceinline__ Num<N> zero(u32 st)
+{
+    Num<N> r;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.w[i] = 0;
+    r.e = 0;
+    r.s = 0;
+    r.st = st;
+    return r;
+}
+
+template <int N> __device__ __forceinline__ Num<N> negate(Num<N> a)
+{
+    a.s = -a.s;
+    return a;
+}
+
+// ------------------------------------------------------------------ shifters
+// x <<= 32*k words, k a runtime value in [0, W]; vacated words are zero.
+template <int W> __device__ __forceinline__ void shl_words(u32 (&x)[W], int k)
+{
+#pragma unroll
+    for (int s = 1; s < W; s <<= 1) {
+        const bool on = (k & s) != 0;
+#pragma unroll
+        for (int i = W - 1; i >= 0; --i) x[i] = on ? (i >= s ? x[i - s] : 0u) : x[i];
+    }
+    if (k >= W) {
+#pragma unroll
+        for (int i = 0; i < W; ++i) x[i] = 0;
+    }
+}
+
+// x <<= b bits, b a runtime value in [0, 31].
+template <int W> __device__ __forceinline__ void shl_bits(u32 (&x)[W], int b)
+{
+    const u32 r = (u32)(32 - b) & 31u;
+#pragma unroll
+    for (int i = W - 1; i >= 0; --i) {
+        u32 lo = i ? x[i - 1] : 0u;
+        u32 v = __builtin_amdgcn_alignbit(x[i], lo, r);
+        x[i] = b ? v : x[i];
+    }
+}
+
+// dst (W words) = src (S words) * 2^off for a runtime bit offset; bits that fall below bit 0 of dst
+// are dropped and reported in the return value (nonzero iff any was set); bits above dst are dropped
+// (callers size dst so that none are).
+template <int W, int S> __device__ __forceinline__ bool place(u32 (&dst)[W], const u32 (&src)[S], i64 off)
+{
+    // Work in a (W+S)-word frame whose bit 32S is dst bit 0.
+    constexpr int F = W + S;
+    u32 t[F];
+#pragma unroll
+    for (int i = 0; i < F; ++i) t[i] = i < S ? src[i] : 0u;
+    bool sticky = false;
+    i64 f = off + 32 * S; // position of src bit 0 in the frame
+    if (f < 0) {
+        // every bit of src lies below dst
+#pragma unroll
+        for (int i = 0; i < S; ++i) sticky |= src[i] != 0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) dst[i] = 0;
+        return sticky;
+    }
+    int fw = (int)(f >> 5), fb = (int)(f & 31);
+    if (f >= 32 * (i64)F) fw = F; // everything above the frame (callers avoid this)
+    shl_bits(t, fb);
+    // bit-shift spill of the top word of src is lost only when S words + shift exceed the frame,
+    // which cannot happen for fw < F because t has W >= 1 spare words above src.
+    shl_words(t, fw);
+#pragma unroll
+    for (int i = 0; i < S; ++i) sticky |= t[i] != 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) dst[i] = t[S + i];
+    return sticky;
+}
+
+// Shift x left until bit 32W-1 is set; returns the number of leading zero bits, or -1 for zero.
+template <int W> __device__ __forceinline__ int normalize(u32 (&x)[W])
+{
+    int lzw = 0;
+    bool seen = false;
+#pragma unroll
+    for (int i = W - 1; i >= 0; --i) {
+        seen = seen || x[i] != 0;
+        lzw += seen ? 0 : 1;
+    }
+    if (lzw == W) return -1;
+    shl_words(x, lzw);
+    int b = __clz(x[W - 1]);
+    shl_bits(x, b);
+    return 32 * lzw + b;
+}
+
+// ------------------------------------------------------------------ rounding
+// x (W words) is normalized (bit 32W-1 set) and represents sign * x * 2^(msb - (32W-1)).
+// Round to nearest even to N words with the reference's range rule (core.hpp:81-99): the msb may be
+// emin-1 before rounding (a carry can rescue it) and must lie in [-1e9, 1e9] after.
+template <int N, int W>
+__device__ __forceinline__ Num<N> round_normalized(const u32 (&x)[W], i64 msb, int sign, u32 st, bool extra_sticky = false)
+{
+    if (msb > EMAX || msb < -(i64)EMAX - 1) return zero<N>(st | ST_OVERFLOW);
+    Num<N> r;
+    r.s = sign;
+    r.st = st;
+    if constexpr (W <= N) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) r.w[i] = i >= N - W ? x[i - (N - W)] : 0u;
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) r.w[i] = x[W - N + i];
+        const u32 g = x[W - N - 1];
+        bool sticky = extra_sticky || (g << 1) != 0;
+#pragma unroll
+        for (int i = 0; i < W - N - 1; ++i) sticky = sticky || x[i] != 0;
+        const bool inc = (g >> 31) && (sticky || (r.w[0] & 1u));
+        u32 c = inc ? 1u : 0u;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            u64 t = (u64)r.w[i] + c;
+            r.w[i] = (u32)t;
+            c = (u32)(t >> 32);
+        }
+        if (c) {
+            r.w[N - 1] = 0x80000000u;
+            msb += 1;
+        }
+    }
+    if (msb > EMAX || msb < -(i64)EMAX) return zero<N>(st | ST_OVERFLOW);
+    r.e = (int)msb;
+    return r;
+}
+
+// RN-even of sign * int(x) * 2^scale (the reference's pack for any word count, core.hpp:81-99).
+template <int N, int W> __device__ __forceinline__ Num<N> pack(u32 (&x)[W], i64 scale, int sign, u32 st)
+{
+    int lz = normalize(x);
+    if (lz < 0) return zero<N>(st);
+    return round_normalized<N, W>(x, scale + (32 * W - 1 - lz), sign, st);
+}
+
+// ------------------------------------------------------------- comparisons
+template <int N> __device__ __forceinline__ int compare_words(const u32 (&a)[N], const u32 (&b)[N])
+{
+    int cmp = 0;
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i)
+        if (cmp == 0 && a[i] != b[i]) cmp = a[i] > b[i] ? 1 : -1;
+    return cmp;
+}
+
+// |a| vs |b| for values without status (core.hpp:100-105)
+template <int N> __device__ __forceinline__ int magnitude_compare(const Num<N>& a, const Num<N>& b)
+{
+    if (!a.s) return b.s ? -1 : 0;
+    if (!b.s) return 1;
+    if (a.e != b.e) return a.e > b.e ? 1 : -1;
+    return compare_words<N>(a.w, b.w);
+}
+
+template <int N> __device__ __forceinline__ void swap_if(bool c, Num<N>& a, Num<N>& b)
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u32 x = a.w[i], y = b.w[i];
+        a.w[i] = c ? y : x;
+        b.w[i] = c ? x : y;
+    }
+    int ae = a.e, be = b.e, as = a.s, bs = b.s;
+    a.e = c ? be : ae;
+    b.e = c ? ae : be;
+    a.s = c ? bs : as;
+    b.s = c ? as : bs;
+}
+
+// -------------------------------------------------------------- add / sub
+// Exact aligned sum in 2N+2 words, one rounding (contract of core.hpp:116-134).
+template <int N> __device__ __forceinline__ Num<N> add(Num<N> a, Num<N> b)
+{
+    const u32 st = a.st | b.st;
+    if (st) return zero<N>(st);
+    if (!a.s) return b;
+    if (!b.s) return a;
+    swap_if<N>(magnitude_compare<N>(a, b) < 0, a, b);
+    const i64 gap = (i64)a.e - b.e;
+    if (gap > 32 * N + 2) return a; // |b| below a quarter ulp of the representable a
+    constexpr int W = 2 * N + 2;
+    u32 x[W], y[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) x[i] = 0;
+    // a at bit offset 32N+2 (compile time): words N.., shifted by 2 bits
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        x[N + i] |= a.w[i] << 2;
+        x[N + i + 1] |= a.w[i] >> 30;
+    }
+    place<W, N>(y, b.w, 32 * N + 2 - gap); // exact: offset >= 0
+    if (a.s == b.s) {
+        u32 c = 0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            u64 t = (u64)x[i] + y[i] + c;
+            x[i] = (u32)t;
+            c = (u32)(t >> 32);
+        }
+    } else {
+        u32 bw = 0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            u64 t = (u64)x[i] - y[i] - bw;
+            x[i] = (u32)t;
+            bw = (u32)(t >> 63);
+        }
+    }
+    return pack<N, W>(x, (i64)a.e - (32 * N - 1) - (32 * N + 2), a.s, ST_OK);
+}
+
+template <int N> __device__ __forceinline__ Num<N> sub(Num<N> a, Num<N> b) { return add<N>(a, negate<N>(b)); }
+
+// -------------------------------------------------------------------- mul
+// p (2N words) = a * b, schoolbook rows of v_mad_u64_u32 carry chains.
+template <int N> __device__ __forceinline__ void mantissa_product(const u32 (&a)[N], const u32 (&b)[N], u32 (&p)[2 * N])
+{
+#pragma unroll
+    for (int i = 0; i < 2 * N; ++i) p[i] = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u32 carry = 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            u64 v = (u64)a[i] * b[j] + p[i + j] + carry;
+            p[i + j] = (u32)v;
+            carry = (u32)(v >> 32);
+        }
+        p[i + N] = carry;
+    }
+}
+
+// Round a normalized-input product: the msb of p is bit 64N-1 or 64N-2 (inputs have their top bits set).
+template <int N>
+__device__ __forceinline__ Num<N> round_product(const u32 (&p)[2 * N], i64 ea_plus_eb, int sign, u32 st)
+{
+    const bool top = (p[2 * N - 1] >> 31) != 0;
+    i64 msb = ea_plus_eb + (top ? 1 : 0);
+    if (msb > EMAX || msb < -(i64)EMAX - 1) return zero<N>(st | ST_OVERFLOW);
+    Num<N> r;
+    r.s = sign;
+    r.st = st;
+    // mantissa = p[N..2N) shifted left by (top ? 0 : 1)
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u32 hi = p[N + i], lo = p[N + i - 1];
+        r.w[i] = top ? hi : __builtin_amdgcn_alignbit(hi, lo, 31u);
+    }
+    const u32 g = p[N - 1];
+    const u32 round = top ? (g >> 31) : ((g >> 30) & 1u);
+    bool sticky = (top ? (g << 1) : (g << 2)) != 0;
+#pragma unroll
+    for (int i = 0; i < N - 1; ++i) sticky = sticky || p[i] != 0;
+    const bool inc = round && (sticky || (r.w[0] & 1u));
+    u32 c = inc ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u64 t = (u64)r.w[i] + c;
+        r.w[i] = (u32)t;
+        c = (u32)(t >> 32);
+    }
+    if (c) {
+        r.w[N - 1] = 0x80000000u;
+        msb += 1;
+    }
+    if (msb > EMAX || msb < -(i64)EMAX) return zero<N>(st | ST_OVERFLOW);
+    r.e = (int)msb;
+    return r;
+}
+
+// RN(a * b) (contract of core.hpp:137-145)
+template <int N> __device__ __forceinline__ Num<N> mul(const Num<N>& a, const Num<N>& b)
+{
+    const u32 st = a.st | b.st;
+    if (st || !a.s || !b.s) return zero<N>(st);
+    u32 p[2 * N];
+    mantissa_product<N>(a.w, b.w, p);
+    return round_product<N>(p, (i64)a.e + b.e, a.s * b.s, ST_OK);
+}
+
+// -------------------------------------------------------------------- div
+// Two-word by one-word quotient with a reused reciprocal: the 2/1 step of Moller & Granlund,
+// "Improved division by invariant integers" (2011), as in core.hpp:176-184.
+// Preconditions: d >= 2^31, high < d, inv = floor((2^64-1)/d) - 2^32.
+__device__ __forceinline__ void divide_word(u32 high, u32 low, u32 d, u32 inv, u32& q, u32& r)
+{
+    u64 product = (u64)high * inv;
+    u64 sum = (u64)(u32)product + low;
+    u32 qq = (u32)((product >> 32) + high + 1 + (sum >> 32));
+    u32 rr = low - qq * d;
+    if (rr > (u32)sum) {
+        --qq;
+        rr += d;
+    }
+    if (rr >= d) {
+        ++qq;
+        rr -= d;
+    }
+    q = qq;
+    r = rr;
+}
+
+// RN(a / b): normalized base-2^32 long division (Knuth D), then the remainder decides the rounding
+// (contract of core.hpp:185-215).  The partial remainder R is an (N+1)-word shift register: each step
+// brings in the next dividend word, so no array is indexed at run time.
+template <int N> __device__ __forceinline__ Num<N> div(const Num<N>& a, const Num<N>& b)
+{
+    const u32 st = a.st | b.st;
+    if (st) return zero<N>(st);
+    if (!b.s) return zero<N>(ST_DIV0);
+    if (!a.s) return zero<N>(ST_OK);
+    const bool lt = compare_words<N>(a.w, b.w) < 0;
+    i64 e = (i64)a.e - b.e - (lt ? 1 : 0);
+    if (e > EMAX || e < -(i64)EMAX - 1) return zero<N>(ST_OVERFLOW);
+    // dividend U = A * 2^(32N-1+lt): its top N words are A (lt) or A >> 1, then one word A[0] << 31
+    // (only when !lt), then zeros.
+    u32 R[N + 1], Q[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u32 hi = i + 1 < N ? a.w[i + 1] : 0u;
+        R[i] = lt ? a.w[i] : __builtin_amdgcn_alignbit(hi, a.w[i], 1u);
+        Q[i] = 0;
+    }
+    R[N] = 0;
+    u32 next = lt ? 0u : (a.w[0] << 31);
+    const u32 d = b.w[N - 1];
+    const u32 dn = b.w[N - 2];
+    const u32 inv = (u32)(~(u64)0 / d - ((u64)1 << 32));
+#pragma unroll 2
+    for (int j = N - 1; j >= 0; --j) {
+#pragma unroll
+        for (int i = N; i > 0; --i) R[i] = R[i - 1];
+        R[0] = next;
+        next = 0;
+        u64 q, rem;
+        if (R[N] >= d) {
+            q = 0xffffffffull;
+            rem = ((((u64)R[N]) << 32) | R[N - 1]) - q * d;
+        } else {
+            u32 qq, rr;
+            divide_word(R[N], R[N - 1], d, inv, qq, rr);
+            q = qq;
+            rem = rr;
+        }
+        while (rem < ((u64)1 << 32) && q * dn > ((rem << 32) | R[N - 2])) {
+            --q;
+            rem += d;
+        }
+        u64 borrow = 0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            u64 p = q * b.w[i] + borrow;
+            u32 lo = (u32)p, old = R[i];
+            R[i] = old - lo;
+            borrow = (p >> 32) + (old < lo ? 1u : 0u);
+        }
+        const bool under = (u64)R[N] < borrow;
+        R[N] = (u32)((u64)R[N] - borrow);
+        if (under) {
+            --q;
+            u32 c = 0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                u64 t = (u64)R[i] + b.w[i] + c;
+                R[i] = (u32)t;
+                c = (u32)(t >> 32);
+            }
+            R[N] += c;
+        }
+#pragma unroll
+        for (int i = N - 1; i > 0; --i) Q[i] = Q[i - 1];
+        Q[0] = (u32)q;
+    }
+    // round: compare 2R (R < B, N words) with B; ties go to even
+    int half;
+    if (R[N - 1] >> 31) half = 1;
+    else {
+        u32 tw[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) tw[i] = __builtin_amdgcn_alignbit(R[i], i ? R[i - 1] : 0u, 31u);
+        half = compare_words<N>(tw, b.w);
+    }
+    const bool inc = half > 0 || (half == 0 && (Q[0] & 1u));
+    Num<N> r;
+    r.s = a.s * b.s;
+    r.st = ST_OK;
+    u32 c = inc ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u64 t = (u64)Q[i] + c;
+        r.w[i] = (u32)t;
+        c = (u32)(t >> 32);
+    }
+    if (c) {
+        r.w[N - 1] = 0x80000000u;
+        e += 1;
+    }
+    if (e > EMAX || e < -(i64)EMAX) return zero<N>(ST_OVERFLOW);
+    r.e = (int)e;
+    return r;
+}
+
+// ------------------------------------------------------------------- sqrt
+// RN(sqrt(a)): restoring integer square root of Z = mantissa * 2^(32N-1+odd), rounded up exactly when
+// Z - Q^2 > Q (contract of core.hpp:216-235; docs/numerics.md "Square and square root").  Only the
+// top N+1 words of Z can be nonzero.  The radicand is consumed two bits at a time from a shift register.
+template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
+{
+    if (a.st || !a.s) return zero<N>(a.st);
+    if (a.s < 0) return zero<N>(ST_INVALID);
+    int e = a.e / 2;
+    if (a.e < 0 && (a.e % 2)) --e;
+    const int odd = a.e - 2 * e;
+    // Z's top N+1 words (the rest are zero): mantissa << (31 + odd) within N+1 words
+    u32 z[N + 1];
+#pragma unroll
+    for (int i = 0; i <= N; ++i) {
+        u32 hi = i < N ? a.w[i] : 0u, lo = i ? a.w[i - 1] : 0u;
+        // word i of mantissa << (31+odd) within N+1 words
+        z[i] = __builtin_amdgcn_alignbit(hi, lo, odd ? 0u : 1u);
+    }
+    u32 rem[N + 1], root[N];
+#pragma unroll
+    for (int i = 0; i <= N; ++i) rem[i] = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) root[i] = 0;
+    // 32N result bits; bits 2*bit, 2*bit+1 of Z for bit = 32N-1 .. 0. Words of Z from the top:
+    // z[N], z[N-1], ..., z[0], then N-1 zero words.
+    for (int word = 0; word < 2 * N; ++word) {
+        u32 cur = z[N];
+#pragma unroll
+        for (int i = N; i > 0; --i) z[i] = z[i - 1];
+        z[0] = 0;
+        for (int pair = 0; pair < 16; ++pair) {
+            const u32 two = cur >> 30;
+            cur <<= 2;
+            // rem = 4 rem + two
+#pragma unroll
+            for (int i = N; i >= 0; --i) rem[i] = (rem[i] << 2) | (i ? rem[i - 1] >> 30 : two);
+            // trial = 4 root_old + 1 = (root << 2) | 1 with root the value before this bit
+            u32 trial[N + 1];
+#pragma unroll
+            for (int i = N; i >= 0; --i) {
+                u32 hi = i < N ? root[i] : 0u, lo = i ? root[i - 1] : 0u;
+                trial[i] = __builtin_amdgcn_alignbit(hi, lo, 30u);
+            }
+            trial[0] |= 1u;
+            u32 bw = 0;
+            u32 diff[N + 1];
+#pragma unroll
+            for (int i = 0; i <= N; ++i) {
+                u64 t = (u64)rem[i] - trial[i] - bw;
+                diff[i] = (u32)t;
+                bw = (u32)(t >> 63);
+            }
+            const bool ge = bw == 0;
+#pragma unroll
+            for (int i = 0; i <= N; ++i) rem[i] = ge ? diff[i] : rem[i];
+            // root = 2 root + ge
+#pragma unroll
+            for (int i = N - 1; i >= 0; --i) root[i] = (root[i] << 1) | (i ? root[i - 1] >> 31 : (ge ? 1u : 0u));
+        }
+    }
+    // round up iff rem > root
+    bool gt;
+    if (rem[N]) gt = true;
+    else {
+        u32 rl[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) rl[i] = rem[i];
+        gt = compare_words<N>(rl, root) > 0;
+    }
+    Num<N> r;
+    r.s = 1;
+    r.st = ST_OK;
+    u32 c = gt ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u64 t = (u64)root[i] + c;
+        r.w[i] = (u32)t;
+        c = (u32)(t >> 32);
+    }
+    if (c) {
+        r.w[N - 1] = 0x80000000u;
+        e += 1;
+    }
+    r.e = e;
+    return r;
+}
+
+// -------------------------------------------------------------------- fma
+// acc (two's complement) += sign * x
+template <int W> __device__ __forceinline__ void accumulate_signed(u32 (&acc)[W], const u32 (&x)[W], int sign)
+{
+    if (sign > 0) {
+        u32 cy = 0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            u64 t = (u64)acc[i] + x[i] + cy;
+            acc[i] = (u32)t;
+            cy = (u32)(t >> 32);
+        }
+    } else {
+        u32 bw = 0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            u64 t = (u64)acc[i] - x[i] - bw;
+            acc[i] = (u32)t;
+            bw = (u32)(t >> 63);
+        }
+    }
+}
+
+// acc (two's complement) += sign (one unit at bit 0)
+template <int W> __device__ __forceinline__ void accumulate_unit(u32 (&acc)[W], int sign)
+{
+    const u32 add = sign > 0 ? 1u : 0xffffffffu; // +1 or -1 (sign-extended)
+    u32 cy = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        u64 t = (u64)acc[i] + (i ? (sign > 0 ? 0u : 0xffffffffu) : add) + cy;
+        acc[i] = (u32)t;
+        cy = (u32)(t >> 32);
+    }
+}
+
+// RN(a*b + c) with one rounding (the mpfr_fma contract of core.hpp:317-326).  Both terms go into one
+// two's-complement window of 3N+6 words whose top lies two bits above the larger term's msb; bits of
+// the smaller term that fall below the window are collapsed into one sticky unit (the argument of
+// core.hpp:251-253: such a term lies far below the larger one, so no cancellation can expose it).
+template <int N> __device__ __forceinline__ Num<N> fma(const Num<N>& a, const Num<N>& b, const Num<N>& c)
+{
+    const u32 st = a.st | b.st | c.st;
+    if (st) return zero<N>(st);
+    const int ps = a.s * b.s;
+    if (!ps) return c.s ? c : zero<N>(ST_OK); // RN(c) = c
+    u32 p[2 * N];
+    mantissa_product<N>(a.w, b.w, p);
+    if (!c.s) return round_product<N>(p, (i64)a.e + b.e, ps, ST_OK);
+    const bool top = (p[2 * N - 1] >> 31) != 0;
+    const i64 p_msb = (i64)a.e + b.e + (top ? 1 : 0);
+    const i64 p_lsb = (i64)a.e + b.e - 2 * (32 * N - 1);
+    const i64 c_msb = c.e, c_lsb = (i64)c.e - (32 * N - 1);
+    const i64 mx = p_msb > c_msb ? p_msb : c_msb;
+    constexpr int FW = 3 * N + 6;
+    // window bit i has weight 2^(L + i); the larger msb sits at bit 32FW-4, leaving a carry bit
+    // and the sign bit above it, and bit 0 is kept free for the sticky unit.
+    const i64 L = mx - (32 * FW - 4);
+    u32 wp[FW], wc[FW];
+    const bool sp = place<FW, 2 * N>(wp, p, p_lsb - L);
+    const bool sc = place<FW, N>(wc, c.w, c_lsb - L);
+    // acc = ps*P + cs*C in two's complement
+    u32 acc[FW];
+#pragma unroll
+    for (int i = 0; i < FW; ++i) acc[i] = 0;
+    accumulate_signed<FW>(acc, wp, ps);
+    accumulate_signed<FW>(acc, wc, c.s);
+    i64 scale = L;
+    if (sp || sc) {
+        // Only the smaller term can lose bits below the window; the exact value is then
+        // acc * 2^L + t * delta with 0 < delta < 2^L and t that term's sign: append one sticky unit.
+        const int t = sp ? ps : c.s;
+        shl_bits(acc, 1);
+        accumulate_unit<FW>(acc, t);
+        scale = L - 1;
+    }
+    int sign = 1;
+    if (acc[FW - 1] >> 31) {
+        sign = -1;
+        u32 cy = 1;
+#pragma unroll
+        for (int i = 0; i < FW; ++i) {
+            u64 v = (u64)(u32)~acc[i] + cy;
+            acc[i] = (u32)v;
+            cy = (u32)(v >> 32);
+        }
+    }
+    return pack<N, FW>(acc, scale, sign, ST_OK);
+}
+
+template <int N> __device__ __forceinline__ Num<N> fms(const Num<N>& a, const Num<N>& b, const Num<N>& c)
+{
+    return fma<N>(a, b, negate<N>(c));
+}
+
+// ---------------------------------------------------------------- complex
+// Composed complex operations, each real primitive rounded (core.hpp:414-419).
+template <int N> struct Cpx {
+    Num<N> re, im;
+};
+
+template <int N> __device__ __forceinline__ Cpx<N> cadd(const Cpx<N>& a, const Cpx<N>& b)
+{
+    return {add<N>(a.re, b.re), add<N>(a.im, b.im)};
+}
+
+template <int N> __device__ __forceinline__ Cpx<N> cmul(const Cpx<N>& a, const Cpx<N>& b)
+{
+    Num<N> re = sub<N>(mul<N>(a.re, b.re), mul<N>(a.im, b.im));
+    Num<N> im = add<N>(mul<N>(a.re, b.im), mul<N>(a.im, b.re));
+    return {re, im};
+}
+
+template <int N> __device__ __forceinline__ Cpx<N> cdiv(const Cpx<N>& a, const Cpx<N>& b)
+{
+    Num<N> d = add<N>(mul<N>(b.re, b.re), mul<N>(b.im, b.im));
+    Num<N> re = div<N>(add<N>(mul<N>(a.re, b.re), mul<N>(a.im, b.im)), d);
+    Num<N> im = div<N>(sub<N>(mul<N>(a.im, b.re), mul<N>(a.re, b.im)), d);
+    return {re, im};
+}
+
+// ---------------------------------------------------------------- casts
+// Exact widening, RN narrowing from M words to N words (kernels.metal:251-258).
+template <int N, int M> __device__ __forceinline__ Num<N> cast(const Num<M>& x)
+{
+    if (x.st || !x.s) return zero<N>(x.st);
+    u32 t[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) t[i] = x.w[i];
+    // x is normalized: msb = x.e
+    return round_normalized<N, M>(t, (i64)x.e, x.s, ST_OK);
+}
+
+} // namespace lf

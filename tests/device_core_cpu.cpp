@@ -1,0 +1,90 @@
+// Host build of the device arithmetic core (limbforge_amd/csrc/limb_core.hpp) checked against the
+// oracle on the reference tests' case classes, for every width.  Test infrastructure: compiled by
+// tests/test_device_core_cpu.py with tests/cpu_emul on the include path.
+#include "limb_core.hpp"
+#include "lf_oracle.h"
+#include "lfo_cases.h"
+#include <cstdio>
+#include <cstring>
+#include <cstdlib>
+
+static long failures = 0, checks = 0;
+
+template <int N> lf::Num<N> to_dev(const lfo_num& x)
+{
+    lf::Num<N> r;
+    for (int i = 0; i < N; ++i) r.w[i] = x.limb[i];
+    r.e = x.exponent;
+    r.s = x.sign;
+    r.st = x.status;
+    return r;
+}
+
+template <int N> bool same(const lf::Num<N>& d, const lfo_num& o)
+{
+    if (d.s != o.sign || d.e != o.exponent || d.st != o.status) return false;
+    for (int i = 0; i < N; ++i)
+        if (d.w[i] != o.limb[i]) return false;
+    return true;
+}
+
+template <int N> void report(const char* op, int i, const lf::Num<N>& d, const lfo_num& o)
+{
+    ++checks;
+    if (same<N>(d, o)) return;
+    if (++failures <= 30)
+        std::printf("MISMATCH %s bits=%d case=%d dev(s=%d e=%d st=%u top=%08x lo=%08x) oracle(s=%d e=%d st=%u top=%08x lo=%08x)\n", op,
+                    32 * N, i, d.s, d.e, d.st, d.w[N - 1], d.w[0], o.sign, o.exponent, o.status, o.limb[N - 1], o.limb[0]);
+}
+
+template <int N> void run_width(int cases)
+{
+    lfo_mt64 g;
+    lfo_mt_seed(&g, 4242 + N);
+    for (int i = 0; i < cases; ++i) {
+        lfo_num a, b, c, root;
+        lfo_binary_case(&g, N, i, &a, &b);
+        auto da = to_dev<N>(a), db = to_dev<N>(b);
+        report<N>("add", i, lf::add<N>(da, db), lfo_add(N, a, b));
+        report<N>("sub", i, lf::sub<N>(da, db), lfo_sub(N, a, b));
+        report<N>("mul", i, lf::mul<N>(da, db), lfo_mul(N, a, b));
+        report<N>("div", i, lf::div<N>(da, db), lfo_div(N, a, b));
+        root = a;
+        lfo_root_case(&g, N, i, &root);
+        if (i % 5 == 0 && root.sign) root.sign = -root.sign;
+        report<N>("sqrt", i, lf::sqrt<N>(to_dev<N>(root)), lfo_sqrt(N, root));
+        lfo_fma_case(&g, N, i, &a, &b, &c);
+        da = to_dev<N>(a), db = to_dev<N>(b);
+        auto dc = to_dev<N>(c);
+        report<N>("fma", i, lf::fma<N>(da, db, dc), lfo_fma(N, a, b, c));
+        report<N>("fms", i, lf::fms<N>(da, db, dc), lfo_fms(N, a, b, c));
+        if (i % 4 == 0) {
+            lfo_num x[2], y[2], r[2];
+            for (int p = 0; p < 2; ++p) { x[p] = lfo_random_number(&g, N, 1000); y[p] = lfo_random_number(&g, N, 1000); }
+            if (i % 12 == 4) { x[1] = x[0]; x[1].sign = -x[1].sign; y[1] = y[0]; }
+            lf::Cpx<N> cx{to_dev<N>(x[0]), to_dev<N>(x[1])}, cy{to_dev<N>(y[0]), to_dev<N>(y[1])};
+            lfo_cdiv(N, x, y, r);
+            auto q = lf::cdiv<N>(cx, cy);
+            report<N>("cdiv.re", i, q.re, r[0]);
+            report<N>("cdiv.im", i, q.im, r[1]);
+            lfo_cmul(N, x, y, r);
+            auto m = lf::cmul<N>(cx, cy);
+            report<N>("cmul.re", i, m.re, r[0]);
+            report<N>("cmul.im", i, m.im, r[1]);
+        }
+    }
+}
+
+template <int N> void run_all(int lo, int hi, int cases)
+{
+    if (N >= lo && N <= hi) run_width<N>(cases);
+    if constexpr (N < 32) run_all<N + 1>(lo, hi, cases);
+}
+
+int main(int argc, char** argv)
+{
+    int lo = argc > 1 ? std::atoi(argv[1]) : 2, hi = argc > 2 ? std::atoi(argv[2]) : 32, cases = argc > 3 ? std::atoi(argv[3]) : 300;
+    run_all<2>(lo, hi, cases);
+    std::printf("device core (host build): %ld checks, %ld failures\n", checks, failures);
+    return failures ? 1 : 0;
+}

@@ -1,0 +1,44 @@
+"""The C ABI library loads without a GPU and exports every entry point include/*.h declares; argument
+checks that need no device answer with the reference's error classes."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(lf_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported(capi):
+    lib = capi.lib()
+    names = declared_symbols()
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_argument_errors_without_device(capi):
+    a = capi.Operand(None, 0)
+    with pytest.raises(ValueError, match="bits"):
+        capi.check(capi.lib().lf_elementwise(48, 0, ctypes.byref(a), ctypes.byref(a), ctypes.byref(a), ctypes.byref(a), 1, None, None, None))
+    with pytest.raises(ValueError, match="32-bit"):
+        capi.check(capi.lib().lf_elementwise(64, 0, ctypes.byref(a), ctypes.byref(a), ctypes.byref(a), ctypes.byref(a), 1 << 33, None, None, None))
+    with pytest.raises(ValueError, match="operation"):
+        capi.check(capi.lib().lf_elementwise(64, 13, ctypes.byref(a), ctypes.byref(a), ctypes.byref(a), ctypes.byref(a), 1, None, None, None))
+    bad = capi.Broadcast(0, 0)
+    with pytest.raises(ValueError, match="stride"):
+        capi.check(capi.lib().lf_elementwise(64, 0, ctypes.byref(a), ctypes.byref(a), ctypes.byref(a), ctypes.byref(a), 1, ctypes.byref(bad), None, None))
+    # empty batches are a no-op
+    capi.check(capi.lib().lf_elementwise(64, 0, ctypes.byref(a), ctypes.byref(a), ctypes.byref(a), ctypes.byref(a), 0, None, None, None))
+    assert capi.lib().lf_plane_pitch(1) == 64 and capi.lib().lf_plane_pitch(64) == 64 and capi.lib().lf_plane_pitch(65) == 128
+    assert capi.lib().lf_planes_bytes(1024, 1, 64) == 2 * 35 * 64 * 4
