@@ -119,6 +119,16 @@ int lf_mul_chain(int bits, const lf_operand* x, const lf_operand* f, const lf_op
 int lf_run_host(int bits, int op, const void* a, const void* b, const void* c, void* out, uint64_t count,
                 const lf_broadcast* b_index, const lf_broadcast* c_index, double* gpu_seconds, double* wall_seconds);
 
+/* Precision cast between resident operands: CommandBatch::cast (engine.hpp:170-176) and the kernels
+ * cast_real / cast_complex (kernels.metal:251-266): exact widening, RN-even narrowing, componentwise for
+ * complex values; statuses kept, exponent_overflow when rounding leaves the range. */
+int lf_cast(int from_bits, int to_bits, int complex_values, const lf_operand* in, const lf_operand* out,
+            uint64_t count, void* stream);
+
+/* Host-array precision cast: Engine::cast (engine.hpp:206-207). */
+int lf_cast_host(int from_bits, int to_bits, int complex_values, const void* in, void* out, uint64_t count,
+                 double* gpu_seconds, double* wall_seconds);
+
 /* ---- measurement support (no reference counterpart) ---- */
 /* Sustained v_mad_u64_u32 rate of the device (32x32->64 word products per second), measured with
  * `blocks` workgroups of 256 lanes, each lane running `iters` rounds of eight independent chains. */

@@ -51,7 +51,7 @@ def _hip_jobs() -> list[tuple[Path, list[str]]]:
     for n in WIDTHS:
         out = BUILD / f"elementwise_w{n}.o"
         jobs.append((out, [HIPCC, *HIP_FLAGS, f"-DLF_WORDS={n}", "-c", str(CSRC / "elementwise_w.hip"), "-o", str(out)]))
-    for name in ("layout", "capi", "calibrate"):
+    for name in ("layout", "capi", "calibrate", "cast"):
         out = BUILD / f"{name}.o"
         jobs.append((out, [HIPCC, *HIP_FLAGS, "-c", str(CSRC / f"{name}.hip"), "-o", str(out)]))
     for src in sorted(CSRC.glob("gemm*.hip")) + sorted(CSRC.glob("mixed*.hip")):

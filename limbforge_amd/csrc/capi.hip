@@ -11,6 +11,8 @@
 #include <string>
 
 namespace lf {
+hipError_t dispatch_cast(int to_words, const uint32_t* in, uint64_t pin, int M, uint32_t* out, uint64_t pout, uint32_t count,
+                         int parts, hipStream_t s);
 hipError_t launch_aos_to_soa(const uint32_t* aos, uint32_t* soa, uint64_t pitch, uint32_t count, int record, hipStream_t s);
 hipError_t launch_soa_to_aos(const uint32_t* soa, uint32_t* aos, uint64_t pitch, uint32_t count, int record, hipStream_t s);
 } // namespace lf
@@ -383,6 +385,58 @@ int lf_mul_chain(int bits, const lf_operand* x, const lf_operand* f, const lf_op
     g.f_period = f_index ? (uint32_t)f_index->period : 0;
     g.steps = steps;
     LF_HIP(tables().chain[bits / 32](g, (hipStream_t)stream), "mul_chain launch");
+    return LF_OK;
+}
+
+int lf_cast(int from_bits, int to_bits, int complex_values, const lf_operand* in, const lf_operand* out, uint64_t count,
+            void* stream)
+{
+    if (!valid_bits(from_bits) || !valid_bits(to_bits)) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    if (count > std::numeric_limits<uint32_t>::max()) return fail(LF_INVALID_ARGUMENT, "batch exceeds 32-bit indexing");
+    if (!count) return LF_OK;
+    if (int r = check_operand(in, count, "in")) return r;
+    if (int r = check_operand(out, count, "out")) return r;
+    if (in->planes == out->planes && from_bits != to_bits) return fail(LF_INVALID_ARGUMENT, "a cast between widths cannot run in place");
+    LF_HIP(lf::dispatch_cast(to_bits / 32, (const uint32_t*)in->planes, in->pitch, from_bits / 32, (uint32_t*)out->planes, out->pitch,
+                             (uint32_t)count, complex_values ? 2 : 1, (hipStream_t)stream),
+           "cast launch");
+    return LF_OK;
+}
+
+int lf_cast_host(int from_bits, int to_bits, int complex_values, const void* in, void* out, uint64_t count,
+                 double* gpu_seconds, double* wall_seconds)
+{
+    const auto start = std::chrono::steady_clock::now();
+    if (gpu_seconds) *gpu_seconds = 0;
+    if (wall_seconds) *wall_seconds = 0;
+    if (!valid_bits(from_bits) || !valid_bits(to_bits)) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    if (count > std::numeric_limits<uint32_t>::max()) return fail(LF_INVALID_ARGUMENT, "batch exceeds 32-bit indexing");
+    if (!count) return LF_OK;
+    if (!in || !out) return fail(LF_INVALID_ARGUMENT, "null host array");
+    const int parts = complex_values ? 2 : 1;
+    const uint64_t Ein = (uint64_t)(from_bits / 8 + 12) * parts, Eout = (uint64_t)(to_bits / 8 + 12) * parts;
+    Staging& st = staging;
+    if (!st.stream) LF_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& e : st.ev)
+        if (!e) LF_HIP(hipEventCreate(&e), "hipEventCreate");
+    const uint64_t pitch = lf_plane_pitch(count);
+    LF_HIP(st.reserve(0, count * (Ein > Eout ? Ein : Eout)), "hipMalloc (staging)");
+    LF_HIP(st.reserve(1, lf_planes_bytes(from_bits, complex_values, count)), "hipMalloc (staging)");
+    LF_HIP(st.reserve(4, lf_planes_bytes(to_bits, complex_values, count)), "hipMalloc (staging)");
+    hipStream_t s = st.stream;
+    lf_operand I{st.dev[1], pitch}, O{st.dev[4], pitch};
+    LF_HIP(hipMemcpyAsync(st.dev[0], in, count * Ein, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    if (int r = lf_aos_to_planes(from_bits, complex_values, st.dev[0], &I, count, s)) return r;
+    LF_HIP(hipEventRecord(st.ev[0], s), "hipEventRecord");
+    if (int r = lf_cast(from_bits, to_bits, complex_values, &I, &O, count, s)) return r;
+    LF_HIP(hipEventRecord(st.ev[1], s), "hipEventRecord");
+    if (int r = lf_planes_to_aos(to_bits, complex_values, &O, st.dev[0], count, s)) return r;
+    LF_HIP(hipMemcpyAsync(out, st.dev[0], count * Eout, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    LF_HIP(hipStreamSynchronize(s), "device execution");
+    float ms = 0;
+    LF_HIP(hipEventElapsedTime(&ms, st.ev[0], st.ev[1]), "hipEventElapsedTime");
+    if (gpu_seconds) *gpu_seconds = ms * 1e-3;
+    if (wall_seconds) *wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     return LF_OK;
 }
 

@@ -164,6 +164,32 @@ __device__ __forceinline__ Num<N> round_normalized(const u32 (&x)[W], i64 msb, i
     return r;
 }
 
+// Round the N-word mantissa m (top bit set) whose discarded tail has leading bit `round` and any
+// other set bit `sticky` to nearest even; msb is the exponent of m's top bit (same range rule).
+template <int N>
+__device__ __forceinline__ Num<N> round_words(const u32 (&m)[N], u32 round, bool sticky, i64 msb, int sign, u32 st)
+{
+    if (msb > EMAX || msb < -(i64)EMAX - 1) return zero<N>(st | ST_OVERFLOW);
+    Num<N> r;
+    r.s = sign;
+    r.st = st;
+    const bool inc = round && (sticky || (m[0] & 1u));
+    u32 c = inc ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u64 t = (u64)m[i] + c;
+        r.w[i] = (u32)t;
+        c = (u32)(t >> 32);
+    }
+    if (c) {
+        r.w[N - 1] = 0x80000000u;
+        msb += 1;
+    }
+    if (msb > EMAX || msb < -(i64)EMAX) return zero<N>(st | ST_OVERFLOW);
+    r.e = (int)msb;
+    return r;
+}
+
 // RN-even of sign * int(x) * 2^scale (the reference's pack for any word count, core.hpp:81-99).
 template <int N, int W> __device__ __forceinline__ Num<N> pack(u32 (&x)[W], i64 scale, int sign, u32 st)
 {

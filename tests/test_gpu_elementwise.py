@@ -153,3 +153,26 @@ def test_empty_and_errors(capi):
     assert out.shape == (0, 11)
     with pytest.raises(ValueError):
         capi.run_host(250, "mul", np.zeros((1, 11), np.uint32), np.zeros((1, 11), np.uint32))
+
+
+@pytest.mark.parametrize("from_bits,to_bits", [(64, 1024), (1024, 64), (96, 544), (544, 96), (512, 512), (1024, 992), (160, 128)])
+def test_casts(capi, oracle, from_bits, to_bits):
+    """Exact widening, RN narrowing with carry and overflow (kernels.metal:251-266), real and complex."""
+    import ctypes as C
+
+    rng = np.random.default_rng(from_bits * 7 + to_bits)
+    n = 500
+    x = lfcases.random_records(rng, from_bits, n, 1000)
+    w = from_bits // 32
+    x[::7, :w] = 0xFFFFFFFF                 # all-ones mantissas carry into the next binade
+    x[::11, w] = 1000000000                 # ... and overflow at the top exponent
+    x[::13] = lfcases.zero(from_bits, 2)
+    x[::17] = lfcases.zero(from_bits)
+    for cplx in (0, 1):
+        src = np.concatenate([x, x[::-1]], axis=1) if cplx else x
+        out = np.zeros((n, (to_bits // 32 + 3) * (2 if cplx else 1)), dtype=np.uint32)
+        g, wl = C.c_double(), C.c_double()
+        capi.check(capi.lib().lf_cast_host(from_bits, to_bits, cplx, src.ctypes.data, out.ctypes.data, n, C.byref(g), C.byref(wl)))
+        want = np.zeros_like(out)
+        assert oracle.lib().lfo_cast(from_bits, to_bits, cplx, src.ctypes.data, want.ctypes.data, n) == 0
+        assert_same(out, want, f"cast {from_bits}->{to_bits} complex={cplx}")
