@@ -129,6 +129,57 @@ int lf_cast(int from_bits, int to_bits, int complex_values, const lf_operand* in
 int lf_cast_host(int from_bits, int to_bits, int complex_values, const void* in, void* out, uint64_t count,
                  double* gpu_seconds, double* wall_seconds);
 
+/* ---- exact dense products (include/limbforge/linalg.hpp) ---- */
+
+/* LinalgOptions (linalg.hpp:18-25) plus `method`: 0 = exponent-band residue GEMM on int8 MFMA with CRT
+ * rebuild (the reference's algorithm), 1 = the plain exact limb GEMM (every output an exact sum of
+ * exact 2N-word products, one rounding): the same results, used as the baseline. */
+typedef struct lf_linalg_options {
+    int band_bits;     /* G: a band holds exponents [top-G, top] of one line (default 64) */
+    int max_bands;     /* lines needing more bands take the exact fallback kernels (default 4) */
+    int max_spread;    /* ... as do lines whose nonzero exponents span more than this (default 512) */
+    unsigned host_threads; /* band-analysis threads; 0 = hardware concurrency */
+    int force_fallback;    /* every output through the exact-dot kernels (testing) */
+    int resident_bands;    /* kept for LinalgOptions compatibility */
+    int method;
+} lf_linalg_options;
+
+typedef struct lf_linalg_pair {
+    uint32_t left_band, right_band, rows, cols, moduli;
+} lf_linalg_pair;
+
+/* LinalgReport (linalg.hpp:33-45): band histograms per side ([b] = lines with exactly b bands; index 0:
+ * zero or status lines; band_slots = max_bands + 1 entries, at most 65), fallback lines and outputs,
+ * band pairs (first 64), extended GEMM sizes, moduli and int8 GEMMs (three per modulus), stage times. */
+typedef struct lf_linalg_report {
+    uint64_t m, n, k;
+    uint32_t band_slots;
+    uint64_t left_bands[65], right_bands[65];
+    uint64_t left_fallback, right_fallback;
+    uint64_t gpu_outputs, multi_band_outputs, fallback_outputs, trivial_outputs;
+    uint32_t npairs;
+    lf_linalg_pair pairs[64];
+    uint64_t gemm_rows, gemm_cols, int8_gemms, moduli;
+    uint32_t crt_words;
+    double analysis_seconds, upload_seconds, gpu_seconds, assembly_seconds, fallback_seconds;
+} lf_linalg_report;
+
+void lf_linalg_default_options(lf_linalg_options* options);
+
+/* C (m x n) = RN(op(A) B) or, with subtract, RN(C - op(A) B), one rounding per entry: Linalg::gemm
+ * (linalg.hpp:153-155; linalg.mm:300-407).  Host arrays, row-major; op(A) = A (m x k) or A^T (A is k x m).
+ * Statuses: an output whose row of op(A) or column of B carries a status is zero with their OR (and the
+ * old entry's for an update); a zero line gives canonical zero (an update leaves C unchanged).
+ * options and report may be NULL. */
+int lf_gemm_host(int bits, int transpose_a, const void* A, const void* B, uint64_t m, uint64_t n, uint64_t k, void* C,
+                 int subtract, const lf_linalg_options* options, lf_linalg_report* report, double* gpu_seconds,
+                 double* wall_seconds);
+
+/* C (cols x cols) = A^T A for A (rows x cols): Linalg::syrk (linalg.hpp:149-152).  lower_only writes
+ * C[i][j] for i >= j only; otherwise the full symmetric matrix.  subtract requires lower_only. */
+int lf_syrk_host(int bits, const void* A, uint64_t rows, uint64_t cols, void* C, int lower_only, int subtract,
+                 const lf_linalg_options* options, lf_linalg_report* report, double* gpu_seconds, double* wall_seconds);
+
 /* ---- measurement support (no reference counterpart) ---- */
 /* Sustained v_mad_u64_u32 rate of the device (32x32->64 word products per second), measured with
  * `blocks` workgroups of 256 lanes, each lane running `iters` rounds of eight independent chains. */

@@ -48,15 +48,16 @@ def _run(cmd: list[str]) -> tuple[list[str], int, str]:
 
 def _hip_jobs() -> list[tuple[Path, list[str]]]:
     jobs = []
+    # sources compiled once per limb count (LF_WORDS), and once
+    per_width = ("elementwise_w", "gemm_w")
+    once = ("layout", "capi", "calibrate", "cast", "gemm")
     for n in WIDTHS:
-        out = BUILD / f"elementwise_w{n}.o"
-        jobs.append((out, [HIPCC, *HIP_FLAGS, f"-DLF_WORDS={n}", "-c", str(CSRC / "elementwise_w.hip"), "-o", str(out)]))
-    for name in ("layout", "capi", "calibrate", "cast"):
+        for name in per_width:
+            out = BUILD / f"{name}{n}.o"
+            jobs.append((out, [HIPCC, *HIP_FLAGS, f"-DLF_WORDS={n}", "-c", str(CSRC / f"{name}.hip"), "-o", str(out)]))
+    for name in once:
         out = BUILD / f"{name}.o"
         jobs.append((out, [HIPCC, *HIP_FLAGS, "-c", str(CSRC / f"{name}.hip"), "-o", str(out)]))
-    for src in sorted(CSRC.glob("gemm*.hip")) + sorted(CSRC.glob("mixed*.hip")):
-        out = BUILD / f"{src.stem}.o"
-        jobs.append((out, [HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(out)]))
     return jobs
 
 

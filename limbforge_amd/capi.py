@@ -279,3 +279,83 @@ class Event:
                 lib().lf_event_destroy(self.handle)
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ dense products
+class LinalgOptions(C.Structure):
+    """LinalgOptions (linalg.hpp:18-25) plus method: 0 residue GEMM (int8 MFMA + CRT), 1 plain limb GEMM."""
+    _fields_ = [("band_bits", C.c_int), ("max_bands", C.c_int), ("max_spread", C.c_int), ("host_threads", C.c_uint),
+                ("force_fallback", C.c_int), ("resident_bands", C.c_int), ("method", C.c_int)]
+
+    @classmethod
+    def default(cls, **kw):
+        o = cls()
+        lib().lf_linalg_default_options(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
+class LinalgPair(C.Structure):
+    _fields_ = [("left_band", C.c_uint32), ("right_band", C.c_uint32), ("rows", C.c_uint32), ("cols", C.c_uint32),
+                ("moduli", C.c_uint32)]
+
+
+class LinalgReport(C.Structure):
+    _fields_ = [("m", C.c_uint64), ("n", C.c_uint64), ("k", C.c_uint64), ("band_slots", C.c_uint32),
+                ("left_bands", C.c_uint64 * 65), ("right_bands", C.c_uint64 * 65),
+                ("left_fallback", C.c_uint64), ("right_fallback", C.c_uint64),
+                ("gpu_outputs", C.c_uint64), ("multi_band_outputs", C.c_uint64), ("fallback_outputs", C.c_uint64),
+                ("trivial_outputs", C.c_uint64), ("npairs", C.c_uint32), ("pairs", LinalgPair * 64),
+                ("gemm_rows", C.c_uint64), ("gemm_cols", C.c_uint64), ("int8_gemms", C.c_uint64), ("moduli", C.c_uint64),
+                ("crt_words", C.c_uint32), ("analysis_seconds", C.c_double), ("upload_seconds", C.c_double),
+                ("gpu_seconds", C.c_double), ("assembly_seconds", C.c_double), ("fallback_seconds", C.c_double)]
+
+    def bands(self, side="left"):
+        arr = self.left_bands if side == "left" else self.right_bands
+        return [int(arr[i]) for i in range(self.band_slots)]
+
+
+def _linalg_sigs(L):
+    P = C.POINTER
+    L.lf_linalg_default_options.argtypes = [P(LinalgOptions)]
+    L.lf_linalg_default_options.restype = None
+    L.lf_gemm_host.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p,
+                               C.c_int, P(LinalgOptions), P(LinalgReport), P(C.c_double), P(C.c_double)]
+    L.lf_gemm_host.restype = C.c_int
+    L.lf_syrk_host.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_int, C.c_int, P(LinalgOptions),
+                               P(LinalgReport), P(C.c_double), P(C.c_double)]
+    L.lf_syrk_host.restype = C.c_int
+
+
+def gemm(bits, transpose_a, A, B, m, n, k, C_in=None, subtract=False, options=None):
+    """Linalg::gemm (linalg.hpp:153-155) on host arrays: returns (C, report, gpu_seconds, wall_seconds)."""
+    L = lib()
+    if not getattr(L, "_linalg", False):
+        _linalg_sigs(L)
+        L._linalg = True
+    rw = bits // 32 + 3
+    Cm = np.zeros((m * n, rw), np.uint32) if C_in is None else np.array(C_in, dtype=np.uint32).reshape(m * n, rw).copy()
+    A = np.ascontiguousarray(A, dtype=np.uint32)
+    B = np.ascontiguousarray(B, dtype=np.uint32)
+    rep = LinalgReport()
+    g, w = C.c_double(), C.c_double()
+    check(L.lf_gemm_host(bits, int(transpose_a), A.ctypes.data, B.ctypes.data, m, n, k, Cm.ctypes.data, int(subtract),
+                         C.byref(options) if options is not None else None, C.byref(rep), C.byref(g), C.byref(w)))
+    return Cm, rep, g.value, w.value
+
+
+def syrk(bits, A, rows, cols, C_in=None, lower_only=True, subtract=False, options=None):
+    """Linalg::syrk (linalg.hpp:149-152) on host arrays: returns (C, report, gpu_seconds, wall_seconds)."""
+    L = lib()
+    if not getattr(L, "_linalg", False):
+        _linalg_sigs(L)
+        L._linalg = True
+    rw = bits // 32 + 3
+    Cm = np.zeros((cols * cols, rw), np.uint32) if C_in is None else np.array(C_in, dtype=np.uint32).reshape(cols * cols, rw).copy()
+    A = np.ascontiguousarray(A, dtype=np.uint32)
+    rep = LinalgReport()
+    g, w = C.c_double(), C.c_double()
+    check(L.lf_syrk_host(bits, A.ctypes.data, rows, cols, Cm.ctypes.data, int(lower_only), int(subtract),
+                         C.byref(options) if options is not None else None, C.byref(rep), C.byref(g), C.byref(w)))
+    return Cm, rep, g.value, w.value

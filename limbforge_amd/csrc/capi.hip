@@ -19,7 +19,11 @@ hipError_t launch_soa_to_aos(const uint32_t* soa, uint32_t* aos, uint64_t pitch,
 
 namespace {
 thread_local std::string last_error;
+}
 
+void lf_set_last_error(const std::string& message) { last_error = message; }
+
+namespace {
 int fail(int code, const std::string& message)
 {
     last_error = message;

@@ -1,0 +1,910 @@
+// gemm.hip — exact dense products on MI355X: the int8-MFMA residue product kernel and the host planner
+// behind lf_gemm_host / lf_syrk_host / lf_gemm_planes (the reference's Linalg::gemm / syrk and
+// Linalg::Impl::run, src/linalg.mm:295-407).
+//
+// Per call: band analysis of every line on the host (linalg.mm:82-111, same classes and bands), modulus
+// count from the widest band (linalg.mm:341-345), CRT constants in exact integer arithmetic, then on one
+// stream: digits of both extended operands, the residue GEMM per modulus on int8 MFMA, one output-centric
+// CRT rebuild pass that also writes the trivial outputs, and the exact-dot kernels for the outputs of lines
+// outside the residue path.  Nothing is computed on the host besides the plan.
+#include "limbforge_hip.h"
+#include "gemm_kernels.hpp"
+#include "gemm_launch.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace lf {
+
+// ------------------------------------------------------------------ int8 MFMA residue product
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int PT = 64;        // output tile per workgroup (rows and columns)
+constexpr int PK = 32;        // K per MFMA step
+constexpr int LROW = PK + 16; // LDS row stride in bytes (padding against bank conflicts)
+
+__device__ __forceinline__ u32 smod(int g, u32 m, u32 c, u32 p31)
+{
+    // g mod m in [0, m) for a signed 32-bit g: (g + 2^31) mod m - 2^31 mod m
+    u32 r = reduce32((u32)g ^ 0x80000000u, m, c);
+    return r >= p31 ? r - p31 : r + m - p31;
+}
+
+// One 64x64 output tile of one modulus: four waves, each a 32x32 sub-tile with three int32 MFMA
+// accumulators (G11, G00, Gmid).  Each 32-deep K step stages both digit planes of the A rows and the B
+// columns in LDS (every thread moves 16 bytes of each), then issues four v_mfma_i32_32x32x32_i8.
+__global__ void __launch_bounds__(256) product_kernel(ProductArgs g)
+{
+    const u32 q = blockIdx.z;
+    const u32 row0 = blockIdx.y * PT, col0 = blockIdx.x * PT;
+    if (g.lower && col0 > row0 + PT - 1) return; // tile strictly above the diagonal
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][PT * LROW];
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][PT * LROW];
+    const u64 aplane = (u64)g.Mp * g.Kp, bplane = (u64)g.Np * g.Kp;
+    const u32 t = threadIdx.x;
+    const u32 lane = t & 63u, wave = t >> 6;
+    const u32 wr = (wave >> 1) * 32u, wc = (wave & 1u) * 32u;
+    const u32 lp = t >> 7, lr = (t & 127u) >> 1, lk = (t & 1u) * 16u;
+    const int8_t* A = (lp ? g.a0 : g.a1) + q * aplane + (u64)(row0 + lr) * g.Kp + lk;
+    const int8_t* B = (lp ? g.b0 : g.b1) + q * bplane + (u64)(col0 + lr) * g.Kp + lk;
+    v16i c11 = {}, c00 = {}, cmid = {};
+    const u32 r = lane & 31u, kg = (lane >> 5) * 16u;
+    for (u32 k0 = 0; k0 < g.Kp; k0 += PK) {
+        const v4i va = *(const v4i*)(A + k0);
+        const v4i vb = *(const v4i*)(B + k0);
+        __syncthreads();
+        *(v4i*)(&sa[lp][lr * LROW + lk]) = va;
+        *(v4i*)(&sb[lp][lr * LROW + lk]) = vb;
+        __syncthreads();
+        const v4i a1 = *(const v4i*)(&sa[0][(wr + r) * LROW + kg]);
+        const v4i a0 = *(const v4i*)(&sa[1][(wr + r) * LROW + kg]);
+        const v4i b1 = *(const v4i*)(&sb[0][(wc + r) * LROW + kg]);
+        const v4i b0 = *(const v4i*)(&sb[1][(wc + r) * LROW + kg]);
+        c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
+        c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
+        cmid = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, cmid, 0, 0, 0);
+        cmid = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, cmid, 0, 0, 0);
+    }
+    const u32 m = g.moduli[q], cm = g.recip[q];
+    const u32 p31 = reduce32(0x80000000u, m, cm);
+    const u32 p16 = reduce32(65536u, m, cm);
+    uint16_t* out = g.res + (u64)q * g.Mp * g.Np;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        // 32x32 accumulator layout: lane l, element i -> row 8(i/4) + 4(l/32) + i%4, column l%32
+        const u32 rr = row0 + wr + 8u * (u32)(i / 4) + 4u * (lane >> 5) + (u32)(i % 4);
+        const u32 cc = col0 + wc + (lane & 31u);
+        const u32 r11 = smod(c11[i], m, cm, p31), r00 = smod(c00[i], m, cm, p31), rm = smod(cmid[i], m, cm, p31);
+        u32 v = reduce32(r11 * p16, m, cm) + reduce32(rm << 8, m, cm);
+        v = (v >= m ? v - m : v) + r00;
+        out[(u64)rr * g.Np + cc] = (uint16_t)(v >= m ? v - m : v);
+    }
+}
+
+hipError_t launch_product(const ProductArgs& g, hipStream_t s)
+{
+    if (!g.L || !g.Mp || !g.Np) return hipSuccess;
+    hipLaunchKernelGGL(product_kernel, dim3(g.Np / PT, g.Mp / PT, g.L), dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+} // namespace lf
+
+// the C ABI's per-thread error message (capi.hip)
+void lf_set_last_error(const std::string& message);
+
+namespace {
+using lf::GemmLine;
+using lf::GEMM_MAX_BANDS;
+typedef unsigned __int128 u128;
+
+constexpr std::size_t MAX_K = 65472; // 2 Kp 2^14 < 2^31 keeps every int8 product sum exact in int32
+
+int fail(int code, const std::string& m)
+{
+    lf_set_last_error(m);
+    return code;
+}
+
+// ---- exact multiword helpers (host) ----
+struct Big {
+    std::vector<uint32_t> w; // little endian
+    void trim()
+    {
+        while (!w.empty() && !w.back()) w.pop_back();
+    }
+    int bits() const
+    {
+        for (int i = int(w.size()) - 1; i >= 0; --i)
+            if (w[i]) return 32 * i + 32 - __builtin_clz(w[i]);
+        return 0;
+    }
+};
+Big big_from(uint64_t v)
+{
+    Big b;
+    b.w = {uint32_t(v), uint32_t(v >> 32)};
+    b.trim();
+    return b;
+}
+void big_mul_small(Big& b, uint32_t m)
+{
+    uint64_t c = 0;
+    for (auto& x : b.w) {
+        uint64_t t = uint64_t(x) * m + c;
+        x = uint32_t(t);
+        c = t >> 32;
+    }
+    if (c) b.w.push_back(uint32_t(c));
+}
+// b / m exactly (m divides b)
+Big big_div_small(const Big& b, uint32_t m)
+{
+    Big q;
+    q.w.assign(b.w.size(), 0);
+    uint64_t r = 0;
+    for (int i = int(b.w.size()) - 1; i >= 0; --i) {
+        uint64_t cur = (r << 32) | b.w[i];
+        q.w[i] = uint32_t(cur / m);
+        r = cur % m;
+    }
+    q.trim();
+    return q;
+}
+uint32_t big_mod_small(const Big& b, uint32_t m)
+{
+    uint64_t r = 0;
+    for (int i = int(b.w.size()) - 1; i >= 0; --i) r = ((r << 32) | b.w[i]) % m;
+    return uint32_t(r);
+}
+// b << s into exactly `words` words (must fit)
+std::vector<uint32_t> big_shifted(const Big& b, int s, int words)
+{
+    std::vector<uint32_t> out(words, 0);
+    const int ws = s / 32, bs = s % 32;
+    for (std::size_t i = 0; i < b.w.size(); ++i) {
+        uint64_t v = uint64_t(b.w[i]) << bs;
+        if (i + ws < std::size_t(words)) out[i + ws] |= uint32_t(v);
+        if (i + ws + 1 < std::size_t(words)) out[i + ws + 1] |= uint32_t(v >> 32);
+    }
+    return out;
+}
+uint32_t inverse_mod(uint64_t a, uint64_t m) // a^-1 mod m (gcd 1)
+{
+    int64_t t = 0, nt = 1, r = int64_t(m), nr = int64_t(a % m);
+    while (nr) {
+        int64_t q = r / nr, x = t - q * nt;
+        t = nt;
+        nt = x;
+        x = r - q * nr;
+        r = nr;
+        nr = x;
+    }
+    return uint32_t(t < 0 ? t + int64_t(m) : t);
+}
+
+// Primes <= 65279, descending: balanced residues then split into two int8 digits (linalg.mm:38-40).
+const std::vector<uint32_t>& primes()
+{
+    static const std::vector<uint32_t> p = [] {
+        std::vector<uint32_t> r;
+        for (uint32_t q = 65279; r.size() < 1200; --q) {
+            bool prime = true;
+            for (uint32_t d = 2; d * d <= q; ++d)
+                if (q % d == 0) {
+                    prime = false;
+                    break;
+                }
+            if (prime) r.push_back(q);
+        }
+        return r;
+    }();
+    return p;
+}
+
+// ---- field access for the band analysis ----
+struct Fields {
+    // element (line, k) is at index line*ls + k*ks; fields at word offsets in records of `stride` words
+    const uint32_t* exponent;
+    const uint32_t* sign;
+    const uint32_t* status;
+    uint64_t stride, ls, ks;
+    void get(std::size_t line, std::size_t k, int& e, int& s, uint32_t& st) const
+    {
+        const uint64_t i = (uint64_t(line) * ls + uint64_t(k) * ks) * stride;
+        e = int(exponent[i]);
+        s = int(sign[i]);
+        st = status[i];
+    }
+};
+
+struct Side {
+    std::vector<GemmLine> lines;
+    std::vector<std::vector<std::pair<long, long>>> bands;
+    std::vector<uint64_t> histogram;
+    uint64_t fallback = 0;
+    std::vector<std::vector<int>> band_members; // band b -> lines having a band b (in line order)
+};
+
+template <class F> void parallel_for(unsigned threads, std::size_t n, F f)
+{
+    threads = unsigned(std::min<std::size_t>(std::max(1u, threads), n));
+    if (threads <= 1) {
+        if (n) f(std::size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> w;
+    for (unsigned t = 0; t < threads; ++t) w.emplace_back(f, n * t / threads, n * (t + 1) / threads);
+    for (auto& x : w) x.join();
+}
+
+// linalg.mm:82-111: status OR, exponent range, greedy bands [hi-G, hi] from the largest exponent down
+// (lo: the smallest exponent inside), classes 0 bands / 1 trivial / 2 fallback.
+Side analyze(const Fields& f, std::size_t lines, std::size_t K, const lf_linalg_options& o, bool force, unsigned threads)
+{
+    Side s;
+    s.lines.assign(lines, GemmLine{});
+    s.bands.assign(lines, {});
+    const long G = o.band_bits;
+    parallel_for(threads, lines, [&](std::size_t first, std::size_t last) {
+        for (std::size_t i = first; i < last; ++i) {
+            GemmLine& L = s.lines[i];
+            L.cls = 1;
+            long top = LONG_MIN, bottom = LONG_MAX;
+            uint32_t status = 0;
+            int e, sg;
+            uint32_t st;
+            for (std::size_t k = 0; k < K; ++k) {
+                f.get(i, k, e, sg, st);
+                status |= st;
+                if (!st && sg) {
+                    top = std::max(top, long(e));
+                    bottom = std::min(bottom, long(e));
+                }
+            }
+            L.status = status;
+            L.top = top == LONG_MIN ? 0 : int(top);
+            L.bottom = bottom == LONG_MAX ? 0 : int(bottom);
+            if (status || top == LONG_MIN) continue;
+            if (force) {
+                L.cls = 2;
+                continue;
+            }
+            auto& b = s.bands[i];
+            if (o.max_bands >= 1 && top - bottom <= G) {
+                b.push_back({bottom, top});
+                L.cls = 0;
+                continue;
+            }
+            long hi = top;
+            bool fallback = top - bottom > o.max_spread;
+            while (!fallback) {
+                long lo = hi, next = LONG_MIN;
+                for (std::size_t k = 0; k < K; ++k) {
+                    f.get(i, k, e, sg, st);
+                    if (st || !sg || e > hi) continue;
+                    if (e >= hi - G) lo = std::min(lo, long(e));
+                    else next = std::max(next, long(e));
+                }
+                b.push_back({lo, hi});
+                if (int(b.size()) > o.max_bands) fallback = true;
+                if (next == LONG_MIN) break;
+                hi = next;
+            }
+            L.cls = fallback ? 2 : 0;
+        }
+    });
+    s.histogram.assign(std::size_t(std::max(o.max_bands, 0)) + 1, 0);
+    for (std::size_t i = 0; i < lines; ++i) {
+        GemmLine& L = s.lines[i];
+        if (L.cls == 1) {
+            ++s.histogram[0];
+            continue;
+        }
+        if (L.cls == 2) {
+            ++s.fallback;
+            s.bands[i].clear();
+            continue;
+        }
+        const int nb = int(s.bands[i].size());
+        ++s.histogram[nb];
+        L.nb = nb;
+        L.low = int(s.bands[i].back().first);
+        for (int b = 0; b < nb && b < GEMM_MAX_BANDS; ++b) {
+            L.lo[b] = int(s.bands[i][b].first);
+            L.hi[b] = int(s.bands[i][b].second);
+        }
+    }
+    return s;
+}
+
+// GPU-path capacity: lines whose bands this build's rebuild kernels cannot hold take the exact-dot path
+// (same results; the report counts them as fallback lines).
+void route_capacity(Side& s, int wx_class)
+{
+    for (std::size_t i = 0; i < s.lines.size(); ++i) {
+        GemmLine& L = s.lines[i];
+        if (L.cls != 0) continue;
+        const bool too_many = L.nb > GEMM_MAX_BANDS;
+        const bool too_wide = L.nb > 1 && (wx_class != 0 || long(L.hi[0]) - L.low > 512);
+        if (too_many || too_wide) {
+            L.cls = 2;
+            ++s.fallback;
+            --s.histogram[L.nb];
+            s.bands[i].clear();
+        }
+    }
+}
+
+int ceil_log2(std::size_t x)
+{
+    int l = 0;
+    while ((std::size_t(1) << l) < x) ++l;
+    return l;
+}
+
+struct DeviceBuf {
+    void* p = nullptr;
+    ~DeviceBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(std::size_t bytes)
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        return hipMalloc(&p, std::max<std::size_t>(bytes, 16));
+    }
+};
+
+#define GEMM_HIP(call, where)                                                                                          \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) return fail(LF_RUNTIME_ERROR, std::string(where) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+typedef hipError_t (*digits_fn)(const lf::DigitsArgs&, hipStream_t);
+typedef hipError_t (*rebuild_fn)(int, const lf::RebuildArgs&, hipStream_t);
+typedef hipError_t (*dot_fn)(int, const lf::DotArgs&, hipStream_t);
+template <int N> void fill_gemm(digits_fn* d, rebuild_fn* r, dot_fn* x)
+{
+    d[N] = lf::launch_digits<N>;
+    r[N] = lf::launch_rebuild<N>;
+    x[N] = lf::launch_dot<N>;
+    if constexpr (N < 32) fill_gemm<N + 1>(d, r, x);
+}
+struct GemmTables {
+    digits_fn digits[33] = {};
+    rebuild_fn rebuild[33] = {};
+    dot_fn dot[33] = {};
+    GemmTables() { fill_gemm<2>(digits, rebuild, dot); }
+};
+const GemmTables& gemm_tables()
+{
+    static const GemmTables t;
+    return t;
+}
+
+// One exact product call on device operands.  left/right: views of op(A) rows and B columns on the
+// device; lf/rf: host field access for the analysis; C: device output planes (m x n, row-major).
+struct Product {
+    int bits;
+    bool syrk, lower_only, subtract;
+    lf::MatView left, right;
+    Fields lf_, rf_;
+    std::size_t m, n, K;
+    lf_operand C;
+    lf_linalg_options opt;
+    hipStream_t stream;
+};
+
+int run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
+{
+    using Clock = std::chrono::steady_clock;
+    const auto since = [](Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); };
+    const int N = P.bits / 32;
+    const unsigned threads = P.opt.host_threads ? P.opt.host_threads : std::max(1u, std::thread::hardware_concurrency());
+    if (rep) {
+        std::memset(rep, 0, sizeof *rep);
+        rep->m = P.m;
+        rep->n = P.n;
+        rep->k = P.K;
+    }
+    auto t0 = Clock::now();
+    const bool force = P.opt.force_fallback || P.K > MAX_K || P.opt.method == 1;
+    Side ls = analyze(P.lf_, P.m, P.K, P.opt, force, threads);
+    Side rs_own;
+    if (!P.syrk) rs_own = analyze(P.rf_, P.n, P.K, P.opt, force, threads);
+    Side& rs = P.syrk ? ls : rs_own;
+    // modulus count from the widest band of both sides (linalg.mm:341-345)
+    int pmax = 0;
+    for (const Side* s : {&ls, &rs})
+        for (std::size_t i = 0; i < s->lines.size(); ++i)
+            if (s->lines[i].cls == 0)
+                for (int b = 0; b < s->lines[i].nb && b < GEMM_MAX_BANDS; ++b) pmax = std::max(pmax, s->lines[i].hi[b] - s->lines[i].lo[b]);
+    const int logk = ceil_log2(P.K);
+    const double need = 2.0 * P.bits + 2 * pmax + logk + 2;
+    double have = 0;
+    unsigned L = 0;
+    while (have <= need) have += std::log2(double(primes()[L++]));
+    // exact check of the bound M > 2^need (the double sum above only mirrors the reference's count)
+    Big M = big_from(1);
+    for (unsigned q = 0; q < L; ++q) big_mul_small(M, primes()[q]);
+    while (M.bits() <= int(need) + 1) {
+        big_mul_small(M, primes()[L]);
+        ++L;
+    }
+    const int mbits = M.bits();
+    int wx_class = mbits + 1 <= 32 * (lf::rebuild_words(N, 0) - 1) ? 0 : mbits + 1 <= 32 * (lf::rebuild_words(N, 1) - 1) ? 1 : 2;
+    if (wx_class == 2) { // beyond the compiled CRT classes: every band line takes the exact-dot path
+        for (Side* s : {&ls, &rs})
+            for (auto& Ln : s->lines)
+                if (Ln.cls == 0) {
+                    --s->histogram[Ln.nb];
+                    Ln.cls = 2;
+                    ++s->fallback;
+                }
+        wx_class = 0;
+    }
+    route_capacity(ls, wx_class);
+    if (!P.syrk) route_capacity(rs, wx_class);
+    // extended operands: band-major member lists (linalg.mm:326-330)
+    std::size_t Me = 0, Ne = 0;
+    auto assign_members = [](Side& s) {
+        std::size_t next = 0;
+        for (int b = 0; b < GEMM_MAX_BANDS; ++b)
+            for (auto& Ln : s.lines)
+                if (Ln.cls == 0 && Ln.nb > b) Ln.member[b] = int(next++);
+        return next;
+    };
+    Me = assign_members(ls);
+    Ne = P.syrk ? Me : assign_members(rs);
+    bool any_pair = false;
+    for (auto& a : ls.lines)
+        if (a.cls == 0) {
+            for (auto& b : rs.lines)
+                if (b.cls == 0) {
+                    any_pair = true;
+                    break;
+                }
+            break;
+        }
+    if (rep) {
+        rep->band_slots = uint32_t(std::min<std::size_t>(ls.histogram.size(), 65));
+        for (std::size_t b = 0; b < ls.histogram.size() && b < 65; ++b) rep->left_bands[b] = ls.histogram[b];
+        for (std::size_t b = 0; b < rs.histogram.size() && b < 65; ++b) rep->right_bands[b] = rs.histogram[b];
+        rep->left_fallback = ls.fallback;
+        rep->right_fallback = rs.fallback;
+        rep->analysis_seconds = since(t0);
+    }
+    t0 = Clock::now();
+    // line tables
+    DeviceBuf dl, dr;
+    GEMM_HIP(dl.alloc(sizeof(GemmLine) * std::max<std::size_t>(P.m, 1)), "hipMalloc (lines)");
+    GEMM_HIP(hipMemcpyAsync(dl.p, ls.lines.data(), sizeof(GemmLine) * P.m, hipMemcpyHostToDevice, P.stream), "upload lines");
+    const GemmLine* dright = (const GemmLine*)dl.p;
+    if (!P.syrk) {
+        GEMM_HIP(dr.alloc(sizeof(GemmLine) * std::max<std::size_t>(P.n, 1)), "hipMalloc (lines)");
+        GEMM_HIP(hipMemcpyAsync(dr.p, rs.lines.data(), sizeof(GemmLine) * P.n, hipMemcpyHostToDevice, P.stream), "upload lines");
+        dright = (const GemmLine*)dr.p;
+    }
+    // outputs for the exact-dot kernels, by window class
+    std::vector<uint32_t> lists[3];
+    uint64_t trivial = 0, gpu_out = 0, multi = 0;
+    for (std::size_t i = 0; i < P.m; ++i)
+        for (std::size_t j = 0; j < (P.syrk ? i + 1 : P.n); ++j) {
+            const GemmLine &a = ls.lines[i], &b = rs.lines[j];
+            if (a.cls == 1 || b.cls == 1) {
+                ++trivial;
+                continue;
+            }
+            if (a.cls == 0 && b.cls == 0) {
+                ++gpu_out;
+                multi += a.nb > 1 || b.nb > 1;
+                continue;
+            }
+            const long span = (long(a.top) + b.top) - (long(a.bottom) + b.bottom) + 64L * N + logk + 3;
+            const int cls = span <= 32L * lf::dot_window_words(0) ? 0 : span <= 32L * lf::dot_window_words(1) ? 1 : 2;
+            lists[cls].push_back(uint32_t(i));
+            lists[cls].push_back(uint32_t(j));
+        }
+    const uint64_t fallback_outputs = (lists[0].size() + lists[1].size() + lists[2].size()) / 2;
+    // residue path
+    const std::size_t Kp = std::max<std::size_t>((std::max<std::size_t>(P.K, 1) + 63) / 64 * 64, 64);
+    const std::size_t Mp = (Me + 63) / 64 * 64, Np = (Ne + 63) / 64 * 64;
+    DeviceBuf dconst, dig_l1, dig_l0, dig_r1, dig_r0, dres, dlist[3], derr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    GEMM_HIP(hipEventCreate(&e0), "hipEventCreate");
+    GEMM_HIP(hipEventCreate(&e1), "hipEventCreate");
+    struct EventGuard {
+        hipEvent_t a, b;
+        ~EventGuard()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{e0, e1};
+    const int WX = lf::rebuild_words(N, wx_class);
+    const int G = P.opt.band_bits;
+    // constants: moduli | recip | chunk_pow[L][64] | shift_pow[L][G+1] | cinv | crt[L][WX] | bound[2][WX]
+    std::vector<uint32_t> cst;
+    std::size_t o_mod = 0, o_rec = 0, o_chunk = 0, o_shift = 0, o_cinv = 0, o_crt = 0, o_bound = 0;
+    int tshift = 0;
+    if (any_pair) {
+        o_mod = cst.size();
+        for (unsigned q = 0; q < L; ++q) cst.push_back(primes()[q]);
+        o_rec = cst.size();
+        for (unsigned q = 0; q < L; ++q) cst.push_back(uint32_t((uint64_t(1) << 32) / primes()[q]));
+        o_chunk = cst.size();
+        for (unsigned q = 0; q < L; ++q) {
+            const uint32_t m = primes()[q];
+            uint64_t p = 1;
+            for (int h = 0; h < 64; ++h) {
+                cst.push_back(uint32_t(p));
+                p = (p << 16) % m;
+            }
+        }
+        o_shift = cst.size();
+        for (unsigned q = 0; q < L; ++q) {
+            const uint32_t m = primes()[q];
+            uint64_t p = 1;
+            for (int s = 0; s <= G; ++s) {
+                cst.push_back(uint32_t(p));
+                p = (p << 1) % m;
+            }
+        }
+        o_cinv = cst.size();
+        tshift = 32 * (WX - 1) - mbits;
+        std::vector<std::vector<uint32_t>> crt(L);
+        for (unsigned q = 0; q < L; ++q) {
+            const uint32_t m = primes()[q];
+            Big Mq = big_div_small(M, m);
+            cst.push_back(inverse_mod(big_mod_small(Mq, m), m));
+            crt[q] = big_shifted(Mq, tshift, WX);
+        }
+        o_crt = cst.size();
+        for (unsigned q = 0; q < L; ++q) cst.insert(cst.end(), crt[q].begin(), crt[q].end());
+        o_bound = cst.size();
+        std::vector<uint32_t> Ms = big_shifted(M, tshift, WX), H(WX, 0);
+        for (int w = 0; w < WX; ++w) H[w] = (Ms[w] >> 1) | (w + 1 < WX ? Ms[w + 1] << 31 : 0u);
+        cst.insert(cst.end(), Ms.begin(), Ms.end());
+        cst.insert(cst.end(), H.begin(), H.end());
+        GEMM_HIP(dconst.alloc(cst.size() * 4), "hipMalloc (constants)");
+        GEMM_HIP(hipMemcpyAsync(dconst.p, cst.data(), cst.size() * 4, hipMemcpyHostToDevice, P.stream), "upload constants");
+        const std::size_t lplane = Mp * Kp, rplane = Np * Kp;
+        GEMM_HIP(dig_l1.alloc(L * lplane), "hipMalloc (digits)");
+        GEMM_HIP(dig_l0.alloc(L * lplane), "hipMalloc (digits)");
+        if (!P.syrk) {
+            GEMM_HIP(dig_r1.alloc(L * rplane), "hipMalloc (digits)");
+            GEMM_HIP(dig_r0.alloc(L * rplane), "hipMalloc (digits)");
+        }
+        GEMM_HIP(dres.alloc(std::size_t(L) * Mp * Np * 2), "hipMalloc (residues)");
+    }
+    for (int c = 0; c < 3; ++c)
+        if (!lists[c].empty()) {
+            GEMM_HIP(dlist[c].alloc(lists[c].size() * 4), "hipMalloc (outputs)");
+            GEMM_HIP(hipMemcpyAsync(dlist[c].p, lists[c].data(), lists[c].size() * 4, hipMemcpyHostToDevice, P.stream), "upload outputs");
+        }
+    GEMM_HIP(derr.alloc(4), "hipMalloc");
+    GEMM_HIP(hipMemsetAsync(derr.p, 0, 4, P.stream), "hipMemsetAsync");
+    if (rep) rep->upload_seconds = since(t0);
+    t0 = Clock::now();
+    GEMM_HIP(hipEventRecord(e0, P.stream), "hipEventRecord");
+    const uint32_t* C32 = (const uint32_t*)dconst.p;
+    if (any_pair) {
+        const std::size_t lplane = Mp * Kp, rplane = Np * Kp;
+        GEMM_HIP(hipMemsetAsync(dig_l1.p, 0, L * lplane, P.stream), "hipMemsetAsync");
+        GEMM_HIP(hipMemsetAsync(dig_l0.p, 0, L * lplane, P.stream), "hipMemsetAsync");
+        if (!P.syrk) {
+            GEMM_HIP(hipMemsetAsync(dig_r1.p, 0, L * rplane, P.stream), "hipMemsetAsync");
+            GEMM_HIP(hipMemsetAsync(dig_r0.p, 0, L * rplane, P.stream), "hipMemsetAsync");
+        }
+        lf::DigitsArgs d{};
+        d.L = L;
+        d.Kp = uint32_t(Kp);
+        d.K = uint32_t(P.K);
+        d.moduli = C32 + o_mod;
+        d.recip = C32 + o_rec;
+        d.chunk_pow = C32 + o_chunk;
+        d.shift_pow = C32 + o_shift;
+        d.shift_max = uint32_t(G);
+        d.v = P.left;
+        d.lines = (const GemmLine*)dl.p;
+        d.nlines = uint32_t(P.m);
+        d.rows_p = uint32_t(Mp);
+        d.d1 = (int8_t*)dig_l1.p;
+        d.d0 = (int8_t*)dig_l0.p;
+        GEMM_HIP(gemm_tables().digits[N](d, P.stream), "digits launch");
+        if (!P.syrk) {
+            d.v = P.right;
+            d.lines = dright;
+            d.nlines = uint32_t(P.n);
+            d.rows_p = uint32_t(Np);
+            d.d1 = (int8_t*)dig_r1.p;
+            d.d0 = (int8_t*)dig_r0.p;
+            GEMM_HIP(gemm_tables().digits[N](d, P.stream), "digits launch");
+        }
+        lf::ProductArgs pa{};
+        pa.a1 = (const int8_t*)dig_l1.p;
+        pa.a0 = (const int8_t*)dig_l0.p;
+        pa.b1 = (const int8_t*)(P.syrk ? dig_l1.p : dig_r1.p);
+        pa.b0 = (const int8_t*)(P.syrk ? dig_l0.p : dig_r0.p);
+        pa.res = (uint16_t*)dres.p;
+        pa.Mp = uint32_t(Mp);
+        pa.Np = uint32_t(Np);
+        pa.Kp = uint32_t(Kp);
+        pa.L = L;
+        pa.moduli = C32 + o_mod;
+        pa.recip = C32 + o_rec;
+        pa.lower = P.syrk;
+        GEMM_HIP(lf::launch_product(pa, P.stream), "product launch");
+    }
+    {
+        lf::RebuildArgs r{};
+        r.res = (const uint16_t*)dres.p;
+        r.Mp = uint32_t(Mp);
+        r.Np = uint32_t(Np);
+        r.L = any_pair ? L : 0;
+        r.moduli = C32 + o_mod;
+        r.recip = C32 + o_rec;
+        r.cinv = C32 + o_cinv;
+        r.crt = C32 + o_crt;
+        r.bound = C32 + o_bound;
+        r.t = tshift;
+        r.left = (const GemmLine*)dl.p;
+        r.right = dright;
+        r.m = uint32_t(P.m);
+        r.n = uint32_t(P.n);
+        r.out = (uint32_t*)P.C.planes;
+        r.out_pitch = P.C.pitch;
+        r.out_stride = P.n;
+        r.syrk = P.syrk;
+        r.lower_only = P.lower_only;
+        r.subtract = P.subtract;
+        GEMM_HIP(gemm_tables().rebuild[N](wx_class, r, P.stream), "rebuild launch");
+    }
+    for (int c = 0; c < 3; ++c) {
+        if (lists[c].empty()) continue;
+        lf::DotArgs x{};
+        x.left = P.left;
+        x.right = P.right;
+        x.lrows = (const GemmLine*)dl.p;
+        x.rcols = dright;
+        x.list = (const uint32_t*)dlist[c].p;
+        x.count = uint32_t(lists[c].size() / 2);
+        x.m = uint32_t(P.m);
+        x.n = uint32_t(P.n);
+        x.K = uint32_t(P.K);
+        x.logk = uint32_t(ceil_log2(P.K + 1));
+        x.out = (uint32_t*)P.C.planes;
+        x.out_pitch = P.C.pitch;
+        x.out_stride = P.n;
+        x.syrk = P.syrk;
+        x.lower_only = P.lower_only;
+        x.subtract = P.subtract;
+        x.mirror = P.syrk && !P.lower_only;
+        x.error = (int32_t*)derr.p;
+        GEMM_HIP(gemm_tables().dot[N](c, x, P.stream), "exact dot launch");
+    }
+    GEMM_HIP(hipEventRecord(e1, P.stream), "hipEventRecord");
+    int32_t err = 0;
+    GEMM_HIP(hipMemcpyAsync(&err, derr.p, 4, hipMemcpyDeviceToHost, P.stream), "download status");
+    GEMM_HIP(hipStreamSynchronize(P.stream), "device execution (linalg)");
+    float ms = 0;
+    GEMM_HIP(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    if (device_ms) *device_ms = ms;
+    if (err) return fail(LF_RUNTIME_ERROR, "exact dot: an exponent cluster exceeds the GPU window (4096 bits)");
+    if (rep) {
+        rep->gpu_seconds = ms * 1e-3;
+        rep->gemm_rows = Me;
+        rep->gemm_cols = Ne;
+        rep->moduli = any_pair ? L : 0;
+        rep->int8_gemms = any_pair ? 3ull * L : 0;
+        rep->gpu_outputs = gpu_out;
+        rep->multi_band_outputs = multi;
+        rep->fallback_outputs = fallback_outputs;
+        rep->trivial_outputs = trivial;
+        rep->crt_words = any_pair ? uint32_t(WX) : 0;
+        uint32_t np = 0;
+        if (any_pair) {
+            int lb = 0, rb = 0;
+            for (auto& Ln : ls.lines) lb = std::max(lb, Ln.cls == 0 ? Ln.nb : 0);
+            for (auto& Ln : rs.lines) rb = std::max(rb, Ln.cls == 0 ? Ln.nb : 0);
+            for (int b = 0; b < lb; ++b)
+                for (int c = 0; c < (P.syrk ? b + 1 : rb); ++c) {
+                    if (np >= 64) break;
+                    uint32_t rows = 0, cols = 0;
+                    for (auto& Ln : ls.lines) rows += Ln.cls == 0 && Ln.nb > b;
+                    for (auto& Ln : rs.lines) cols += Ln.cls == 0 && Ln.nb > c;
+                    rep->pairs[np++] = {uint32_t(b), uint32_t(c), rows, cols, L};
+                }
+        }
+        rep->npairs = np;
+        rep->assembly_seconds = since(t0);
+    }
+    return LF_OK;
+}
+
+int check_options(const lf_linalg_options* o)
+{
+    if (o->band_bits < 0 || o->band_bits > 1024 || o->max_bands < 0 || o->max_spread < 0 || o->max_spread > 8192)
+        return fail(LF_INVALID_ARGUMENT, "linalg options: band_bits in [0,1024], max_bands >= 0, max_spread in [0,8192]");
+    if (o->method < 0 || o->method > 1) return fail(LF_INVALID_ARGUMENT, "linalg options: method 0 (residue GEMM) or 1 (limb GEMM)");
+    return LF_OK;
+}
+
+struct HostStage {
+    DeviceBuf aos, pa, pb, pc;
+    hipStream_t stream = nullptr;
+    ~HostStage()
+    {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+} // namespace
+
+extern "C" {
+
+void lf_linalg_default_options(lf_linalg_options* o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->band_bits = 64;
+    o->max_bands = 4;
+    o->max_spread = 512;
+    o->resident_bands = 2;
+}
+
+int lf_gemm_host(int bits, int transpose_a, const void* A, const void* B, uint64_t m, uint64_t n, uint64_t k, void* C, int subtract,
+                 const lf_linalg_options* options, lf_linalg_report* report, double* gpu_seconds, double* wall_seconds)
+{
+    const auto start = std::chrono::steady_clock::now();
+    if (gpu_seconds) *gpu_seconds = 0;
+    if (wall_seconds) *wall_seconds = 0;
+    lf_linalg_options o;
+    if (options) o = *options;
+    else lf_linalg_default_options(&o);
+    if (int r = check_options(&o)) return r;
+    if (bits < 64 || bits > 1024 || bits % 32) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    const uint64_t limit = uint64_t(1) << 31;
+    if (m >= limit || n >= limit || k >= limit || m * n >= limit || m * k >= limit || k * n >= limit)
+        return fail(LF_INVALID_ARGUMENT, "matrix dimensions exceed 32-bit indexing");
+    if (m && n && !C) return fail(LF_INVALID_ARGUMENT, "null output");
+    if (m * n && k && (!A || !B)) return fail(LF_INVALID_ARGUMENT, "null input");
+    if (report) {
+        std::memset(report, 0, sizeof *report);
+        report->m = m;
+        report->n = n;
+        report->k = k;
+    }
+    if (!m || !n) return LF_OK;
+    const int N = bits / 32;
+    const std::size_t E = bits / 8 + 12;
+    HostStage st;
+    GEMM_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking), "hipStreamCreate");
+    const std::size_t na = m * k, nb = k * n, nc = m * n;
+    GEMM_HIP(st.aos.alloc(std::max({na, nb, nc}) * E), "hipMalloc (staging)");
+    GEMM_HIP(st.pa.alloc(lf_planes_bytes(bits, 0, std::max<uint64_t>(na, 1))), "hipMalloc (A)");
+    GEMM_HIP(st.pb.alloc(lf_planes_bytes(bits, 0, std::max<uint64_t>(nb, 1))), "hipMalloc (B)");
+    GEMM_HIP(st.pc.alloc(lf_planes_bytes(bits, 0, nc)), "hipMalloc (C)");
+    lf_operand OA{st.pa.p, lf_plane_pitch(std::max<uint64_t>(na, 1))}, OB{st.pb.p, lf_plane_pitch(std::max<uint64_t>(nb, 1))},
+        OC{st.pc.p, lf_plane_pitch(nc)};
+    if (na) {
+        GEMM_HIP(hipMemcpyAsync(st.aos.p, A, na * E, hipMemcpyHostToDevice, st.stream), "upload A");
+        if (int r = lf_aos_to_planes(bits, 0, st.aos.p, &OA, na, st.stream)) return r;
+    }
+    if (nb) {
+        GEMM_HIP(hipMemcpyAsync(st.aos.p, B, nb * E, hipMemcpyHostToDevice, st.stream), "upload B");
+        if (int r = lf_aos_to_planes(bits, 0, st.aos.p, &OB, nb, st.stream)) return r;
+    }
+    if (subtract) {
+        GEMM_HIP(hipMemcpyAsync(st.aos.p, C, nc * E, hipMemcpyHostToDevice, st.stream), "upload C");
+        if (int r = lf_aos_to_planes(bits, 0, st.aos.p, &OC, nc, st.stream)) return r;
+    }
+    Product P{};
+    P.bits = bits;
+    P.syrk = false;
+    P.lower_only = false;
+    P.subtract = subtract != 0;
+    // op(A) row i, entry k: A[i*k + kk] or (transpose) A[kk*m + i]; B column j, entry kk: B[kk*n + j]
+    P.left = {(const uint32_t*)OA.planes, OA.pitch, transpose_a ? 1 : k, transpose_a ? m : 1};
+    P.right = {(const uint32_t*)OB.planes, OB.pitch, 1, n};
+    const uint32_t* a32 = (const uint32_t*)A;
+    const uint32_t* b32 = (const uint32_t*)B;
+    P.lf_ = {a32 + N, a32 + N + 1, a32 + N + 2, uint64_t(N + 3), P.left.ls, P.left.ks};
+    P.rf_ = {b32 + N, b32 + N + 1, b32 + N + 2, uint64_t(N + 3), 1, n};
+    P.m = m;
+    P.n = n;
+    P.K = k;
+    P.C = OC;
+    P.opt = o;
+    P.stream = st.stream;
+    float ms = 0;
+    if (int r = run_product(P, report, &ms)) return r;
+    if (int r = lf_planes_to_aos(bits, 0, &OC, st.aos.p, nc, st.stream)) return r;
+    GEMM_HIP(hipMemcpyAsync(C, st.aos.p, nc * E, hipMemcpyDeviceToHost, st.stream), "download C");
+    GEMM_HIP(hipStreamSynchronize(st.stream), "device execution (linalg)");
+    if (gpu_seconds) *gpu_seconds = ms * 1e-3;
+    if (wall_seconds) *wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    return LF_OK;
+}
+
+int lf_syrk_host(int bits, const void* A, uint64_t rows, uint64_t cols, void* C, int lower_only, int subtract,
+                 const lf_linalg_options* options, lf_linalg_report* report, double* gpu_seconds, double* wall_seconds)
+{
+    const auto start = std::chrono::steady_clock::now();
+    if (gpu_seconds) *gpu_seconds = 0;
+    if (wall_seconds) *wall_seconds = 0;
+    lf_linalg_options o;
+    if (options) o = *options;
+    else lf_linalg_default_options(&o);
+    if (int r = check_options(&o)) return r;
+    if (bits < 64 || bits > 1024 || bits % 32) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    if (subtract && !lower_only) return fail(LF_INVALID_ARGUMENT, "syrk: subtract updates the lower triangle only (lower_only)");
+    const uint64_t limit = uint64_t(1) << 31;
+    if (rows >= limit || cols >= limit || cols * cols >= limit || rows * cols >= limit)
+        return fail(LF_INVALID_ARGUMENT, "matrix dimensions exceed 32-bit indexing");
+    if (cols && !C) return fail(LF_INVALID_ARGUMENT, "null output");
+    if (cols && rows && !A) return fail(LF_INVALID_ARGUMENT, "null input");
+    if (report) {
+        std::memset(report, 0, sizeof *report);
+        report->m = cols;
+        report->n = cols;
+        report->k = rows;
+    }
+    if (!cols) return LF_OK;
+    const int N = bits / 32;
+    const std::size_t E = bits / 8 + 12;
+    HostStage st;
+    GEMM_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking), "hipStreamCreate");
+    const std::size_t na = rows * cols, nc = cols * cols;
+    GEMM_HIP(st.aos.alloc(std::max(na, nc) * E), "hipMalloc (staging)");
+    GEMM_HIP(st.pa.alloc(lf_planes_bytes(bits, 0, std::max<uint64_t>(na, 1))), "hipMalloc (A)");
+    GEMM_HIP(st.pc.alloc(lf_planes_bytes(bits, 0, nc)), "hipMalloc (C)");
+    lf_operand OA{st.pa.p, lf_plane_pitch(std::max<uint64_t>(na, 1))}, OC{st.pc.p, lf_plane_pitch(nc)};
+    if (na) {
+        GEMM_HIP(hipMemcpyAsync(st.aos.p, A, na * E, hipMemcpyHostToDevice, st.stream), "upload A");
+        if (int r = lf_aos_to_planes(bits, 0, st.aos.p, &OA, na, st.stream)) return r;
+    }
+    if (subtract || lower_only) { // untouched entries keep their old values
+        GEMM_HIP(hipMemcpyAsync(st.aos.p, C, nc * E, hipMemcpyHostToDevice, st.stream), "upload C");
+        if (int r = lf_aos_to_planes(bits, 0, st.aos.p, &OC, nc, st.stream)) return r;
+    }
+    Product P{};
+    P.bits = bits;
+    P.syrk = true;
+    P.lower_only = lower_only != 0;
+    P.subtract = subtract != 0;
+    // line i = column i of A: element (i, kk) = A[kk*cols + i]
+    P.left = {(const uint32_t*)OA.planes, OA.pitch, 1, cols};
+    P.right = P.left;
+    const uint32_t* a32 = (const uint32_t*)A;
+    P.lf_ = {a32 + N, a32 + N + 1, a32 + N + 2, uint64_t(N + 3), 1, cols};
+    P.rf_ = P.lf_;
+    P.m = cols;
+    P.n = cols;
+    P.K = rows;
+    P.C = OC;
+    P.opt = o;
+    P.stream = st.stream;
+    float ms = 0;
+    if (int r = run_product(P, report, &ms)) return r;
+    if (int r = lf_planes_to_aos(bits, 0, &OC, st.aos.p, nc, st.stream)) return r;
+    GEMM_HIP(hipMemcpyAsync(C, st.aos.p, nc * E, hipMemcpyDeviceToHost, st.stream), "download C");
+    GEMM_HIP(hipStreamSynchronize(st.stream), "device execution (linalg)");
+    if (gpu_seconds) *gpu_seconds = ms * 1e-3;
+    if (wall_seconds) *wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    return LF_OK;
+}
+
+} // extern "C"

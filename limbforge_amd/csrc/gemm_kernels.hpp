@@ -1,0 +1,480 @@
+// gemm_kernels.hpp — kernels of the exact dense products C = RN(op(A) B) and RN(C - op(A) B).
+//
+// Residue path (the reference's algorithm, src/linalg.metal:24-152, rebuilt for CDNA4):
+//   digits_kernel   each entry's exact band integer X = m * 2^(e-lo) reduced modulo every prime (16-bit
+//                   chunk x (2^16h mod m) dot products, then the band shift), balanced residue
+//                   v in (-m/2, m/2], split into int8 digits v = 256 d1 + d0 (linalg.metal:24-39)
+//   product_kernel  per modulus and 64x64 tile: G11 = D1 D1', G00 = D0 D0', Gmid = D1 D0' + D0 D1' on
+//                   v_mfma_i32_32x32x32_i8 with int32 accumulation from LDS-staged tiles, then
+//                   (65536 G11 + 256 Gmid + G00) mod m written as uint16 (linalg.metal:66-96)
+//   rebuild_kernel  one lane per output: for each band pair, the exact integer by direct CRT
+//                   X' = sum_q y_q (M/m_q) 2^t with y_q = r_q (M/m_q)^-1 mod m_q (independent 64-bit
+//                   column sums, no carry chains), reduced mod M' and mapped to the symmetric range,
+//                   then one rounding, or exact accumulation over the band pairs then one rounding
+//                   (the reference's Garner `reconstruct` + `finish`, linalg.metal:109-152; the same integer)
+// Exact dot kernels (plain limb GEMM and the lines the residue path does not take, linalg.hpp:268-318):
+//   dot_window_kernel   every exact 2N-word product added into one two's-complement window
+//   dot_cluster_kernel  exponent clusters with a sticky sign for outputs whose terms spread further
+#pragma once
+#include "gemm_common.hpp"
+#include "limb_core.hpp"
+
+namespace lf {
+
+__device__ __forceinline__ u32 reduce32(u32 x, u32 m, u32 c)
+{
+    u32 y = x - __umulhi(x, c) * m;
+    return y >= m ? y - m : y;
+}
+
+__device__ __forceinline__ u32 mulmod16(u32 a, u32 b, u32 m, u32 c) { return reduce32(a * b, m, c); }
+
+// ---------------------------------------------------------------- digits
+template <int N> __global__ void __launch_bounds__(256) digits_kernel(DigitsArgs g)
+{
+    const u32 kblocks = (g.K + 255u) / 256u;
+    const u32 line = blockIdx.x / kblocks;
+    const u32 k = (blockIdx.x - line * kblocks) * 256u + threadIdx.x;
+    if (k >= g.K || line >= g.nlines) return;
+    const GemmLine* L = g.lines + line;
+    if (L->cls != 0) return;
+    const u64 idx = (u64)line * g.v.ls + (u64)k * g.v.ks;
+    const u32* P = g.v.planes;
+    const u64 pitch = g.v.pitch;
+    const u32 st = P[(u64)(N + 2) * pitch + idx];
+    const int s = (int)P[(u64)(N + 1) * pitch + idx];
+    if (st || !s) return;
+    const int e = (int)P[(u64)N * pitch + idx];
+    int row = -1, shift = 0;
+    for (int b = 0; b < L->nb; ++b)
+        if (e >= L->lo[b] && e <= L->hi[b]) {
+            row = L->member[b];
+            shift = e - L->lo[b];
+        }
+    if (row < 0) return;
+    u32 c[2 * N];
+#pragma unroll
+    for (int w = 0; w < N; ++w) {
+        const u32 x = P[(u64)w * pitch + idx];
+        c[2 * w] = x & 0xffffu;
+        c[2 * w + 1] = x >> 16;
+    }
+    const u64 plane = (u64)g.rows_p * g.Kp;
+    const u64 at = (u64)row * g.Kp + k;
+    for (u32 q = 0; q < g.L; ++q) {
+        const u32 m = g.moduli[q], cm = g.recip[q];
+        const u32* cp = g.chunk_pow + q * 64u;
+        u64 acc = 0;
+#pragma unroll
+        for (int h = 0; h < 2 * N; ++h) acc += (u64)c[h] * cp[h];
+        // acc < 2N * 2^32: fold the high part with 2^32 mod m = cp[2]
+        u32 r = reduce32((u32)acc, m, cm) + (u32)(acc >> 32) * cp[2];
+        r = reduce32(r, m, cm);
+        r = mulmod16(r, g.shift_pow[q * (g.shift_max + 1) + (u32)shift], m, cm);
+        int v = (int)r;
+        if (s < 0) v = v ? (int)m - v : 0;
+        if (v > (int)(m >> 1)) v -= (int)m;
+        const int d0 = ((v + 128) & 255) - 128;
+        const int d1 = (v - d0) / 256;
+        g.d1[q * plane + at] = (int8_t)d1;
+        g.d0[q * plane + at] = (int8_t)d0;
+    }
+}
+
+// ---------------------------------------------------------------- CRT rebuild
+// X' = |sum| * 2^t from the L residues at (row, col); returns the sign (+1/-1) of the symmetric value.
+template <int WX>
+__device__ __forceinline__ int crt_rebuild(const RebuildArgs& g, u32 row, u32 col, u32 (&X)[WX])
+{
+    u64 colsum[WX];
+#pragma unroll
+    for (int w = 0; w < WX; ++w) colsum[w] = 0;
+    const u64 plane = (u64)g.Mp * g.Np, at = (u64)row * g.Np + col;
+    for (u32 q = 0; q < g.L; ++q) {
+        const u32 m = g.moduli[q], cm = g.recip[q];
+        const u32 r = g.res[q * plane + at];
+        const u32 y = mulmod16(r, g.cinv[q], m, cm);
+        const u32* C = g.crt + q * WX;
+#pragma unroll
+        for (int w = 0; w < WX; ++w) colsum[w] += (u64)y * C[w];
+    }
+    u64 carry = 0;
+#pragma unroll
+    for (int w = 0; w < WX; ++w) {
+        const u64 v = colsum[w] + carry;
+        X[w] = (u32)v;
+        carry = v >> 32;
+    }
+    // X' < L M' with M' normalised to bit 32(WX-1)-1: estimate q = floor(X'/M') from the top words
+    const u32* Mb = g.bound;
+    const u64 xt = ((u64)X[WX - 1] << 32) | X[WX - 2];
+    u64 qe = xt / ((u64)Mb[WX - 2] + 1u);
+    // X' -= qe * M'
+    u64 bw = 0;
+#pragma unroll
+    for (int w = 0; w < WX; ++w) {
+        const u64 p = qe * Mb[w] + bw;
+        const u32 lo = (u32)p, old = X[w];
+        X[w] = old - lo;
+        bw = (p >> 32) + (old < lo ? 1u : 0u);
+    }
+    for (int rep = 0; rep < 3; ++rep) {
+        int cmp = 0;
+#pragma unroll
+        for (int w = WX - 1; w >= 0; --w)
+            if (cmp == 0 && X[w] != Mb[w]) cmp = X[w] > Mb[w] ? 1 : -1;
+        if (cmp < 0) break;
+        u32 b2 = 0;
+#pragma unroll
+        for (int w = 0; w < WX; ++w) {
+            const u64 v = (u64)X[w] - Mb[w] - b2;
+            X[w] = (u32)v;
+            b2 = (u32)(v >> 63);
+        }
+    }
+    // symmetric range: X' > floor(M'/2) means the negative value X' - M'
+    const u32* H = g.bound + WX;
+    int cmp = 0;
+#pragma unroll
+    for (int w = WX - 1; w >= 0; --w)
+        if (cmp == 0 && X[w] != H[w]) cmp = X[w] > H[w] ? 1 : -1;
+    if (cmp <= 0) return 1;
+    u32 b3 = 0;
+#pragma unroll
+    for (int w = 0; w < WX; ++w) {
+        const u64 v = (u64)Mb[w] - X[w] - b3;
+        X[w] = (u32)v;
+        b3 = (u32)(v >> 63);
+    }
+    return -1;
+}
+
+template <int N> __device__ __forceinline__ void store_out(u32* base, u64 pitch, u64 stride, u32 i, u32 j, bool mirror, const Num<N>& r)
+{
+    const u64 at = (u64)i * stride + j;
+#pragma unroll
+    for (int k = 0; k < N; ++k) base[k * pitch + at] = r.w[k];
+    base[N * pitch + at] = (u32)r.e;
+    base[(N + 1) * pitch + at] = (u32)r.s;
+    base[(N + 2) * pitch + at] = r.st;
+    if (mirror && i != j) {
+        const u64 bt = (u64)j * stride + i;
+#pragma unroll
+        for (int k = 0; k < N; ++k) base[k * pitch + bt] = r.w[k];
+        base[N * pitch + bt] = (u32)r.e;
+        base[(N + 1) * pitch + bt] = (u32)r.s;
+        base[(N + 2) * pitch + bt] = r.st;
+    }
+}
+
+template <int N> __device__ __forceinline__ Num<N> load_out(const u32* base, u64 pitch, u64 at)
+{
+    Num<N> r;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.w[k] = base[k * pitch + at];
+    r.e = (int)base[N * pitch + at];
+    r.s = (int)base[(N + 1) * pitch + at];
+    r.st = base[(N + 2) * pitch + at];
+    return r;
+}
+
+// WX: CRT integer words (M' normalised to bit 32(WX-1)-1); ACC: multi-band accumulator words.
+template <int N, int WX, int ACC> __global__ void __launch_bounds__(256) rebuild_kernel(RebuildArgs g)
+{
+    const u32 nblocks = (g.n + 255u) / 256u;
+    const u32 i = blockIdx.x / nblocks;
+    const u32 j = (blockIdx.x - i * nblocks) * 256u + threadIdx.x;
+    if (i >= g.m || j >= g.n) return;
+    if (g.syrk && j > i) return; // SYRK computes the lower triangle (mirrored when full)
+    const bool mirror = g.syrk && !g.lower_only;
+    const GemmLine* Li = g.left + i;
+    const GemmLine* Lj = g.right + j;
+    const u64 at = (u64)i * g.out_stride + j;
+    Num<N> old = zero<N>(ST_OK);
+    if (g.subtract) old = load_out<N>(g.out, g.out_pitch, at);
+    if (Li->cls == 1 || Lj->cls == 1) {
+        // zero or status line: zero with the statuses; a zero line leaves an update unchanged
+        u32 st = Li->status | Lj->status | (g.subtract ? old.st : 0u);
+        if (!g.subtract || st) store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, zero<N>(st));
+        return;
+    }
+    if (Li->cls != 0 || Lj->cls != 0) return; // exact dot kernels
+    if (g.subtract && old.st) {
+        store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, zero<N>(old.st));
+        return;
+    }
+    const int nbi = Li->nb, nbj = Lj->nb;
+    const i64 base = (i64)Li->low + Lj->low - 2 * (32 * N - 1);
+    const i64 old_scale = (i64)old.e - (32 * N - 1);
+    Num<N> result;
+    if (nbi == 1 && nbj == 1) {
+        u32 X[WX];
+        u32 r0 = (u32)Li->member[0], c0 = (u32)Lj->member[0];
+        if (g.syrk && c0 > r0) { // the extended SYRK product is symmetric: read its lower triangle
+            const u32 t0 = r0;
+            r0 = c0;
+            c0 = t0;
+        }
+        const int sg = crt_rebuild<WX>(g, r0, c0, X);
+        if (!g.subtract) result = pack<N, WX>(X, base - g.t, sg, ST_OK);
+        else result = round_sum2<N, WX, N>(X, base - g.t, -sg, old.w, old_scale, old.s);
+    } else if constexpr (ACC <= WX) {
+        result = zero<N>(ST_INVALID); // not reached: the planner sends multi-band lines of this class elsewhere
+    } else {
+        const int tb = (g.t + 31) & ~31;
+        u32 acc[ACC];
+#pragma unroll
+        for (int w = 0; w < ACC; ++w) acc[w] = 0;
+        for (int b = 0; b < nbi; ++b)
+            for (int c = 0; c < nbj; ++c) {
+                u32 X[WX], P[ACC];
+                u32 rr = (u32)Li->member[b], rc = (u32)Lj->member[c];
+                if (g.syrk && rc > rr) {
+                    const u32 t0 = rr;
+                    rr = rc;
+                    rc = t0;
+                }
+                const int sg = crt_rebuild<WX>(g, rr, rc, X);
+                const i64 shift = (i64)(Li->lo[b] - Li->low) + (Lj->lo[c] - Lj->low) + tb - g.t;
+                place<ACC, WX>(P, X, shift);
+                accumulate_signed<ACC>(acc, P, sg);
+            }
+        int sign = 1;
+        if (acc[ACC - 1] >> 31) {
+            sign = -1;
+            u32 cy = 1;
+#pragma unroll
+            for (int w = 0; w < ACC; ++w) {
+                const u64 v = (u64)(u32)~acc[w] + cy;
+                acc[w] = (u32)v;
+                cy = (u32)(v >> 32);
+            }
+        }
+        if (!g.subtract) result = pack<N, ACC>(acc, base - tb, sign, ST_OK);
+        else result = round_sum2<N, ACC, N>(acc, base - tb, -sign, old.w, old_scale, old.s);
+    }
+    store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, result);
+}
+
+// ---------------------------------------------------------------- exact dot products
+__device__ __forceinline__ void output_of(const DotArgs& g, u32 idx, u32& i, u32& j)
+{
+    if (g.list) {
+        i = g.list[2 * idx];
+        j = g.list[2 * idx + 1];
+    } else {
+        i = idx / g.n;
+        j = idx - i * g.n;
+    }
+}
+
+template <int N> __device__ __forceinline__ Num<N> load_view(const MatView& v, u32 line, u32 k)
+{
+    return load_out<N>(v.planes, v.pitch, (u64)line * v.ls + (u64)k * v.ks);
+}
+
+// The plain exact limb GEMM: every term a_k b_k is formed exactly (2N words) and added into one W-word
+// two's-complement window whose lsb is the smallest possible term exponent of the output; one rounding.
+template <int N, int W> __global__ void __launch_bounds__(256) dot_window_kernel(DotArgs g)
+{
+    const u32 idx = blockIdx.x * 256u + threadIdx.x;
+    if (idx >= g.count) return;
+    u32 i, j;
+    output_of(g, idx, i, j);
+    if (g.syrk && j > i) return;
+    const GemmLine* Li = g.lrows + i;
+    const GemmLine* Lj = g.rcols + j;
+    const u64 at = (u64)i * g.out_stride + j;
+    const bool mirror = g.mirror != 0;
+    Num<N> old = zero<N>(ST_OK);
+    if (g.subtract) old = load_out<N>(g.out, g.out_pitch, at);
+    if (Li->cls == 1 || Lj->cls == 1) {
+        u32 st = Li->status | Lj->status | (g.subtract ? old.st : 0u);
+        if (!g.subtract || st) store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, zero<N>(st));
+        return;
+    }
+    if (g.subtract && old.st) {
+        store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, zero<N>(old.st));
+        return;
+    }
+    const i64 lo = (i64)Li->bottom + Lj->bottom;
+    u32 acc[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) acc[w] = 0;
+    for (u32 k = 0; k < g.K; ++k) {
+        const Num<N> a = load_view<N>(g.left, i, k);
+        const Num<N> b = load_view<N>(g.right, j, k);
+        if (!a.s || !b.s) continue;
+        u32 p[2 * N], w[W];
+        mantissa_product<N>(a.w, b.w, p);
+        place<W, 2 * N>(w, p, (i64)a.e + b.e - lo);
+        accumulate_signed<W>(acc, w, a.s * b.s);
+    }
+    int sign = 1;
+    if (acc[W - 1] >> 31) {
+        sign = -1;
+        u32 cy = 1;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const u64 v = (u64)(u32)~acc[w] + cy;
+            acc[w] = (u32)v;
+            cy = (u32)(v >> 32);
+        }
+    }
+    const i64 scale = lo - 2 * (32 * N - 1);
+    Num<N> r;
+    if (!g.subtract) r = pack<N, W>(acc, scale, sign, ST_OK);
+    else r = round_sum2<N, W, N>(acc, scale, -sign, old.w, (i64)old.e - (32 * N - 1), old.s);
+    store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, r);
+}
+
+// Exact dot product for terms spread beyond any fixed window (the cluster algorithm of
+// linalg.hpp:298-317): terms in descending exponent order form clusters (a term joins while its top bit
+// lies within gap = 32N + ceil(log2(K+1)) + 4 bits of the cluster's lowest bit); the first nonzero
+// cluster sum S is exact and only the sign of the next nonzero cluster matters, so the result is
+// RN(S * 2^D +/- 1) * 2^-D with D = 32N + 64.  Clusters are found by repeated passes over the terms
+// (no sort); a cluster wider than WC words sets *error.  The update's old value is one more term.
+template <int N, int WC> __global__ void __launch_bounds__(256) dot_cluster_kernel(DotArgs g)
+{
+    const u32 idx = blockIdx.x * 256u + threadIdx.x;
+    if (idx >= g.count) return;
+    u32 i, j;
+    output_of(g, idx, i, j);
+    if (g.syrk && j > i) return;
+    const GemmLine* Li = g.lrows + i;
+    const GemmLine* Lj = g.rcols + j;
+    const u64 at = (u64)i * g.out_stride + j;
+    const bool mirror = g.mirror != 0;
+    Num<N> old = zero<N>(ST_OK);
+    if (g.subtract) old = load_out<N>(g.out, g.out_pitch, at);
+    if (Li->cls == 1 || Lj->cls == 1) {
+        u32 st = Li->status | Lj->status | (g.subtract ? old.st : 0u);
+        if (!g.subtract || st) store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, zero<N>(st));
+        return;
+    }
+    if (g.subtract && old.st) {
+        store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, zero<N>(old.st));
+        return;
+    }
+    const bool has_c = g.subtract && old.s;
+    const i64 width = 64 * N, gap = 32 * N + (i64)g.logk + 4;
+    const i64 NONE = -((i64)1 << 62);
+    // term k < K: product scale; k == K: the old value
+    auto scale_of = [&](u32 k, int& sg) -> i64 {
+        if (k == g.K) {
+            sg = old.s;
+            return (i64)old.e - (32 * N - 1);
+        }
+        const u64 ia = (u64)i * g.left.ls + (u64)k * g.left.ks, ib = (u64)j * g.right.ls + (u64)k * g.right.ks;
+        const int sa = (int)g.left.planes[(u64)(N + 1) * g.left.pitch + ia];
+        const int sb = (int)g.right.planes[(u64)(N + 1) * g.right.pitch + ib];
+        sg = g.subtract ? -sa * sb : sa * sb;
+        if (!sg) return NONE;
+        const int ea = (int)g.left.planes[(u64)N * g.left.pitch + ia];
+        const int eb = (int)g.right.planes[(u64)N * g.right.pitch + ib];
+        return (i64)ea + eb - 2 * (32 * N - 1);
+    };
+    const u32 terms = g.K + (has_c ? 1u : 0u);
+    i64 cutoff = ((i64)1 << 62);
+    int nsums = 0, sign1 = 0, sign2 = 0;
+    i64 scale1 = 0;
+    u32 S[WC];
+#pragma unroll
+    for (int w = 0; w < WC; ++w) S[w] = 0;
+    while (nsums < 2) {
+        // highest remaining term
+        i64 hi = NONE;
+        for (u32 k = 0; k < terms; ++k) {
+            int sg;
+            const i64 sc = scale_of(k, sg);
+            if (sg && sc < cutoff && sc > hi) hi = sc;
+        }
+        if (hi == NONE) break;
+        // close the cluster downwards
+        i64 low = hi;
+        for (bool changed = true; changed;) {
+            changed = false;
+            for (u32 k = 0; k < terms; ++k) {
+                int sg;
+                const i64 sc = scale_of(k, sg);
+                if (sg && sc < low && sc + width - 1 + gap >= low) {
+                    low = sc;
+                    changed = true;
+                }
+            }
+        }
+        const i64 words = (hi + width - low) / 32 + 3;
+        if (words > WC) {
+            atomicOr(g.error, 1);
+            return;
+        }
+        u32 acc[WC];
+#pragma unroll
+        for (int w = 0; w < WC; ++w) acc[w] = 0;
+        for (u32 k = 0; k < terms; ++k) {
+            int sg;
+            const i64 sc = scale_of(k, sg);
+            if (!sg || sc < low || sc >= cutoff) continue;
+            u32 w[WC];
+            if (k == g.K) {
+                place<WC, N>(w, old.w, sc - low);
+            } else {
+                const Num<N> a = load_view<N>(g.left, i, k);
+                const Num<N> b = load_view<N>(g.right, j, k);
+                u32 p[2 * N];
+                mantissa_product<N>(a.w, b.w, p);
+                place<WC, 2 * N>(w, p, sc - low);
+            }
+            accumulate_signed<WC>(acc, w, sg);
+        }
+        int sign = 1;
+        if (acc[WC - 1] >> 31) {
+            sign = -1;
+            u32 cy = 1;
+#pragma unroll
+            for (int w = 0; w < WC; ++w) {
+                const u64 v = (u64)(u32)~acc[w] + cy;
+                acc[w] = (u32)v;
+                cy = (u32)(v >> 32);
+            }
+        }
+        bool nonzero = false;
+#pragma unroll
+        for (int w = 0; w < WC; ++w) nonzero = nonzero || acc[w] != 0;
+        if (nonzero) {
+            if (!nsums) {
+#pragma unroll
+                for (int w = 0; w < WC; ++w) S[w] = acc[w];
+                scale1 = low;
+                sign1 = sign;
+            } else
+                sign2 = sign;
+            ++nsums;
+        }
+        cutoff = low;
+    }
+    Num<N> r;
+    if (!nsums) r = zero<N>(ST_OK);
+    else if (nsums == 1) r = pack<N, WC>(S, scale1, sign1, ST_OK);
+    else {
+        // S * 2^D -/+ 1 puts the sticky unit below the rounding position of S
+        constexpr int DW = N + 2, EW = WC + DW + 1;
+        u32 ext[EW];
+#pragma unroll
+        for (int w = 0; w < EW; ++w) ext[w] = (w >= DW && w - DW < WC) ? S[w - DW] : 0u;
+        if (sign2 == sign1) ext[0] = 1;
+        else {
+            u32 bw = 1;
+#pragma unroll
+            for (int w = 0; w < EW; ++w) {
+                const u64 v = (u64)ext[w] - bw;
+                ext[w] = (u32)v;
+                bw = (u32)(v >> 63);
+            }
+        }
+        r = pack<N, EW>(ext, scale1 - 32 * DW, sign1, ST_OK);
+    }
+    store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, r);
+}
+
+} // namespace lf

@@ -1,0 +1,41 @@
+// One width of the dense-product kernels: compiled once per LF_WORDS in [2, 32].
+#include "gemm_kernels.hpp"
+#include "gemm_launch.hpp"
+
+#ifndef LF_WORDS
+#error "compile with -DLF_WORDS=<2..32>"
+#endif
+
+namespace lf {
+
+template <int N> hipError_t launch_digits(const DigitsArgs& g, hipStream_t s)
+{
+    if (!g.K || !g.nlines || !g.L) return hipSuccess;
+    hipLaunchKernelGGL((digits_kernel<N>), dim3(((g.K + 255u) / 256u) * g.nlines), dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+template <int N> hipError_t launch_rebuild(int wx_class, const RebuildArgs& g, hipStream_t s)
+{
+    if (!g.m || !g.n) return hipSuccess;
+    const dim3 grid(((g.n + 255u) / 256u) * g.m), block(256);
+    if (wx_class == 0) hipLaunchKernelGGL((rebuild_kernel<N, rebuild_words(N, 0), rebuild_words(N, 0) + 33>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((rebuild_kernel<N, rebuild_words(N, 1), 1>), grid, block, 0, s, g);
+    return hipGetLastError();
+}
+
+template <int N> hipError_t launch_dot(int window_class, const DotArgs& g, hipStream_t s)
+{
+    if (!g.count) return hipSuccess;
+    const dim3 grid((g.count + 255u) / 256u), block(256);
+    if (window_class == 0) hipLaunchKernelGGL((dot_window_kernel<N, 64>), grid, block, 0, s, g);
+    else if (window_class == 1) hipLaunchKernelGGL((dot_window_kernel<N, 128>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((dot_cluster_kernel<N, 128>), grid, block, 0, s, g);
+    return hipGetLastError();
+}
+
+template hipError_t launch_digits<LF_WORDS>(const DigitsArgs&, hipStream_t);
+template hipError_t launch_rebuild<LF_WORDS>(int, const RebuildArgs&, hipStream_t);
+template hipError_t launch_dot<LF_WORDS>(int, const DotArgs&, hipStream_t);
+
+} // namespace lf

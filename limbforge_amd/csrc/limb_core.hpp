@@ -643,6 +643,64 @@ template <int N> __device__ __forceinline__ Num<N> fma(const Num<N>& a, const Nu
     return pack<N, FW>(acc, scale, sign, ST_OK);
 }
 
+// Position of the highest set bit of x, or -1 (a scan over compile-time indices).
+template <int W> __device__ __forceinline__ int highest_bit(const u32 (&x)[W])
+{
+    int h = -1;
+#pragma unroll
+    for (int i = 0; i < W; ++i)
+        if (x[i]) h = 32 * i + 31 - __clz(x[i]);
+    return h;
+}
+
+// RN(s1 * int(t1) * 2^e1 + s2 * int(t2) * 2^e2) with one rounding, for exact terms of W1 and W2 words
+// (either may be zero).  The window holds the larger term entirely and reaches far enough below it that
+// a smaller term losing bits below the window lies more than 32N+60 bits below the larger msb, where it
+// only contributes the sign of one sticky unit (the argument of core.hpp:251-253).  Used for the update
+// form RN(c - sum) of the dense products (linalg.metal:13-16,100-105 round c against the exact sum).
+template <int N, int W1, int W2>
+__device__ __forceinline__ Num<N> round_sum2(const u32 (&t1)[W1], i64 e1, int s1, const u32 (&t2)[W2], i64 e2, int s2)
+{
+    const int h1 = highest_bit<W1>(t1), h2 = highest_bit<W2>(t2);
+    if (h1 < 0) s1 = 0;
+    if (h2 < 0) s2 = 0;
+    if (!s1 && !s2) return zero<N>(ST_OK);
+    const i64 m1 = e1 + h1, m2 = e2 + h2;
+    const i64 mx = !s2 ? m1 : !s1 ? m2 : (m1 > m2 ? m1 : m2);
+    constexpr int FW = W1 + W2 + N + 2;
+    const i64 L = mx + 3 - 32 * (i64)FW; // the larger msb sits at window bit 32FW-3
+    u32 acc[FW], w[FW];
+#pragma unroll
+    for (int i = 0; i < FW; ++i) acc[i] = 0;
+    bool d1 = false, d2 = false;
+    if (s1) {
+        d1 = place<FW, W1>(w, t1, e1 - L);
+        accumulate_signed<FW>(acc, w, s1);
+    }
+    if (s2) {
+        d2 = place<FW, W2>(w, t2, e2 - L);
+        accumulate_signed<FW>(acc, w, s2);
+    }
+    i64 scale = L;
+    if (d1 || d2) {
+        shl_bits(acc, 1);
+        accumulate_unit<FW>(acc, d1 ? s1 : s2);
+        scale = L - 1;
+    }
+    int sign = 1;
+    if (acc[FW - 1] >> 31) {
+        sign = -1;
+        u32 cy = 1;
+#pragma unroll
+        for (int i = 0; i < FW; ++i) {
+            u64 v = (u64)(u32)~acc[i] + cy;
+            acc[i] = (u32)v;
+            cy = (u32)(v >> 32);
+        }
+    }
+    return pack<N, FW>(acc, scale, sign, ST_OK);
+}
+
 template <int N> __device__ __forceinline__ Num<N> fms(const Num<N>& a, const Num<N>& b, const Num<N>& c)
 {
     return fma<N>(a, b, negate<N>(c));
