@@ -61,10 +61,35 @@ def _hip_jobs() -> list[tuple[Path, list[str]]]:
     return jobs
 
 
-def build_hip(jobs: int) -> Path:
+def _deps(src: Path, seen=None) -> set[Path]:
+    """The source and every quoted header it includes from csrc/ or include/, recursively."""
+    seen = set() if seen is None else seen
+    if src in seen or not src.exists():
+        return seen
+    seen.add(src)
+    for line in src.read_text().splitlines():
+        line = line.strip()
+        if line.startswith("#include \""):
+            name = line.split('"')[1]
+            for d in (CSRC, INCLUDE):
+                if (d / name).exists():
+                    _deps(d / name, seen)
+    return seen
+
+
+def _stale(out: Path, cmd: list[str]) -> bool:
+    if not out.exists():
+        return True
+    src = Path(cmd[cmd.index("-c") + 1])
+    newest = max(p.stat().st_mtime for p in _deps(src) | {Path(__file__)})
+    return out.stat().st_mtime < newest
+
+
+def build_hip(jobs: int, incremental: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
     LIB.mkdir(exist_ok=True)
-    work = _hip_jobs()
+    every = _hip_jobs()
+    work = [j for j in every if not incremental or _stale(*j)]
     # the heaviest translation units first
     work.sort(key=lambda j: -int(j[0].stem.split("_w")[-1]) if "_w" in j[0].stem else -100)
     failures = []
@@ -76,7 +101,7 @@ def build_hip(jobs: int) -> Path:
         for cmd, log in failures:
             sys.stderr.write(" ".join(cmd) + "\n" + log + "\n")
         raise RuntimeError(f"{len(failures)} HIP translation unit(s) failed to compile")
-    objs = [str(o) for o, _ in work]
+    objs = [str(o) for o, _ in every]
     lib = LIB / "liblimbforge_hip.so"
     tmp = LIB / "liblimbforge_hip.so.tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs]
@@ -123,11 +148,12 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--jobs", type=int, default=min(16, os.cpu_count() or 4))
     ap.add_argument("--only", choices=["hip", "cxx"], default=None)
     ap.add_argument("--clean", action="store_true")
+    ap.add_argument("--incremental", action="store_true", help="recompile only stale objects")
     a = ap.parse_args(argv)
     if a.clean and BUILD.exists():
         shutil.rmtree(BUILD)
     if a.only in (None, "hip"):
-        print("built", build_hip(a.jobs))
+        print("built", build_hip(a.jobs, a.incremental))
     if a.only in (None, "cxx"):
         lib = build_cxx(a.jobs)
         if lib:

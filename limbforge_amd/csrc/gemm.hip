@@ -375,19 +375,27 @@ struct DeviceBuf {
 typedef hipError_t (*digits_fn)(const lf::DigitsArgs&, hipStream_t);
 typedef hipError_t (*rebuild_fn)(int, const lf::RebuildArgs&, hipStream_t);
 typedef hipError_t (*dot_fn)(int, const lf::DotArgs&, hipStream_t);
-template <int N> void fill_gemm(digits_fn* d, rebuild_fn* r, dot_fn* x)
-{
-    d[N] = lf::launch_digits<N>;
-    r[N] = lf::launch_rebuild<N>;
-    x[N] = lf::launch_dot<N>;
-    if constexpr (N < 32) fill_gemm<N + 1>(d, r, x);
-}
+typedef hipError_t (*csize_fn)(const lf::DotArgs&, uint32_t*, hipStream_t);
+typedef hipError_t (*csum_fn)(const lf::DotArgs&, const uint32_t*, const uint64_t*, uint32_t*, hipStream_t);
+struct GemmTables;
+template <int N> void fill_gemm(GemmTables& t);
 struct GemmTables {
     digits_fn digits[33] = {};
     rebuild_fn rebuild[33] = {};
     dot_fn dot[33] = {};
-    GemmTables() { fill_gemm<2>(digits, rebuild, dot); }
+    csize_fn cluster_size[33] = {};
+    csum_fn cluster_sum[33] = {};
+    GemmTables() { fill_gemm<2>(*this); }
 };
+template <int N> void fill_gemm(GemmTables& t)
+{
+    t.digits[N] = lf::launch_digits<N>;
+    t.rebuild[N] = lf::launch_rebuild<N>;
+    t.dot[N] = lf::launch_dot<N>;
+    t.cluster_size[N] = lf::launch_cluster_size<N>;
+    t.cluster_sum[N] = lf::launch_cluster_sum<N>;
+    if constexpr (N < 32) fill_gemm<N + 1>(t);
+}
 const GemmTables& gemm_tables()
 {
     static const GemmTables t;
@@ -693,7 +701,42 @@ int run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
         x.subtract = P.subtract;
         x.mirror = P.syrk && !P.lower_only;
         x.error = (int32_t*)derr.p;
-        GEMM_HIP(gemm_tables().dot[N](c, x, P.stream), "exact dot launch");
+        if (c < 2) {
+            GEMM_HIP(gemm_tables().dot[N](c, x, P.stream), "exact dot launch");
+            continue;
+        }
+        // exponent clusters: size every output's widest cluster, then sum in exactly sized scratch
+        const std::size_t cnt = x.count;
+        DeviceBuf dwords, doffs, dscratch;
+        GEMM_HIP(dwords.alloc(cnt * 4), "hipMalloc (cluster sizes)");
+        GEMM_HIP(gemm_tables().cluster_size[N](x, (uint32_t*)dwords.p, P.stream), "cluster size launch");
+        std::vector<uint32_t> words(cnt);
+        GEMM_HIP(hipMemcpyAsync(words.data(), dwords.p, cnt * 4, hipMemcpyDeviceToHost, P.stream), "download cluster sizes");
+        GEMM_HIP(hipStreamSynchronize(P.stream), "device execution (cluster sizes)");
+        // scratch in chunks of outputs within a budget (two accumulators per output)
+        const uint64_t budget = uint64_t(4) << 30; // bytes
+        for (std::size_t first = 0; first < cnt;) {
+            std::vector<uint64_t> offs;
+            uint64_t total = 0;
+            std::size_t last = first;
+            while (last < cnt && (last == first || (total + 2ull * words[last]) * 4 <= budget)) {
+                offs.push_back(total);
+                total += 2ull * words[last];
+                ++last;
+            }
+            if (total * 4 > (uint64_t(64) << 30))
+                return fail(LF_RUNTIME_ERROR, "exact dot: an exponent cluster needs more than 64 GB of accumulator");
+            GEMM_HIP(doffs.alloc(offs.size() * 8), "hipMalloc (cluster offsets)");
+            GEMM_HIP(dscratch.alloc(std::max<uint64_t>(total, 1) * 4), "hipMalloc (cluster scratch)");
+            GEMM_HIP(hipMemcpyAsync(doffs.p, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, P.stream), "upload offsets");
+            lf::DotArgs y = x;
+            y.list = x.list + 2 * first;
+            y.count = uint32_t(last - first);
+            GEMM_HIP(gemm_tables().cluster_sum[N](y, (const uint32_t*)dwords.p + first, (const uint64_t*)doffs.p, (uint32_t*)dscratch.p, P.stream),
+                     "cluster sum launch");
+            GEMM_HIP(hipStreamSynchronize(P.stream), "device execution (clusters)");
+            first = last;
+        }
     }
     GEMM_HIP(hipEventRecord(e1, P.stream), "hipEventRecord");
     int32_t err = 0;
@@ -702,7 +745,7 @@ int run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     float ms = 0;
     GEMM_HIP(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
     if (device_ms) *device_ms = ms;
-    if (err) return fail(LF_RUNTIME_ERROR, "exact dot: an exponent cluster exceeds the GPU window (4096 bits)");
+    if (err) return fail(LF_RUNTIME_ERROR, "exact dot: internal window overflow");
     if (rep) {
         rep->gpu_seconds = ms * 1e-3;
         rep->gemm_rows = Me;

@@ -328,140 +328,220 @@ template <int N, int W> __global__ void __launch_bounds__(256) dot_window_kernel
     store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, r);
 }
 
-// Exact dot product for terms spread beyond any fixed window (the cluster algorithm of
-// linalg.hpp:298-317): terms in descending exponent order form clusters (a term joins while its top bit
-// lies within gap = 32N + ceil(log2(K+1)) + 4 bits of the cluster's lowest bit); the first nonzero
-// cluster sum S is exact and only the sign of the next nonzero cluster matters, so the result is
-// RN(S * 2^D +/- 1) * 2^-D with D = 32N + 64.  Clusters are found by repeated passes over the terms
-// (no sort); a cluster wider than WC words sets *error.  The update's old value is one more term.
-template <int N, int WC> __global__ void __launch_bounds__(256) dot_cluster_kernel(DotArgs g)
-{
-    const u32 idx = blockIdx.x * 256u + threadIdx.x;
-    if (idx >= g.count) return;
+// Exact dot product for terms spread beyond any fixed window: the cluster algorithm of
+// linalg.hpp:298-317.  Terms in descending exponent order form clusters (a term joins while its top bit
+// lies within gap = 32N + ceil(log2(K+1)) + 4 bits of the cluster's lowest bit); the first nonzero cluster
+// sum S is exact and only the sign of the next nonzero cluster matters, so the result is
+// RN(S * 2^D +/- 1) * 2^-D with D = 32N + 64.  Clusters are found by repeated passes over the term
+// exponents (no sort).  cluster_size_kernel measures the widest cluster of every listed output;
+// cluster_sum_kernel then sums each cluster exactly in that output's own global scratch (separate
+// positive and negative accumulators, so carries stay short) and keeps only the top N+3 words of S and a
+// sticky unit for the rounding.  The update's old value is one more term.
+struct ClusterTerms {
+    const DotArgs& g;
     u32 i, j;
-    output_of(g, idx, i, j);
-    if (g.syrk && j > i) return;
-    const GemmLine* Li = g.lrows + i;
-    const GemmLine* Lj = g.rcols + j;
-    const u64 at = (u64)i * g.out_stride + j;
-    const bool mirror = g.mirror != 0;
-    Num<N> old = zero<N>(ST_OK);
-    if (g.subtract) old = load_out<N>(g.out, g.out_pitch, at);
-    if (Li->cls == 1 || Lj->cls == 1) {
-        u32 st = Li->status | Lj->status | (g.subtract ? old.st : 0u);
-        if (!g.subtract || st) store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, zero<N>(st));
-        return;
-    }
-    if (g.subtract && old.st) {
-        store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, zero<N>(old.st));
-        return;
-    }
-    const bool has_c = g.subtract && old.s;
-    const i64 width = 64 * N, gap = 32 * N + (i64)g.logk + 4;
-    const i64 NONE = -((i64)1 << 62);
-    // term k < K: product scale; k == K: the old value
-    auto scale_of = [&](u32 k, int& sg) -> i64 {
+    Num<1> cinfo; // sign and exponent of the old value (limbs not used)
+    bool has_c;
+    __device__ i64 scale_of(u32 k, int& sg, int N) const
+    {
         if (k == g.K) {
-            sg = old.s;
-            return (i64)old.e - (32 * N - 1);
+            sg = cinfo.s;
+            return (i64)cinfo.e - (32 * N - 1);
         }
         const u64 ia = (u64)i * g.left.ls + (u64)k * g.left.ks, ib = (u64)j * g.right.ls + (u64)k * g.right.ks;
         const int sa = (int)g.left.planes[(u64)(N + 1) * g.left.pitch + ia];
         const int sb = (int)g.right.planes[(u64)(N + 1) * g.right.pitch + ib];
         sg = g.subtract ? -sa * sb : sa * sb;
-        if (!sg) return NONE;
+        if (!sg) return -((i64)1 << 62);
         const int ea = (int)g.left.planes[(u64)N * g.left.pitch + ia];
         const int eb = (int)g.right.planes[(u64)N * g.right.pitch + ib];
         return (i64)ea + eb - 2 * (32 * N - 1);
-    };
-    const u32 terms = g.K + (has_c ? 1u : 0u);
-    i64 cutoff = ((i64)1 << 62);
-    int nsums = 0, sign1 = 0, sign2 = 0;
-    i64 scale1 = 0;
-    u32 S[WC];
-#pragma unroll
-    for (int w = 0; w < WC; ++w) S[w] = 0;
-    while (nsums < 2) {
-        // highest remaining term
-        i64 hi = NONE;
-        for (u32 k = 0; k < terms; ++k) {
+    }
+    __device__ u32 terms() const { return g.K + (has_c ? 1u : 0u); }
+    // next cluster below `cutoff`: its highest and lowest term exponents; false when none is left
+    __device__ bool next(i64 cutoff, i64& hi, i64& low, int N) const
+    {
+        const i64 width = 64 * N, gap = 32 * N + (i64)g.logk + 4, NONE = -((i64)1 << 62);
+        hi = NONE;
+        for (u32 k = 0; k < terms(); ++k) {
             int sg;
-            const i64 sc = scale_of(k, sg);
+            const i64 sc = scale_of(k, sg, N);
             if (sg && sc < cutoff && sc > hi) hi = sc;
         }
-        if (hi == NONE) break;
-        // close the cluster downwards
-        i64 low = hi;
+        if (hi == NONE) return false;
+        low = hi;
         for (bool changed = true; changed;) {
             changed = false;
-            for (u32 k = 0; k < terms; ++k) {
+            for (u32 k = 0; k < terms(); ++k) {
                 int sg;
-                const i64 sc = scale_of(k, sg);
+                const i64 sc = scale_of(k, sg, N);
                 if (sg && sc < low && sc + width - 1 + gap >= low) {
                     low = sc;
                     changed = true;
                 }
             }
         }
-        const i64 words = (hi + width - low) / 32 + 3;
-        if (words > WC) {
-            atomicOr(g.error, 1);
-            return;
-        }
-        u32 acc[WC];
+        return true;
+    }
+};
+
+// Trivial outputs and updates of status entries; true when the output is finished.
+template <int N> __device__ __forceinline__ bool dot_prologue(const DotArgs& g, u32 i, u32 j, Num<N>& old)
+{
+    const GemmLine* Li = g.lrows + i;
+    const GemmLine* Lj = g.rcols + j;
+    const bool mirror = g.mirror != 0;
+    old = zero<N>(ST_OK);
+    if (g.subtract) old = load_out<N>(g.out, g.out_pitch, (u64)i * g.out_stride + j);
+    if (Li->cls == 1 || Lj->cls == 1) {
+        u32 st = Li->status | Lj->status | (g.subtract ? old.st : 0u);
+        if (!g.subtract || st) store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, zero<N>(st));
+        return true;
+    }
+    if (g.subtract && old.st) {
+        store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, zero<N>(old.st));
+        return true;
+    }
+    return false;
+}
+
+// words[idx] = accumulator words of the widest cluster of listed output idx (0 if trivial)
+template <int N> __global__ void __launch_bounds__(256) cluster_size_kernel(DotArgs g, u32* words)
+{
+    const u32 idx = blockIdx.x * 256u + threadIdx.x;
+    if (idx >= g.count) return;
+    u32 i, j;
+    output_of(g, idx, i, j);
+    words[idx] = 0;
+    if (g.syrk && j > i) return;
+    const GemmLine* Li = g.lrows + i;
+    const GemmLine* Lj = g.rcols + j;
+    if (Li->cls == 1 || Lj->cls == 1) return;
+    Num<1> c = zero<1>(ST_OK);
+    bool has_c = false;
+    if (g.subtract) {
+        const u64 at = (u64)i * g.out_stride + j;
+        c.e = (int)g.out[(u64)N * g.out_pitch + at];
+        c.s = (int)g.out[(u64)(N + 1) * g.out_pitch + at];
+        has_c = c.s != 0;
+    }
+    const ClusterTerms T{g, i, j, c, has_c};
+    i64 cutoff = ((i64)1 << 62), hi, low;
+    i64 widest = 0;
+    while (T.next(cutoff, hi, low, N)) {
+        const i64 w = (hi + 64 * N - low) / 32 + 3;
+        widest = w > widest ? w : widest;
+        cutoff = low;
+    }
+    words[idx] = (u32)(widest < 0xffffffffll ? widest : 0xffffffffll);
+}
+
+// acc[0..W) += x (S words) * 2^off, off >= 0, in global memory; carries propagate as far as needed
+__device__ __forceinline__ void global_add(u32* acc, u64 W, const u32* x, int S, i64 off)
+{
+    const u64 ws = (u64)(off >> 5);
+    const u32 bs = (u32)(off & 31);
+    u64 carry = 0;
+    for (int t = 0; t <= S; ++t) {
+        const u32 hi = t < S ? x[t] : 0u, lo = t ? x[t - 1] : 0u;
+        const u32 v = bs ? __builtin_amdgcn_alignbit(hi, lo, 32u - bs) : hi;
+        const u64 at = ws + (u64)t;
+        if (at >= W) break;
+        const u64 s = (u64)acc[at] + v + carry;
+        acc[at] = (u32)s;
+        carry = s >> 32;
+    }
+    for (u64 at = ws + (u64)S + 1; carry && at < W; ++at) {
+        const u64 s = (u64)acc[at] + carry;
+        acc[at] = (u32)s;
+        carry = s >> 32;
+    }
+}
+
+template <int N> __global__ void __launch_bounds__(256) cluster_sum_kernel(DotArgs g, const u32* words, const u64* offsets, u32* scratch)
+{
+    const u32 idx = blockIdx.x * 256u + threadIdx.x;
+    if (idx >= g.count) return;
+    u32 i, j;
+    output_of(g, idx, i, j);
+    if (g.syrk && j > i) return;
+    Num<N> old;
+    if (dot_prologue<N>(g, i, j, old)) return;
+    Num<1> c = zero<1>(ST_OK);
+    c.e = old.e;
+    c.s = old.s;
+    const ClusterTerms T{g, i, j, c, g.subtract && old.s != 0};
+    const u64 W = words[idx];
+    u32* pos = scratch + offsets[idx];
+    u32* neg = pos + W;
+    constexpr int KW = N + 4; // kept words of S: a sticky word, then the top N+3 words
+    u32 S[KW];
 #pragma unroll
-        for (int w = 0; w < WC; ++w) acc[w] = 0;
-        for (u32 k = 0; k < terms; ++k) {
+    for (int w = 0; w < KW; ++w) S[w] = 0;
+    i64 scale1 = 0, cutoff = ((i64)1 << 62), hi, low;
+    int nsums = 0, sign1 = 0, sign2 = 0;
+    while (nsums < 2 && T.next(cutoff, hi, low, N)) {
+        for (u64 w = 0; w < 2 * W; ++w) pos[w] = 0;
+        for (u32 k = 0; k < T.terms(); ++k) {
             int sg;
-            const i64 sc = scale_of(k, sg);
+            const i64 sc = T.scale_of(k, sg, N);
             if (!sg || sc < low || sc >= cutoff) continue;
-            u32 w[WC];
-            if (k == g.K) {
-                place<WC, N>(w, old.w, sc - low);
-            } else {
+            u32* dst = sg > 0 ? pos : neg;
+            if (k == g.K) global_add(dst, W, old.w, N, sc - low);
+            else {
                 const Num<N> a = load_view<N>(g.left, i, k);
                 const Num<N> b = load_view<N>(g.right, j, k);
                 u32 p[2 * N];
                 mantissa_product<N>(a.w, b.w, p);
-                place<WC, 2 * N>(w, p, sc - low);
-            }
-            accumulate_signed<WC>(acc, w, sg);
-        }
-        int sign = 1;
-        if (acc[WC - 1] >> 31) {
-            sign = -1;
-            u32 cy = 1;
-#pragma unroll
-            for (int w = 0; w < WC; ++w) {
-                const u64 v = (u64)(u32)~acc[w] + cy;
-                acc[w] = (u32)v;
-                cy = (u32)(v >> 32);
+                global_add(dst, W, p, 2 * N, sc - low);
             }
         }
-        bool nonzero = false;
-#pragma unroll
-        for (int w = 0; w < WC; ++w) nonzero = nonzero || acc[w] != 0;
-        if (nonzero) {
+        // sign and magnitude of pos - neg
+        int cmp = 0;
+        for (u64 w = W; w-- > 0;)
+            if (pos[w] != neg[w]) {
+                cmp = pos[w] > neg[w] ? 1 : -1;
+                break;
+            }
+        if (cmp) {
             if (!nsums) {
+                u32* big = cmp > 0 ? pos : neg;
+                const u32* small = cmp > 0 ? neg : pos;
+                u32 bw = 0;
+                u64 top = 0;
+                for (u64 w = 0; w < W; ++w) {
+                    const u64 v = (u64)big[w] - small[w] - bw;
+                    big[w] = (u32)v;
+                    bw = (u32)(v >> 63);
+                    if (big[w]) top = w;
+                }
+                // keep words [top-(N+2), top] and a sticky word for everything below
+                const i64 first = (i64)top - (N + 2);
+                bool sticky = false;
+                for (i64 w = 0; w < first; ++w) sticky = sticky || big[w] != 0;
+                S[0] = sticky ? 1u : 0u;
 #pragma unroll
-                for (int w = 0; w < WC; ++w) S[w] = acc[w];
-                scale1 = low;
-                sign1 = sign;
+                for (int t = 0; t < N + 3; ++t) {
+                    const i64 w = first + t;
+                    S[t + 1] = (w >= 0 && w < (i64)W) ? big[w] : 0u;
+                }
+                scale1 = low + 32 * (first - 1);
+                sign1 = cmp;
             } else
-                sign2 = sign;
+                sign2 = cmp;
             ++nsums;
         }
         cutoff = low;
     }
     Num<N> r;
     if (!nsums) r = zero<N>(ST_OK);
-    else if (nsums == 1) r = pack<N, WC>(S, scale1, sign1, ST_OK);
+    else if (nsums == 1) r = pack<N, KW>(S, scale1, sign1, ST_OK);
     else {
         // S * 2^D -/+ 1 puts the sticky unit below the rounding position of S
-        constexpr int DW = N + 2, EW = WC + DW + 1;
+        constexpr int DW = N + 2, EW = KW + DW;
         u32 ext[EW];
 #pragma unroll
-        for (int w = 0; w < EW; ++w) ext[w] = (w >= DW && w - DW < WC) ? S[w - DW] : 0u;
+        for (int w = 0; w < EW; ++w) ext[w] = w >= DW ? S[w - DW] : 0u;
         if (sign2 == sign1) ext[0] = 1;
         else {
             u32 bw = 1;
@@ -474,7 +554,7 @@ template <int N, int WC> __global__ void __launch_bounds__(256) dot_cluster_kern
         }
         r = pack<N, EW>(ext, scale1 - 32 * DW, sign1, ST_OK);
     }
-    store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, r);
+    store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, g.mirror != 0, r);
 }
 
 } // namespace lf

@@ -29,13 +29,29 @@ template <int N> hipError_t launch_dot(int window_class, const DotArgs& g, hipSt
     if (!g.count) return hipSuccess;
     const dim3 grid((g.count + 255u) / 256u), block(256);
     if (window_class == 0) hipLaunchKernelGGL((dot_window_kernel<N, 64>), grid, block, 0, s, g);
-    else if (window_class == 1) hipLaunchKernelGGL((dot_window_kernel<N, 128>), grid, block, 0, s, g);
-    else hipLaunchKernelGGL((dot_cluster_kernel<N, 128>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((dot_window_kernel<N, 128>), grid, block, 0, s, g);
+    return hipGetLastError();
+}
+
+template <int N> hipError_t launch_cluster_size(const DotArgs& g, uint32_t* words, hipStream_t s)
+{
+    if (!g.count) return hipSuccess;
+    hipLaunchKernelGGL((cluster_size_kernel<N>), dim3((g.count + 255u) / 256u), dim3(256), 0, s, g, words);
+    return hipGetLastError();
+}
+
+template <int N>
+hipError_t launch_cluster_sum(const DotArgs& g, const uint32_t* words, const uint64_t* offsets, uint32_t* scratch, hipStream_t s)
+{
+    if (!g.count) return hipSuccess;
+    hipLaunchKernelGGL((cluster_sum_kernel<N>), dim3((g.count + 255u) / 256u), dim3(256), 0, s, g, words, offsets, scratch);
     return hipGetLastError();
 }
 
 template hipError_t launch_digits<LF_WORDS>(const DigitsArgs&, hipStream_t);
 template hipError_t launch_rebuild<LF_WORDS>(int, const RebuildArgs&, hipStream_t);
 template hipError_t launch_dot<LF_WORDS>(int, const DotArgs&, hipStream_t);
+template hipError_t launch_cluster_size<LF_WORDS>(const DotArgs&, uint32_t*, hipStream_t);
+template hipError_t launch_cluster_sum<LF_WORDS>(const DotArgs&, const uint32_t*, const uint64_t*, uint32_t*, hipStream_t);
 
 } // namespace lf
