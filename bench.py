@@ -125,6 +125,73 @@ def cpu_baseline(x, f, budget_s: float):
     return res
 
 
+def gemm_inputs(bits: int, n: int, seed: int):
+    """Config-3 entries: random sign, uniform limbs with the top bit set, exponent c + round(8 z) with c uniform
+    in {-200, 0, +200} per entry and z standard normal, seeded."""
+    w = bits // 32
+    rng = np.random.default_rng(seed)
+
+    def mat():
+        r = np.zeros((n * n, w + 3), dtype=np.uint32)
+        r[:, :w] = rng.integers(0, 1 << 32, size=(n * n, w), dtype=np.uint64).astype(np.uint32)
+        r[:, w - 1] |= np.uint32(0x80000000)
+        e = rng.choice([-200, 0, 200], size=n * n) + np.round(8 * rng.standard_normal(n * n)).astype(np.int64)
+        r[:, w] = (e & 0xFFFFFFFF).astype(np.uint32)
+        r[:, w + 1] = (np.where(rng.integers(0, 2, size=n * n) == 1, 1, -1) & 0xFFFFFFFF).astype(np.uint32)
+        return r
+
+    return mat(), mat()
+
+
+def gemm_section(reps: int, oracle_samples: int):
+    """512-bit exact GEMM C = A B at M = N = K = 512 (three exponent bands per line): the int8-MFMA residue
+    GEMM (default options and the one-band lever band_bits = 512) against the plain on-GPU limb GEMM; all
+    three bit-identical, a sample of outputs checked against the oracle."""
+    from limbforge_amd import capi
+
+    bits, n = 512, 512
+    A, B = gemm_inputs(bits, n, 20261007 + bits)
+    res = {}
+    outs = {}
+    for name, kw in (("residue", {}), ("residue_band512", {"band_bits": 512}), ("limb", {"method": 1})):
+        o = capi.LinalgOptions.default(**kw)
+        times, walls = [], []
+        r = rep = None
+        for i in range(reps if name != "limb" else max(2, reps // 3)):
+            r, rep, g, w = capi.gemm(bits, False, A, B, n, n, n, options=o)
+            if i:  # first call warms
+                times.append(g)
+                walls.append(w)
+        outs[name] = r
+        res[name] = {"device_ms": round(1e3 * float(np.median(times)), 4), "wall_ms": round(1e3 * float(np.median(walls)), 3),
+                     "moduli": int(rep.moduli), "int8_gemms": int(rep.int8_gemms), "gemm_rows": int(rep.gemm_rows),
+                     "band_pairs": int(rep.npairs), "fallback_outputs": int(rep.fallback_outputs)}
+        log(f"[bench] gemm {name}: {res[name]}")
+    same = all((outs[k] == outs["residue"]).all() for k in outs)
+    check = None
+    if oracle_samples:
+        from oracle import lf_oracle
+
+        rng = np.random.default_rng(1)
+        rows = rng.choice(n, size=max(1, oracle_samples // n + 1), replace=False)
+        bad = 0
+        for i in rows:
+            want = lf_oracle.gemm(bits, False, A[i * n:(i + 1) * n], B, 1, n, n)
+            bad += int((outs["residue"][i * n:(i + 1) * n] != want).any(axis=1).sum())
+        check = {"rows": len(rows), "outputs": len(rows) * n, "mismatches": bad}
+    return {
+        "workload": "512-bit exact GEMM C = A B, M = N = K = 512, exponents in bands -200/0/+200 (sigma 8) (config 3)",
+        "residue_ms": res["residue"]["device_ms"],
+        "limb_ms": res["limb"]["device_ms"],
+        "speedup_residue_vs_limb": round(res["limb"]["device_ms"] / res["residue"]["device_ms"], 2),
+        "speedup_band512_vs_limb": round(res["limb"]["device_ms"] / res["residue_band512"]["device_ms"], 2),
+        "bit_identical_all_paths": bool(same),
+        "oracle_check": check,
+        "paths": res,
+        "timing": "device time of the product kernels (HIP events), median; wall includes host transfers and band analysis",
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +200,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU work for the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-gemm", action="store_true")
+    ap.add_argument("--gemm-reps", type=int, default=6)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})", file=sys.stderr)
@@ -195,6 +264,11 @@ def main(argv=None) -> int:
         parity = {"checked_values": COUNT, "mismatches": mism, "against": "CPU oracle (MPFR-pinned)"}
         log(f"[bench] parity: {mism} mismatching values of {COUNT}")
 
+    gemm = None
+    if not a.no_gemm:
+        log("[bench] 512-bit exact GEMM 512^3")
+        gemm = gemm_section(a.gemm_reps, 0 if a.no_parity else 256)
+
     base = None
     if not a.no_cpu_baseline:
         log("[bench] timing the CPU baseline")
@@ -219,6 +293,7 @@ def main(argv=None) -> int:
         "roofline": roofline,
         "cpu_baseline": base,
         "parity": parity,
+        "gemm": gemm,
     }
     print(json.dumps(line), flush=True)
     return 0

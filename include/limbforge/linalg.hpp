@@ -1,0 +1,107 @@
+#pragma once
+// Exact dense products of the LimbForge/WolfNum API on AMD Instinct MI355X: C[i][j] = RN(sum_k L[i][k] R[k][j])
+// or RN(C[i][j] - sum ...) as an update, one rounding per output (ties to even), for any K, exponent spread
+// or cancellation.  The public surface follows the reference's include/limbforge/linalg.hpp for the products
+// (LinalgOptions, LinalgReport, Linalg::gemm / syrk and their *_exact names, report(), options(),
+// exact_dot_add / exact_dot).  Behind it are the C ABI entries lf_gemm_host / lf_syrk_host of
+// liblimbforge_hip.so: exponent-band residue GEMMs on int8 MFMA with CRT reconstruction, and exact-dot
+// kernels for lines outside the bands (include/limbforge_hip.h).  Statuses propagate: an output whose row of
+// L or column of R carries a status is zero with the OR of those statuses (and the old entry's for an update).
+// Empty K gives canonical zero (an update leaves C unchanged).
+#include "core.hpp"
+#include "engine.hpp"
+
+#include <cstddef>
+#include <cstdint>
+#include <future>
+#include <memory>
+#include <vector>
+
+namespace limbforge {
+struct LinalgOptions {
+    int band_bits = 64;        // G: a band holds exponents [top-G, top] of one row/column
+    int max_bands = 4;         // rows/columns needing more bands use the exact dot kernels
+    int max_spread = 512;      // ... as do rows/columns whose nonzero exponents span more than this
+    unsigned host_threads = 0; // band-analysis threads; 0 = hardware concurrency
+    bool force_fallback = false; // every output through the exact dot kernels (testing)
+    int resident_bands = 2;
+};
+struct LinalgPrewarm {
+    std::vector<int> bits;
+    std::vector<std::size_t> inner_sizes;
+    bool resident = false;
+};
+struct LinalgPair {
+    unsigned left_band, right_band, rows, cols, moduli;
+};
+// Diagnostics of the last call. "Lines" are rows of the left operand and columns of the right operand.
+struct LinalgReport {
+    std::size_t m = 0, n = 0, k = 0;
+    std::vector<std::size_t> left_bands, right_bands; // [b] = lines with exactly b bands (index 0: zero or status lines)
+    std::size_t left_fallback = 0, right_fallback = 0;
+    std::size_t gpu_outputs = 0, multi_band_outputs = 0, fallback_outputs = 0, trivial_outputs = 0;
+    std::vector<LinalgPair> pairs;
+    std::size_t gemm_rows = 0, gemm_cols = 0, int8_gemms = 0;
+    bool zero_copy_output = false;
+    bool resident_overflow = false;
+    double analysis_seconds = 0, upload_seconds = 0, gpu_seconds = 0, assembly_seconds = 0, fallback_seconds = 0;
+};
+
+// One Linalg per host thread.  C must not overlap A or B.  Calls return after the GPU has finished.
+class Linalg {
+public:
+    explicit Linalg(LinalgOptions options = {});
+    explicit Linalg(Engine& engine, LinalgOptions options = {});
+    ~Linalg();
+    Linalg(const Linalg&) = delete;
+    Linalg& operator=(const Linalg&) = delete;
+    void prewarm(const LinalgPrewarm&);
+    std::future<void> prewarm_async(LinalgPrewarm);
+    std::string device_name() const;
+    // C (cols x cols) = A^T A for A (rows x cols).  lower_only writes C[i][j] for i >= j and leaves the rest
+    // untouched; otherwise the full symmetric matrix.  With subtract, C[i][j] = RN(C[i][j] - (A^T A)[i][j])
+    // (lower triangle only: subtract requires lower_only).
+    Timing syrk(int bits, const void* A, std::size_t rows, std::size_t cols, void* C, bool lower_only = true, bool subtract = false);
+    // C (m x n) = op(A) B with B (k x n); op(A) = A (m x k) or, with transpose_a, A^T for A (k x m).
+    // With subtract, C = RN(C - op(A) B), one rounding per entry.
+    Timing gemm(int bits, bool transpose_a, const void* A, const void* B, std::size_t m, std::size_t n, std::size_t k, void* C,
+                bool subtract = false);
+    const LinalgReport& report() const;
+    const LinalgOptions& options() const;
+    Timing gemm_exact(int bits, bool transpose_a, const void* A, const void* B, std::size_t m, std::size_t n, std::size_t k, void* C,
+                      bool subtract = false)
+    {
+        return gemm(bits, transpose_a, A, B, m, n, k, C, subtract);
+    }
+    Timing syrk_exact(int bits, const void* A, std::size_t rows, std::size_t cols, void* C, bool lower_only = true, bool subtract = false)
+    {
+        return syrk(bits, A, rows, cols, C, lower_only, subtract);
+    }
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl;
+};
+
+namespace detail {
+// One exact dot product on the GPU: RN(c -/+ sum a[k*sa] b[k*sb]) (c may be null).
+void exact_dot_device(int bits, const void* c, bool subtract, const void* a, std::ptrdiff_t sa, const void* b, std::ptrdiff_t sb,
+                      std::size_t K, void* out);
+} // namespace detail
+
+// RN(c - sum_k a[k*sa]*b[k*sb]) (subtract) or RN(c + sum ...) with one rounding; c == nullptr is a zero
+// addend (the reference's linalg.hpp:268-318 contract, computed on the GPU by the exact-dot kernels).
+template <int N>
+Number<N> exact_dot_add(const Number<N>* c, bool subtract, const Number<N>* a, std::ptrdiff_t sa, const Number<N>* b, std::ptrdiff_t sb,
+                        std::size_t K)
+{
+    Number<N> r;
+    detail::exact_dot_device(32 * N, c, subtract, a, sa, b, sb, K, &r);
+    return r;
+}
+// RN(sum_k a[k*sa]*b[k*sb]) with one rounding; empty K and exact cancellation give canonical zero.
+template <int N> Number<N> exact_dot(const Number<N>* a, std::ptrdiff_t sa, const Number<N>* b, std::ptrdiff_t sb, std::size_t K)
+{
+    return exact_dot_add<N>(nullptr, false, a, sa, b, sb, K);
+}
+} // namespace limbforge

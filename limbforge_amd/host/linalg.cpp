@@ -1,0 +1,156 @@
+// linalg.cpp — the C++17 Linalg facade over lf_gemm_host / lf_syrk_host (include/limbforge_hip.h).
+// Option validation, report translation and the error classes follow the reference's src/linalg.mm:255-307.
+#include "limbforge/linalg.hpp"
+#include "engine_internal.hpp"
+
+#include <cstring>
+#include <stdexcept>
+#include <thread>
+
+namespace limbforge {
+
+struct Linalg::Impl {
+    LinalgOptions options;
+    LinalgReport report;
+    lf_linalg_options c_options() const
+    {
+        lf_linalg_options o;
+        lf_linalg_default_options(&o);
+        o.band_bits = options.band_bits;
+        o.max_bands = options.max_bands;
+        o.max_spread = options.max_spread;
+        o.host_threads = options.host_threads;
+        o.force_fallback = options.force_fallback;
+        o.resident_bands = options.resident_bands;
+        return o;
+    }
+    void take(const lf_linalg_report& r)
+    {
+        LinalgReport out;
+        out.m = r.m;
+        out.n = r.n;
+        out.k = r.k;
+        out.left_bands.assign(r.left_bands, r.left_bands + r.band_slots);
+        out.right_bands.assign(r.right_bands, r.right_bands + r.band_slots);
+        out.left_fallback = r.left_fallback;
+        out.right_fallback = r.right_fallback;
+        out.gpu_outputs = r.gpu_outputs;
+        out.multi_band_outputs = r.multi_band_outputs;
+        out.fallback_outputs = r.fallback_outputs;
+        out.trivial_outputs = r.trivial_outputs;
+        for (uint32_t i = 0; i < r.npairs; ++i)
+            out.pairs.push_back({r.pairs[i].left_band, r.pairs[i].right_band, r.pairs[i].rows, r.pairs[i].cols, r.pairs[i].moduli});
+        out.gemm_rows = r.gemm_rows;
+        out.gemm_cols = r.gemm_cols;
+        out.int8_gemms = r.int8_gemms;
+        out.analysis_seconds = r.analysis_seconds;
+        out.upload_seconds = r.upload_seconds;
+        out.gpu_seconds = r.gpu_seconds;
+        out.assembly_seconds = r.assembly_seconds;
+        out.fallback_seconds = r.fallback_seconds;
+        report = std::move(out);
+    }
+};
+
+static void check_bits(int bits)
+{
+    if (bits < 64 || bits > 1024 || bits % 32) throw std::invalid_argument("bits must be a multiple of 32 in [64,1024]");
+}
+
+Linalg::Linalg(LinalgOptions options) : impl(std::make_unique<Impl>())
+{
+    if (options.band_bits < 0 || options.band_bits > 1024 || options.max_bands < 0 || options.max_spread < 0 || options.max_spread > 8192)
+        throw std::invalid_argument("linalg options: band_bits in [0,1024], max_bands >= 0, max_spread in [0,8192]");
+    int devices = 0;
+    lf_device_count(&devices);
+    if (!devices) throw std::runtime_error("no AMD GPU available");
+    impl->options = options;
+}
+
+Linalg::Linalg(Engine&, LinalgOptions options) : Linalg(options) {}
+Linalg::~Linalg() = default;
+
+void Linalg::prewarm(const LinalgPrewarm& r)
+{
+    for (int bits : r.bits) check_bits(bits);
+    for (auto k : r.inner_sizes)
+        if (k > 65472) throw std::invalid_argument("linalg prewarm: resident K exceeds 65472");
+}
+
+std::future<void> Linalg::prewarm_async(LinalgPrewarm r)
+{
+    prewarm(r);
+    std::promise<void> done;
+    done.set_value();
+    return done.get_future();
+}
+
+std::string Linalg::device_name() const
+{
+    char buf[256] = {};
+    detail::check(lf_device_name(buf, sizeof buf), "device_name");
+    return buf;
+}
+
+const LinalgReport& Linalg::report() const { return impl->report; }
+const LinalgOptions& Linalg::options() const { return impl->options; }
+
+Timing Linalg::syrk(int bits, const void* A, std::size_t rows, std::size_t cols, void* C, bool lower_only, bool subtract)
+{
+    check_bits(bits);
+    if (subtract && !lower_only) throw std::invalid_argument("syrk: subtract updates the lower triangle only (lower_only)");
+    lf_linalg_options o = impl->c_options();
+    lf_linalg_report r;
+    Timing t{0, 0};
+    detail::check(lf_syrk_host(bits, A, rows, cols, C, lower_only, subtract, &o, &r, &t.gpu_seconds, &t.wall_seconds), "Linalg::syrk");
+    impl->take(r);
+    return t;
+}
+
+Timing Linalg::gemm(int bits, bool transpose_a, const void* A, const void* B, std::size_t m, std::size_t n, std::size_t k, void* C,
+                    bool subtract)
+{
+    check_bits(bits);
+    lf_linalg_options o = impl->c_options();
+    lf_linalg_report r;
+    Timing t{0, 0};
+    detail::check(lf_gemm_host(bits, transpose_a, A, B, m, n, k, C, subtract, &o, &r, &t.gpu_seconds, &t.wall_seconds), "Linalg::gemm");
+    impl->take(r);
+    return t;
+}
+
+namespace detail {
+void exact_dot_device(int bits, const void* c, bool subtract, const void* a, std::ptrdiff_t sa, const void* b, std::ptrdiff_t sb,
+                      std::size_t K, void* out)
+{
+    const std::size_t E = std::size_t(bits / 8 + 12);
+    // gather the strided operands into a 1 x K row and a K x 1 column
+    std::vector<unsigned char> row(std::max<std::size_t>(K, 1) * E), col(std::max<std::size_t>(K, 1) * E), cc(E, 0);
+    for (std::size_t k = 0; k < K; ++k) {
+        std::memcpy(&row[k * E], static_cast<const unsigned char*>(a) + std::ptrdiff_t(k) * sa * std::ptrdiff_t(E), E);
+        std::memcpy(&col[k * E], static_cast<const unsigned char*>(b) + std::ptrdiff_t(k) * sb * std::ptrdiff_t(E), E);
+    }
+    if (c) std::memcpy(cc.data(), c, E);
+    lf_linalg_options o;
+    lf_linalg_default_options(&o);
+    o.force_fallback = 1; // a single output: the exact dot kernels directly
+    if (!c) {
+        check(lf_gemm_host(bits, 0, row.data(), col.data(), 1, 1, K, cc.data(), 0, &o, nullptr, nullptr, nullptr), "exact_dot");
+    } else if (subtract) {
+        check(lf_gemm_host(bits, 0, row.data(), col.data(), 1, 1, K, cc.data(), 1, &o, nullptr, nullptr, nullptr), "exact_dot_add");
+    } else {
+        // c + sum = -( (-c) - sum )
+        int sign;
+        std::memcpy(&sign, &cc[E - 8], 4);
+        sign = -sign;
+        std::memcpy(&cc[E - 8], &sign, 4);
+        check(lf_gemm_host(bits, 0, row.data(), col.data(), 1, 1, K, cc.data(), 1, &o, nullptr, nullptr, nullptr), "exact_dot_add");
+        std::memcpy(&sign, &cc[E - 8], 4);
+        sign = -sign;
+        std::memcpy(&cc[E - 8], &sign, 4);
+    }
+    std::memcpy(out, cc.data(), E);
+}
+} // namespace detail
+
+} // namespace limbforge
