@@ -4,7 +4,7 @@
  *
  * The reference exposes this path only as a C++17 API (include/limbforge/engine.hpp and linalg.hpp);
  * it has no C ABI of its own.  Each entry point below names the reference interface it replaces, so a
- * C++ facade (this repository's include/limbforge/*.hpp) or any FFI (ctypes, cgo, JNI) can drive the
+ * C++ facade (the headers under this repository's include/limbforge/) or any FFI (ctypes, cgo, JNI) can drive the
  * same kernels.  The ABI uses plain pointers, sizes and integer status codes: no C++ types, no
  * exceptions.  Arithmetic errors are never reported here: they are status bits inside the results
  * (division by zero 1, exponent overflow 2, invalid 4), as in the reference (docs/numerics.md:14-31).
@@ -179,6 +179,50 @@ int lf_gemm_host(int bits, int transpose_a, const void* A, const void* B, uint64
  * C[i][j] for i >= j only; otherwise the full symmetric matrix.  subtract requires lower_only. */
 int lf_syrk_host(int bits, const void* A, uint64_t rows, uint64_t cols, void* C, int lower_only, int subtract,
                  const lf_linalg_options* options, lf_linalg_report* report, double* gpu_seconds, double* wall_seconds);
+
+/* ---- blocked Cholesky and triangular solves (linalg.hpp:73-87, 157-163; src/linalg.mm:411-486) ----
+ * The rounding sequence is fixed by `block` alone (docs/numerics.md:687-727): every entry takes
+ * blk(j) + 2 roundings.  FactorOptions' host_macs, solve_host_macs and gpu choose the reference's
+ * GPU/host placement of each update; placement never changes a bit, and here every step runs on the
+ * GPU, so they are accepted and not used. */
+typedef struct lf_factor_options {
+    uint64_t block;         /* column block nb (default 32); 0 = one block */
+    double host_macs;       /* default 5e4 */
+    double solve_host_macs; /* default 4e5 */
+    int gpu;                /* default 1 */
+} lf_factor_options;
+
+/* CholeskyInfo (linalg.hpp:82-87): pivot = n on success, else the first column p whose pivot
+ * s_pp = RN(a_pp - sum_k l_pk^2) is zero, negative or carries a status; then pivot_sign is its sign
+ * (0 for zero or a status) and pivot_status its status, columns j < p of L are final, and every lower
+ * entry with j >= p is zero with status invalid. */
+typedef struct lf_cholesky_info {
+    uint64_t pivot;
+    int pivot_sign;
+    uint32_t pivot_status;
+    double timing_gpu_seconds, timing_wall_seconds;
+    uint64_t blocks, gpu_updates, host_updates;
+    double panel_seconds, update_seconds;
+} lf_cholesky_info;
+
+void lf_factor_default_options(lf_factor_options* options);
+
+/* Linalg::cholesky on device operands (Linalg::cholesky(const Buffer&, n, Buffer&), linalg.hpp:196-197):
+ * L = lower factor of the lower triangle of A (n x n limb planes, element i*n + j); the strict upper
+ * triangle of L is set to zero; L may be A.  linalg: options of the update products (NULL = defaults).
+ * Returns after the device has finished. */
+int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, const lf_factor_options* options,
+                const lf_linalg_options* linalg, lf_cholesky_info* info, void* stream);
+/* Linalg::cholesky on host arrays (linalg.hpp:157-159). */
+int lf_cholesky_host(int bits, const void* A, uint64_t n, void* L, const lf_factor_options* options, const lf_linalg_options* linalg,
+                     lf_cholesky_info* info);
+/* Triangular solves with lower L (strict upper triangle not read), X and B n x nrhs: passes 1 = L X = B
+ * (Linalg::trsm), 2 = L^T X = B (trsm with transpose), 3 = both, forward first (Linalg::cholesky_solve);
+ * linalg.hpp:160-163, src/linalg.mm:459-486.  X may be B.  Device form and host-array form. */
+int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_operand* B, uint64_t nrhs, const lf_operand* X,
+             const lf_factor_options* options, const lf_linalg_options* linalg, double* gpu_seconds, double* wall_seconds, void* stream);
+int lf_solve_host(int bits, int passes, const void* L, uint64_t n, const void* B, uint64_t nrhs, void* X, const lf_factor_options* options,
+                  const lf_linalg_options* linalg, double* gpu_seconds, double* wall_seconds);
 
 /* ---- measurement support (no reference counterpart) ---- */
 /* Sustained v_mad_u64_u32 rate of the device (32x32->64 word products per second), measured with

@@ -359,3 +359,72 @@ def syrk(bits, A, rows, cols, C_in=None, lower_only=True, subtract=False, option
     check(L.lf_syrk_host(bits, A.ctypes.data, rows, cols, Cm.ctypes.data, int(lower_only), int(subtract),
                          C.byref(options) if options is not None else None, C.byref(rep), C.byref(g), C.byref(w)))
     return Cm, rep, g.value, w.value
+
+
+# ---------------------------------------------------------------- Cholesky and triangular solves
+class FactorOptions(C.Structure):
+    """FactorOptions (linalg.hpp:73-78)."""
+    _fields_ = [("block", C.c_uint64), ("host_macs", C.c_double), ("solve_host_macs", C.c_double), ("gpu", C.c_int)]
+
+    @classmethod
+    def default(cls, **kw):
+        o = cls()
+        L = lib()
+        _factor_sigs(L)
+        L.lf_factor_default_options(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
+class CholeskyInfo(C.Structure):
+    """CholeskyInfo (linalg.hpp:82-87)."""
+    _fields_ = [("pivot", C.c_uint64), ("pivot_sign", C.c_int), ("pivot_status", C.c_uint32),
+                ("timing_gpu_seconds", C.c_double), ("timing_wall_seconds", C.c_double),
+                ("blocks", C.c_uint64), ("gpu_updates", C.c_uint64), ("host_updates", C.c_uint64),
+                ("panel_seconds", C.c_double), ("update_seconds", C.c_double)]
+
+
+def _factor_sigs(L):
+    P = C.POINTER
+    if getattr(L, "_factor", False):
+        return
+    if not getattr(L, "_linalg", False):
+        _linalg_sigs(L)
+        L._linalg = True
+    L.lf_factor_default_options.argtypes = [P(FactorOptions)]
+    L.lf_factor_default_options.restype = None
+    L.lf_cholesky_host.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, P(FactorOptions), P(LinalgOptions), P(CholeskyInfo)]
+    L.lf_solve_host.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p, P(FactorOptions),
+                                P(LinalgOptions), P(C.c_double), P(C.c_double)]
+    L._factor = True
+
+
+def cholesky(bits, A, n, block=32, options=None, linalg=None):
+    """Linalg::cholesky (linalg.hpp:157-159) on host arrays: returns (L, CholeskyInfo)."""
+    L = lib()
+    _factor_sigs(L)
+    rw = bits // 32 + 3
+    A = np.ascontiguousarray(A, dtype=np.uint32).reshape(n * n, rw)
+    Lm = np.zeros((n * n, rw), np.uint32)
+    fo = options if options is not None else FactorOptions.default(block=block)
+    info = CholeskyInfo()
+    check(L.lf_cholesky_host(bits, A.ctypes.data, n, Lm.ctypes.data, C.byref(fo),
+                             C.byref(linalg) if linalg is not None else None, C.byref(info)))
+    return Lm, info
+
+
+def solve(bits, passes, Lf, n, B, nrhs, block=32, options=None, linalg=None):
+    """Linalg::trsm (passes 1: L X = B, 2: L^T X = B) and cholesky_solve (passes 3), linalg.hpp:160-163, on
+    host arrays: returns (X, gpu_seconds, wall_seconds)."""
+    L = lib()
+    _factor_sigs(L)
+    rw = bits // 32 + 3
+    Lf = np.ascontiguousarray(Lf, dtype=np.uint32).reshape(n * n, rw)
+    B = np.ascontiguousarray(B, dtype=np.uint32).reshape(n * nrhs, rw)
+    X = np.zeros((n * nrhs, rw), np.uint32)
+    fo = options if options is not None else FactorOptions.default(block=block)
+    g, w = C.c_double(), C.c_double()
+    check(L.lf_solve_host(bits, int(passes), Lf.ctypes.data, n, B.ctypes.data, nrhs, X.ctypes.data, C.byref(fo),
+                          C.byref(linalg) if linalg is not None else None, C.byref(g), C.byref(w)))
+    return X, g.value, w.value

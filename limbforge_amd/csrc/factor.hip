@@ -1,0 +1,412 @@
+// factor.hip — the blocked Cholesky factorization and triangular solves on MI355X behind lf_cholesky /
+// lf_solve and their host-array forms (the reference's Linalg::cholesky / trsm / cholesky_solve,
+// src/linalg.mm:411-486, docs/numerics.md:674-748).
+//
+// The work matrix stays in limb planes on the device for the whole call.  Per column block [k0, k1):
+//   1. the diagonal block, column by column in one workgroup (pivot, sqrt, then div of the entries below);
+//   2. the rows below it, one thread per row (chol_panel_kernel);
+//   3. one download of the panel's exponent / sign / status words (and the pivot record), then the trailing
+//      update a_ij = D(a_ij; l_i, l_j; [k0, k1)) for i >= j >= k1 as one SYRK product of the update GEMM
+//      (residue GEMM on int8 MFMA plus exact-dot lines, gemm.hip) written in place.
+// The reference splits step 3 into the next panel's columns and the rest and overlaps the rest with the
+// next panel on a second host thread; every entry is one rounding of the same exact value either way, so
+// one in-place product gives the same bits.  The solves run block by block the same way: the in-block
+// recurrence on one thread per right-hand side, the off-block updates as products of the update GEMM.
+#include "factor_launch.hpp"
+#include "gemm_plan.hpp"
+#include "limbforge_hip.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+void lf_set_last_error(const std::string& message);
+
+namespace {
+using Clock = std::chrono::steady_clock;
+
+int fail(int code, const std::string& m)
+{
+    lf_set_last_error(m);
+    return code;
+}
+
+#define FACTOR_HIP(call, where)                                                                                        \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) return fail(LF_RUNTIME_ERROR, std::string(where) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(std::size_t bytes)
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        return hipMalloc(&p, std::max<std::size_t>(bytes, 16));
+    }
+};
+
+struct OwnedStream {
+    hipStream_t s = nullptr;
+    ~OwnedStream()
+    {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+typedef hipError_t (*diag_fn)(const lf::FactorArgs&, uint32_t, hipStream_t);
+typedef hipError_t (*panel_fn)(const lf::FactorArgs&, hipStream_t);
+typedef hipError_t (*invalid_fn)(uint32_t*, uint64_t, uint32_t, uint32_t, hipStream_t);
+typedef hipError_t (*solve_fn)(int, const lf::SolveArgs&, hipStream_t);
+struct FactorTables;
+template <int N> void fill_factor(FactorTables& t);
+struct FactorTables {
+    diag_fn diag[33] = {};
+    panel_fn panel[33] = {};
+    invalid_fn invalid[33] = {};
+    hipError_t (*zero_upper[33])(uint32_t*, uint64_t, uint32_t, hipStream_t) = {};
+    solve_fn solve[33] = {};
+    FactorTables() { fill_factor<2>(*this); }
+};
+template <int N> void fill_factor(FactorTables& t)
+{
+    t.diag[N] = lf::launch_chol_diag<N>;
+    t.panel[N] = lf::launch_chol_panel<N>;
+    t.invalid[N] = lf::launch_chol_invalid<N>;
+    t.zero_upper[N] = lf::launch_zero_upper<N>;
+    t.solve[N] = lf::launch_solve<N>;
+    if constexpr (N < 32) fill_factor<N + 1>(t);
+}
+const FactorTables& tables()
+{
+    static const FactorTables t;
+    return t;
+}
+
+bool valid_bits(int bits) { return bits >= 64 && bits <= 1024 && bits % 32 == 0; }
+
+// The exponent, sign and status words of a rows x cols region of an n-column plane matrix, from element
+// first, into host arrays (row-major, rows x cols).
+struct RegionFields {
+    std::vector<uint32_t> e, s, st;
+    int download(const uint32_t* planes, uint64_t pitch, int N, uint64_t first, uint64_t row_stride, std::size_t rows, std::size_t cols,
+                 hipStream_t stream)
+    {
+        e.assign(rows * cols, 0);
+        s.assign(rows * cols, 0);
+        st.assign(rows * cols, 0);
+        if (!rows || !cols) return LF_OK;
+        std::vector<uint32_t>* dst[3] = {&e, &s, &st};
+        for (int f = 0; f < 3; ++f)
+            FACTOR_HIP(hipMemcpy2DAsync(dst[f]->data(), cols * 4, planes + (uint64_t)(N + f) * pitch + first, row_stride * 4, cols * 4, rows,
+                                        hipMemcpyDeviceToHost, stream),
+                       "download fields");
+        return LF_OK;
+    }
+    lf::Fields fields(uint64_t ls, uint64_t ks) const { return {e.data(), s.data(), st.data(), 1, ls, ks}; }
+};
+
+// Copies the N+3 planes of `count` elements between operands of possibly different pitch.
+int copy_planes(int N, const lf_operand& from, const lf_operand& to, uint64_t count, hipStream_t s)
+{
+    if (from.planes == to.planes && from.pitch == to.pitch) return LF_OK;
+    if (from.pitch == to.pitch)
+        FACTOR_HIP(hipMemcpyAsync(to.planes, from.planes, (uint64_t)(N + 3) * from.pitch * 4, hipMemcpyDeviceToDevice, s), "copy planes");
+    else
+        FACTOR_HIP(hipMemcpy2DAsync(to.planes, to.pitch * 4, from.planes, from.pitch * 4, count * 4, N + 3, hipMemcpyDeviceToDevice, s),
+                   "copy planes");
+    return LF_OK;
+}
+
+int check_factor(int bits, uint64_t n, uint64_t nrhs, const lf_factor_options* fo, lf_factor_options& o, const lf_linalg_options* lo,
+                 lf_linalg_options& l)
+{
+    if (fo) o = *fo;
+    else lf_factor_default_options(&o);
+    if (lo) l = *lo;
+    else lf_linalg_default_options(&l);
+    if (int r = lf::check_linalg_options(&l)) return r;
+    if (!valid_bits(bits)) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    const uint64_t limit = uint64_t(1) << 31;
+    if (n >= limit || n * n >= limit || nrhs >= limit || n * nrhs >= limit)
+        return fail(LF_INVALID_ARGUMENT, "matrix dimensions exceed 32-bit indexing");
+    return LF_OK;
+}
+
+// One update product on the work matrices: C (m x nout at cbase, row stride cstride) -= left * right^T.
+int update(int bits, const lf_linalg_options& o, bool syrk, const lf::MatView& left, const lf::Fields& lf_, const lf::MatView& right,
+           const lf::Fields& rf_, std::size_t m, std::size_t nout, std::size_t K, uint32_t* cbase, uint64_t cpitch, uint64_t cstride,
+           hipStream_t s, float& ms)
+{
+    lf::Product P{};
+    P.bits = bits;
+    P.syrk = syrk;
+    P.lower_only = syrk;
+    P.subtract = true;
+    P.left = left;
+    P.right = syrk ? left : right;
+    P.lf_ = lf_;
+    P.rf_ = syrk ? lf_ : rf_;
+    P.m = m;
+    P.n = nout;
+    P.K = K;
+    P.C = {cbase, cpitch};
+    P.opt = o;
+    P.stream = s;
+    P.out_stride = cstride;
+    return lf::run_product(P, nullptr, &ms);
+}
+
+double since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+} // namespace
+
+extern "C" {
+
+void lf_factor_default_options(lf_factor_options* o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->block = 32;
+    o->host_macs = 5e4;
+    o->solve_host_macs = 4e5;
+    o->gpu = 1;
+}
+
+int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, const lf_factor_options* options,
+                const lf_linalg_options* linalg, lf_cholesky_info* info, void* stream)
+{
+    const auto start = Clock::now();
+    lf_factor_options fo;
+    lf_linalg_options lo;
+    if (int r = check_factor(bits, n, 0, options, fo, linalg, lo)) return r;
+    if (n && (!A || !L || !A->planes || !L->planes)) return fail(LF_INVALID_ARGUMENT, "null operand");
+    lf_cholesky_info ci;
+    std::memset(&ci, 0, sizeof ci);
+    ci.pivot = n;
+    if (!n) {
+        if (info) *info = ci;
+        return LF_OK;
+    }
+    const int N = bits / 32;
+    const hipStream_t s = (hipStream_t)stream;
+    const uint32_t n32 = uint32_t(n);
+    const uint64_t nb = fo.block ? std::min<uint64_t>(fo.block, n) : n;
+    uint32_t* W = (uint32_t*)L->planes;
+    const uint64_t pitch = L->pitch;
+    if (int r = copy_planes(N, *A, *L, n * n, s)) return r;
+    FACTOR_HIP(tables().zero_upper[N](W, pitch, n32, s), "zero upper launch");
+    // pivot record and per-thread cluster scratch (rows of a panel, or the diagonal block's threads)
+    DevBuf dinfo, dscratch;
+    const uint64_t swords = lf::factor_scratch_words(N, nb);
+    const uint64_t threads = std::max<uint64_t>(n, 256);
+    if (2 * swords * threads * 4 > (uint64_t(32) << 30))
+        return fail(LF_INVALID_ARGUMENT, "cholesky: the block is too large for the exact-dot scratch (use a smaller block)");
+    FACTOR_HIP(dinfo.alloc(sizeof(lf::FactorInfo)), "hipMalloc (pivot)");
+    FACTOR_HIP(dscratch.alloc(2 * swords * threads * 4), "hipMalloc (scratch)");
+    lf::FactorInfo hinfo{-1, 0, 0, 0};
+    FACTOR_HIP(hipMemcpyAsync(dinfo.p, &hinfo, sizeof hinfo, hipMemcpyHostToDevice, s), "upload pivot");
+    hipEvent_t e0, e1;
+    FACTOR_HIP(hipEventCreate(&e0), "hipEventCreate");
+    FACTOR_HIP(hipEventCreate(&e1), "hipEventCreate");
+    struct EventGuard {
+        hipEvent_t a, b;
+        ~EventGuard()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{e0, e1};
+    RegionFields panel;
+    for (uint64_t k0 = 0; k0 < n;) {
+        const uint64_t k1 = std::min(n, k0 + nb), w = k1 - k0;
+        auto t = Clock::now();
+        lf::FactorArgs f{W, pitch, n32, uint32_t(k0), uint32_t(k1), (lf::FactorInfo*)dinfo.p, (uint32_t*)dscratch.p, swords};
+        FACTOR_HIP(hipEventRecord(e0, s), "hipEventRecord");
+        FACTOR_HIP(tables().diag[N](f, uint32_t(std::min<uint64_t>(256, (w + 63) / 64 * 64)), s), "panel launch");
+        FACTOR_HIP(tables().panel[N](f, s), "panel launch");
+        FACTOR_HIP(hipEventRecord(e1, s), "hipEventRecord");
+        FACTOR_HIP(hipMemcpyAsync(&hinfo, dinfo.p, sizeof hinfo, hipMemcpyDeviceToHost, s), "download pivot");
+        if (int r = panel.download(W, pitch, N, k1 * n + k0, n, n - k1, w, s)) return r;
+        FACTOR_HIP(hipStreamSynchronize(s), "device execution (panel)");
+        float ms = 0;
+        FACTOR_HIP(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        ci.timing_gpu_seconds += ms * 1e-3;
+        ci.panel_seconds += since(t);
+        ++ci.blocks;
+        if (hinfo.error) return fail(LF_LOGIC_ERROR, "cholesky: exact-dot scratch overflow");
+        if (hinfo.pivot >= 0) {
+            ci.pivot = uint64_t(hinfo.pivot);
+            ci.pivot_sign = hinfo.sign;
+            ci.pivot_status = hinfo.status;
+            FACTOR_HIP(tables().invalid[N](W, pitch, n32, uint32_t(hinfo.pivot), s), "invalid launch");
+            FACTOR_HIP(hipStreamSynchronize(s), "device execution");
+            break;
+        }
+        if (k1 < n) {
+            t = Clock::now();
+            const lf::MatView v{W + (k1 * n + k0), pitch, n, 1};
+            if (int r = update(bits, lo, true, v, panel.fields(w, 1), v, panel.fields(w, 1), n - k1, n - k1, w, W + (k1 * n + k1), pitch, n,
+                               s, ms))
+                return r;
+            ci.timing_gpu_seconds += ms * 1e-3;
+            ci.update_seconds += since(t);
+            ++ci.gpu_updates;
+        }
+        k0 = k1;
+    }
+    ci.timing_wall_seconds = since(start);
+    if (info) *info = ci;
+    return LF_OK;
+}
+
+int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_operand* B, uint64_t nrhs, const lf_operand* X,
+             const lf_factor_options* options, const lf_linalg_options* linalg, double* gpu_seconds, double* wall_seconds, void* stream)
+{
+    const auto start = Clock::now();
+    if (gpu_seconds) *gpu_seconds = 0;
+    if (wall_seconds) *wall_seconds = 0;
+    lf_factor_options fo;
+    lf_linalg_options lo;
+    if (int r = check_factor(bits, n, nrhs, options, fo, linalg, lo)) return r;
+    if (passes < 1 || passes > 3) return fail(LF_INVALID_ARGUMENT, "solve: passes 1 (L X = B), 2 (L^T X = B) or 3 (both)");
+    if (!n || !nrhs) return LF_OK;
+    if (!L || !B || !X || !L->planes || !B->planes || !X->planes) return fail(LF_INVALID_ARGUMENT, "null operand");
+    const int N = bits / 32;
+    const hipStream_t s = (hipStream_t)stream;
+    const uint64_t nb = fo.block ? std::min<uint64_t>(fo.block, n) : n;
+    const uint32_t* F = (const uint32_t*)L->planes;
+    const uint64_t fpitch = L->pitch;
+    uint32_t* Xp = (uint32_t*)X->planes;
+    const uint64_t xpitch = X->pitch;
+    if (int r = copy_planes(N, *B, *X, n * nrhs, s)) return r;
+    DevBuf dscratch, derr;
+    const uint64_t swords = lf::factor_scratch_words(N, nb);
+    if (2 * swords * nrhs * 4 > (uint64_t(32) << 30))
+        return fail(LF_INVALID_ARGUMENT, "solve: the block is too large for the exact-dot scratch (use a smaller block)");
+    FACTOR_HIP(dscratch.alloc(2 * swords * nrhs * 4), "hipMalloc (scratch)");
+    FACTOR_HIP(derr.alloc(4), "hipMalloc");
+    FACTOR_HIP(hipMemsetAsync(derr.p, 0, 4, s), "hipMemsetAsync");
+    double gpu = 0;
+    RegionFields lf_, rf_;
+    auto block_solve = [&](int backward, uint64_t r0, uint64_t r1) {
+        lf::SolveArgs a{F, fpitch, Xp, xpitch, uint32_t(n), uint32_t(nrhs), uint32_t(r0), uint32_t(r1), (uint32_t*)dscratch.p, swords,
+                        (int32_t*)derr.p};
+        return tables().solve[N](backward, a, s);
+    };
+    float ms = 0;
+    if (passes & 1)
+        for (uint64_t r0 = 0; r0 < n; r0 += nb) {
+            const uint64_t r1 = std::min(n, r0 + nb), K = r1 - r0;
+            FACTOR_HIP(block_solve(0, r0, r1), "solve launch");
+            if (r1 == n) break;
+            // rows [r1, n) of X -= L[r1:, r0:r1] X[r0:r1, :]
+            if (int r = lf_.download(F, fpitch, N, r1 * n + r0, n, n - r1, K, s)) return r;
+            if (int r = rf_.download(Xp, xpitch, N, r0 * nrhs, nrhs, K, nrhs, s)) return r;
+            FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
+            const lf::MatView left{F + (r1 * n + r0), fpitch, n, 1}, right{Xp + r0 * nrhs, xpitch, 1, nrhs};
+            if (int r = update(bits, lo, false, left, lf_.fields(K, 1), right, rf_.fields(1, nrhs), n - r1, nrhs, K, Xp + r1 * nrhs, xpitch,
+                               nrhs, s, ms))
+                return r;
+            gpu += ms * 1e-3;
+        }
+    if (passes & 2)
+        for (uint64_t r0 = (n - 1) / nb * nb;; r0 -= nb) {
+            const uint64_t r1 = std::min(n, r0 + nb), K = r1 - r0;
+            FACTOR_HIP(block_solve(1, r0, r1), "solve launch");
+            if (!r0) break;
+            // rows [0, r0) of X -= L[r0:r1, :r0]^T X[r0:r1, :]
+            if (int r = lf_.download(F, fpitch, N, r0 * n, n, K, r0, s)) return r;
+            if (int r = rf_.download(Xp, xpitch, N, r0 * nrhs, nrhs, K, nrhs, s)) return r;
+            FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
+            const lf::MatView left{F + r0 * n, fpitch, 1, n}, right{Xp + r0 * nrhs, xpitch, 1, nrhs};
+            if (int r = update(bits, lo, false, left, lf_.fields(1, r0), right, rf_.fields(1, nrhs), r0, nrhs, K, Xp, xpitch, nrhs, s, ms))
+                return r;
+            gpu += ms * 1e-3;
+        }
+    int32_t err = 0;
+    FACTOR_HIP(hipMemcpyAsync(&err, derr.p, 4, hipMemcpyDeviceToHost, s), "download status");
+    FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
+    if (err) return fail(LF_LOGIC_ERROR, "solve: exact-dot scratch overflow");
+    if (gpu_seconds) *gpu_seconds = gpu;
+    if (wall_seconds) *wall_seconds = since(start);
+    return LF_OK;
+}
+
+// ---- host-array forms: upload, the device call on one stream, download ----
+int lf_cholesky_host(int bits, const void* A, uint64_t n, void* L, const lf_factor_options* options, const lf_linalg_options* linalg,
+                     lf_cholesky_info* info)
+{
+    const auto start = Clock::now();
+    lf_factor_options fo;
+    lf_linalg_options lo;
+    if (int r = check_factor(bits, n, 0, options, fo, linalg, lo)) return r;
+    if (n && (!A || !L)) return fail(LF_INVALID_ARGUMENT, "null input");
+    if (!n) {
+        if (info) {
+            std::memset(info, 0, sizeof *info);
+        }
+        return LF_OK;
+    }
+    const std::size_t E = bits / 8 + 12, count = n * n;
+    OwnedStream st;
+    FACTOR_HIP(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking), "hipStreamCreate");
+    DevBuf aos, planes;
+    FACTOR_HIP(aos.alloc(count * E), "hipMalloc (staging)");
+    FACTOR_HIP(planes.alloc(lf_planes_bytes(bits, 0, count)), "hipMalloc (work matrix)");
+    lf_operand W{planes.p, lf_plane_pitch(count)};
+    FACTOR_HIP(hipMemcpyAsync(aos.p, A, count * E, hipMemcpyHostToDevice, st.s), "upload A");
+    if (int r = lf_aos_to_planes(bits, 0, aos.p, &W, count, st.s)) return r;
+    lf_cholesky_info ci;
+    if (int r = lf_cholesky(bits, &W, n, &W, &fo, &lo, &ci, st.s)) return r;
+    if (int r = lf_planes_to_aos(bits, 0, &W, aos.p, count, st.s)) return r;
+    FACTOR_HIP(hipMemcpyAsync(L, aos.p, count * E, hipMemcpyDeviceToHost, st.s), "download L");
+    FACTOR_HIP(hipStreamSynchronize(st.s), "device execution (cholesky)");
+    ci.timing_wall_seconds = since(start);
+    if (info) *info = ci;
+    return LF_OK;
+}
+
+int lf_solve_host(int bits, int passes, const void* L, uint64_t n, const void* B, uint64_t nrhs, void* X, const lf_factor_options* options,
+                  const lf_linalg_options* linalg, double* gpu_seconds, double* wall_seconds)
+{
+    const auto start = Clock::now();
+    if (gpu_seconds) *gpu_seconds = 0;
+    if (wall_seconds) *wall_seconds = 0;
+    lf_factor_options fo;
+    lf_linalg_options lo;
+    if (int r = check_factor(bits, n, nrhs, options, fo, linalg, lo)) return r;
+    if (passes < 1 || passes > 3) return fail(LF_INVALID_ARGUMENT, "solve: passes 1 (L X = B), 2 (L^T X = B) or 3 (both)");
+    if (!n || !nrhs) return LF_OK;
+    if (!L || !B || !X) return fail(LF_INVALID_ARGUMENT, "null input");
+    const std::size_t E = bits / 8 + 12, fc = n * n, xc = n * nrhs;
+    OwnedStream st;
+    FACTOR_HIP(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking), "hipStreamCreate");
+    DevBuf aos, fplanes, xplanes;
+    FACTOR_HIP(aos.alloc(std::max(fc, xc) * E), "hipMalloc (staging)");
+    FACTOR_HIP(fplanes.alloc(lf_planes_bytes(bits, 0, fc)), "hipMalloc (factor)");
+    FACTOR_HIP(xplanes.alloc(lf_planes_bytes(bits, 0, xc)), "hipMalloc (right-hand sides)");
+    lf_operand F{fplanes.p, lf_plane_pitch(fc)}, Xo{xplanes.p, lf_plane_pitch(xc)};
+    FACTOR_HIP(hipMemcpyAsync(aos.p, L, fc * E, hipMemcpyHostToDevice, st.s), "upload L");
+    if (int r = lf_aos_to_planes(bits, 0, aos.p, &F, fc, st.s)) return r;
+    FACTOR_HIP(hipStreamSynchronize(st.s), "upload L");
+    FACTOR_HIP(hipMemcpyAsync(aos.p, B, xc * E, hipMemcpyHostToDevice, st.s), "upload B");
+    if (int r = lf_aos_to_planes(bits, 0, aos.p, &Xo, xc, st.s)) return r;
+    double g = 0;
+    if (int r = lf_solve(bits, passes, &F, n, &Xo, nrhs, &Xo, &fo, &lo, &g, nullptr, st.s)) return r;
+    if (int r = lf_planes_to_aos(bits, 0, &Xo, aos.p, xc, st.s)) return r;
+    FACTOR_HIP(hipMemcpyAsync(X, aos.p, xc * E, hipMemcpyDeviceToHost, st.s), "download X");
+    FACTOR_HIP(hipStreamSynchronize(st.s), "device execution (solve)");
+    if (gpu_seconds) *gpu_seconds = g;
+    if (wall_seconds) *wall_seconds = since(start);
+    return LF_OK;
+}
+
+} // extern "C"

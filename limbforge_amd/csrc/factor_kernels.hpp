@@ -1,0 +1,348 @@
+// factor_kernels.hpp — kernels of the blocked Cholesky factorization and the triangular solves on a
+// device-resident work matrix in limb planes (row-major element i*n + j).
+//
+// The rounding sequence is the reference's (docs/numerics.md:674-748, src/linalg.mm:411-486): with
+// D(c; x, y; K) = RN(c - sum_{k<K} x_k y_k) rounded once,
+//   panel [k0, k1): for every column j in order, s_jj = D(a_jj; l_j, l_j; [k0, j)), l_jj = sqrt(s_jj),
+//                   then l_ij = div(D(a_ij; l_i, l_j; [k0, j)), l_jj) for the rows i > j;
+//   the trailing updates by a block are products of the update GEMM (gemm.hip), not kernels here;
+//   forward solve in block [r0, r1): x_i = div(D(b_i; l_i, x; [r0, i)), l_ii), i ascending;
+//   backward solve in block [r0, r1): x_i = div(D(b_i; L[., i], x; (i, r1)), l_ii), i descending.
+// Each entry depends only on final entries, so computing a panel column by column (all rows of the
+// column in parallel) gives the bits of the reference's row-by-row host order.
+//
+// D is computed in one of two exact ways, chosen per call from the terms' exponents (same result):
+//   window    every exact 2N-word product and c added into one W-word two's-complement register window
+//   clusters  exponent clusters with a sticky sign (the reference's linalg.hpp:298-317), each cluster
+//             summed exactly in the thread's own global scratch (positive and negative accumulators)
+#pragma once
+#include "factor_launch.hpp"
+#include "gemm_kernels.hpp"
+
+namespace lf {
+
+// A vector of plane elements: element k at plane index base + k*stride.
+struct VecView {
+    const u32* planes;
+    u64 pitch;
+    i64 base, stride;
+};
+
+__device__ __forceinline__ u64 vec_at(const VecView& v, u32 k) { return (u64)(v.base + (i64)k * v.stride); }
+
+template <int N> __device__ __forceinline__ void vec_meta(const VecView& v, u32 k, int& e, int& s, u32& st)
+{
+    const u64 a = vec_at(v, k);
+    e = (int)v.planes[(u64)N * v.pitch + a];
+    s = (int)v.planes[(u64)(N + 1) * v.pitch + a];
+    st = v.planes[(u64)(N + 2) * v.pitch + a];
+}
+
+template <int N> __device__ __forceinline__ Num<N> vec_load(const VecView& v, u32 k) { return load_out<N>(v.planes, v.pitch, vec_at(v, k)); }
+
+// Per-thread scratch of the cluster path: two accumulators of `words` words each.
+struct DotScratch {
+    u32* base;
+    u64 words;
+    int32_t* error; // set (and the result zeroed) if a cluster would not fit: the host sizes scratch so that none does
+};
+
+template <int N>
+__device__ __noinline__ Num<N> dot_sub_clusters(const Num<N>& c, const VecView& x, const VecView& y, u32 K, const DotScratch& sc)
+{
+    constexpr i64 NONE = -((i64)1 << 62);
+    int logk = 0;
+    while (((u64)1 << logk) < (u64)K + 1) ++logk;
+    const i64 width = 64 * N, gap = 32 * N + logk + 4;
+    // term k < K: -x_k y_k; term K: c
+    auto scale_of = [&](u32 k, int& sg) -> i64 {
+        if (k == K) {
+            sg = c.s;
+            return (i64)c.e - (32 * N - 1);
+        }
+        int ex, sx, ey, sy;
+        u32 tx, ty;
+        vec_meta<N>(x, k, ex, sx, tx);
+        vec_meta<N>(y, k, ey, sy, ty);
+        sg = -sx * sy;
+        return sg ? (i64)ex + ey - 2 * (32 * N - 1) : NONE;
+    };
+    // next cluster below `cutoff`: its highest and lowest term scales; false when none is left
+    auto next = [&](i64 cutoff, i64& hi, i64& low) -> bool {
+        hi = NONE;
+        for (u32 k = 0; k <= K; ++k) {
+            int sg;
+            const i64 s = scale_of(k, sg);
+            if (sg && s < cutoff && s > hi) hi = s;
+        }
+        if (hi == NONE) return false;
+        low = hi;
+        for (bool changed = true; changed;) {
+            changed = false;
+            for (u32 k = 0; k <= K; ++k) {
+                int sg;
+                const i64 s = scale_of(k, sg);
+                if (sg && s < low && s + width - 1 + gap >= low) {
+                    low = s;
+                    changed = true;
+                }
+            }
+        }
+        return true;
+    };
+    constexpr int KW = N + 4; // kept words of the first nonzero cluster sum: a sticky word, then the top N+3
+    u32 S[KW];
+#pragma unroll
+    for (int w = 0; w < KW; ++w) S[w] = 0;
+    i64 scale1 = 0, cutoff = ((i64)1 << 62), hi, low;
+    int nsums = 0, sign1 = 0, sign2 = 0;
+    while (nsums < 2 && next(cutoff, hi, low)) {
+        const u64 W = (u64)((hi + width - low) / 32 + 3);
+        if (W > sc.words) {
+            *sc.error = 1;
+            return zero<N>(ST_OK);
+        }
+        u32* pos = sc.base;
+        u32* neg = sc.base + W;
+        for (u64 w = 0; w < 2 * W; ++w) pos[w] = 0;
+        for (u32 k = 0; k <= K; ++k) {
+            int sg;
+            const i64 s = scale_of(k, sg);
+            if (!sg || s < low || s >= cutoff) continue;
+            u32* dst = sg > 0 ? pos : neg;
+            if (k == K) global_add(dst, W, c.w, N, s - low);
+            else {
+                const Num<N> a = vec_load<N>(x, k), b = vec_load<N>(y, k);
+                u32 p[2 * N];
+                mantissa_product<N>(a.w, b.w, p);
+                global_add(dst, W, p, 2 * N, s - low);
+            }
+        }
+        int cmp = 0;
+        for (u64 w = W; w-- > 0;)
+            if (pos[w] != neg[w]) {
+                cmp = pos[w] > neg[w] ? 1 : -1;
+                break;
+            }
+        if (cmp) {
+            if (!nsums) {
+                u32* big = cmp > 0 ? pos : neg;
+                const u32* small = cmp > 0 ? neg : pos;
+                u32 bw = 0;
+                u64 top = 0;
+                for (u64 w = 0; w < W; ++w) {
+                    const u64 v = (u64)big[w] - small[w] - bw;
+                    big[w] = (u32)v;
+                    bw = (u32)(v >> 63);
+                    if (big[w]) top = w;
+                }
+                const i64 first = (i64)top - (N + 2);
+                bool sticky = false;
+                for (i64 w = 0; w < first; ++w) sticky = sticky || big[w] != 0;
+                S[0] = sticky ? 1u : 0u;
+#pragma unroll
+                for (int t = 0; t < N + 3; ++t) {
+                    const i64 w = first + t;
+                    S[t + 1] = (w >= 0 && w < (i64)W) ? big[w] : 0u;
+                }
+                scale1 = low + 32 * (first - 1);
+                sign1 = cmp;
+            } else
+                sign2 = cmp;
+            ++nsums;
+        }
+        cutoff = low;
+    }
+    if (!nsums) return zero<N>(ST_OK);
+    if (nsums == 1) return pack<N, KW>(S, scale1, sign1, ST_OK);
+    // S * 2^D -/+ 1 puts the sign of the remaining clusters below the rounding position of S
+    constexpr int DW = N + 2, EW = KW + DW;
+    u32 ext[EW];
+#pragma unroll
+    for (int w = 0; w < EW; ++w) ext[w] = w >= DW ? S[w - DW] : 0u;
+    if (sign2 == sign1) ext[0] = 1;
+    else {
+        u32 bw = 1;
+#pragma unroll
+        for (int w = 0; w < EW; ++w) {
+            const u64 v = (u64)ext[w] - bw;
+            ext[w] = (u32)v;
+            bw = (u32)(v >> 63);
+        }
+    }
+    return pack<N, EW>(ext, scale1 - 32 * DW, sign1, ST_OK);
+}
+
+// D(c; x, y; K) = RN(c - sum_{k<K} x_k y_k) with one rounding (the contract of the reference's
+// exact_dot_add with subtract, linalg.hpp:268-318): zero with the OR of the statuses of c and of every
+// x_k, y_k; c itself when no product is nonzero.
+template <int N> __device__ Num<N> dot_sub(const Num<N>& c, const VecView& x, const VecView& y, u32 K, const DotScratch& sc)
+{
+    constexpr int W = factor_window(N);
+    constexpr i64 NONE = -((i64)1 << 62);
+    u32 st = c.st;
+    i64 hi = NONE, lo = -NONE;
+    u32 count = 0;
+    for (u32 k = 0; k < K; ++k) {
+        int ex, sx, ey, sy;
+        u32 tx, ty;
+        vec_meta<N>(x, k, ex, sx, tx);
+        vec_meta<N>(y, k, ey, sy, ty);
+        st |= tx | ty;
+        if (sx && sy) {
+            const i64 s = (i64)ex + ey - 2 * (32 * N - 1);
+            hi = s > hi ? s : hi;
+            lo = s < lo ? s : lo;
+            ++count;
+        }
+    }
+    if (st) return zero<N>(st);
+    if (!count) return c;
+    if (c.s) {
+        const i64 s = (i64)c.e - (32 * N - 1);
+        hi = s > hi ? s : hi;
+        lo = s < lo ? s : lo;
+        ++count;
+    }
+    int logc = 0;
+    while ((1u << logc) < count) ++logc;
+    if (hi - lo + 64 * N + logc + 1 > 32 * W) return dot_sub_clusters<N>(c, x, y, K, sc);
+    u32 acc[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) acc[w] = 0;
+    for (u32 k = 0; k < K; ++k) {
+        const Num<N> a = vec_load<N>(x, k), b = vec_load<N>(y, k);
+        if (!a.s || !b.s) continue;
+        u32 p[2 * N], w[W];
+        mantissa_product<N>(a.w, b.w, p);
+        place<W, 2 * N>(w, p, (i64)a.e + b.e - 2 * (32 * N - 1) - lo);
+        accumulate_signed<W>(acc, w, -a.s * b.s);
+    }
+    if (c.s) {
+        u32 w[W];
+        place<W, N>(w, c.w, (i64)c.e - (32 * N - 1) - lo);
+        accumulate_signed<W>(acc, w, c.s);
+    }
+    int sign = 1;
+    if (acc[W - 1] >> 31) {
+        sign = -1;
+        u32 cy = 1;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const u64 v = (u64)(u32)~acc[w] + cy;
+            acc[w] = (u32)v;
+            cy = (u32)(v >> 32);
+        }
+    }
+    return pack<N, W>(acc, lo, sign, ST_OK);
+}
+
+// ---------------------------------------------------------------- Cholesky
+template <int N> __device__ __forceinline__ Num<N> element(const FactorArgs& f, u32 i, u32 j)
+{
+    return load_out<N>(f.w, f.pitch, (u64)i * f.n + j);
+}
+
+// The diagonal block [k0, k1) of a panel, column by column, in one workgroup: the pivot of column j on
+// thread 0, then the entries below it in the block on every thread.  Stops at a failing pivot.
+template <int N> __global__ void __launch_bounds__(256) chol_diag_kernel(FactorArgs f)
+{
+    __shared__ int stop;
+    const u32 t = threadIdx.x, T = blockDim.x;
+    const DotScratch sc{f.scratch + (u64)t * 2 * f.swords, f.swords, &f.info->error};
+    for (u32 j = f.k0; j < f.k1; ++j) {
+        const VecView rj{f.w, f.pitch, (i64)j * f.n + f.k0, 1};
+        if (t == 0) {
+            const Num<N> s = dot_sub<N>(element<N>(f, j, j), rj, rj, j - f.k0, sc);
+            if (s.st || s.s <= 0) {
+                f.info->pivot = (int32_t)j;
+                f.info->sign = s.st ? 0 : s.s;
+                f.info->status = s.st;
+                stop = 1;
+            } else {
+                store_out<N>(f.w, f.pitch, f.n, j, j, false, sqrt<N>(s));
+                stop = 0;
+            }
+        }
+        __syncthreads();
+        if (stop) return;
+        const Num<N> d = element<N>(f, j, j);
+        for (u32 i = j + 1 + t; i < f.k1; i += T) {
+            const VecView ri{f.w, f.pitch, (i64)i * f.n + f.k0, 1};
+            const Num<N> v = dot_sub<N>(element<N>(f, i, j), ri, rj, j - f.k0, sc);
+            store_out<N>(f.w, f.pitch, f.n, i, j, false, div<N>(v, d));
+        }
+        __syncthreads();
+    }
+}
+
+// The rows below the diagonal block, one thread per row i >= k1: columns [k0, p) in order, p the failing
+// pivot of the block or k1.
+template <int N> __global__ void __launch_bounds__(64) chol_panel_kernel(FactorArgs f)
+{
+    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+    const u32 i = f.k1 + r;
+    if (i >= f.n) return;
+    const int32_t piv = f.info->pivot;
+    const u32 pe = piv >= 0 ? (u32)piv : f.k1;
+    const DotScratch sc{f.scratch + (u64)r * 2 * f.swords, f.swords, &f.info->error};
+    const VecView ri{f.w, f.pitch, (i64)i * f.n + f.k0, 1};
+    for (u32 j = f.k0; j < pe; ++j) {
+        const VecView rj{f.w, f.pitch, (i64)j * f.n + f.k0, 1};
+        const Num<N> v = dot_sub<N>(element<N>(f, i, j), ri, rj, j - f.k0, sc);
+        store_out<N>(f.w, f.pitch, f.n, i, j, false, div<N>(v, element<N>(f, j, j)));
+    }
+}
+
+// Entries (i, j) with i >= j >= p after a failing pivot p: zero with status invalid.  Thread per entry of
+// the trailing (n-p) x (n-p) square.
+template <int N> __global__ void __launch_bounds__(256) chol_invalid_kernel(u32* w, u64 pitch, u32 n, u32 p)
+{
+    const u64 m = n - p, idx = (u64)blockIdx.x * 256u + threadIdx.x;
+    if (idx >= m * m) return;
+    const u32 i = p + (u32)(idx / m), j = p + (u32)(idx % m);
+    if (j <= i) store_out<N>(w, pitch, n, i, j, false, zero<N>(ST_INVALID));
+}
+
+// The strict upper triangle of the work matrix set to zero (the factor's upper part).
+template <int N> __global__ void __launch_bounds__(256) zero_upper_kernel(u32* w, u64 pitch, u32 n)
+{
+    const u64 idx = (u64)blockIdx.x * 256u + threadIdx.x;
+    if (idx >= (u64)n * n) return;
+    const u32 i = (u32)(idx / n), j = (u32)(idx % n);
+    if (j > i) store_out<N>(w, pitch, n, i, j, false, zero<N>(ST_OK));
+}
+
+// ---------------------------------------------------------------- triangular solves
+// Forward block [r0, r1) of L X = B, one thread per right-hand side.
+template <int N> __global__ void __launch_bounds__(64) solve_forward_kernel(SolveArgs a)
+{
+    const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nrhs) return;
+    const DotScratch sc{a.scratch + (u64)c * 2 * a.swords, a.swords, a.error};
+    const VecView xc{a.x, a.xpitch, (i64)a.r0 * a.nrhs + c, (i64)a.nrhs};
+    for (u32 i = a.r0; i < a.r1; ++i) {
+        const VecView li{a.f, a.fpitch, (i64)i * a.n + a.r0, 1};
+        const Num<N> v = dot_sub<N>(load_out<N>(a.x, a.xpitch, (u64)i * a.nrhs + c), li, xc, i - a.r0, sc);
+        store_out<N>(a.x, a.xpitch, a.nrhs, i, c, false, div<N>(v, load_out<N>(a.f, a.fpitch, (u64)i * a.n + i)));
+    }
+}
+
+// Backward block [r0, r1) of L^T X = B (column i of L as the vector), one thread per right-hand side.
+template <int N> __global__ void __launch_bounds__(64) solve_backward_kernel(SolveArgs a)
+{
+    const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nrhs) return;
+    const DotScratch sc{a.scratch + (u64)c * 2 * a.swords, a.swords, a.error};
+    for (u32 i = a.r1; i-- > a.r0;) {
+        Num<N> v = load_out<N>(a.x, a.xpitch, (u64)i * a.nrhs + c);
+        if (i + 1 < a.r1) {
+            const VecView col{a.f, a.fpitch, (i64)(i + 1) * a.n + i, (i64)a.n};
+            const VecView xc{a.x, a.xpitch, (i64)(i + 1) * a.nrhs + c, (i64)a.nrhs};
+            v = dot_sub<N>(v, col, xc, a.r1 - i - 1, sc);
+        }
+        store_out<N>(a.x, a.xpitch, a.nrhs, i, c, false, div<N>(v, load_out<N>(a.f, a.fpitch, (u64)i * a.n + i)));
+    }
+}
+
+} // namespace lf

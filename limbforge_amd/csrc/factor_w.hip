@@ -1,0 +1,56 @@
+// One width of the Cholesky / triangular-solve kernels: compiled once per LF_WORDS in [2, 32].
+#include "factor_kernels.hpp"
+#include "factor_launch.hpp"
+
+#ifndef LF_WORDS
+#error "compile with -DLF_WORDS=<2..32>"
+#endif
+
+namespace lf {
+
+template <int N> hipError_t launch_chol_diag(const FactorArgs& f, uint32_t threads, hipStream_t s)
+{
+    if (f.k1 <= f.k0) return hipSuccess;
+    hipLaunchKernelGGL((chol_diag_kernel<N>), dim3(1), dim3(threads), 0, s, f);
+    return hipGetLastError();
+}
+
+template <int N> hipError_t launch_chol_panel(const FactorArgs& f, hipStream_t s)
+{
+    if (f.k1 >= f.n) return hipSuccess;
+    hipLaunchKernelGGL((chol_panel_kernel<N>), dim3((f.n - f.k1 + 63u) / 64u), dim3(64), 0, s, f);
+    return hipGetLastError();
+}
+
+template <int N> hipError_t launch_chol_invalid(uint32_t* w, uint64_t pitch, uint32_t n, uint32_t p, hipStream_t s)
+{
+    const uint64_t m = n - p;
+    if (!m) return hipSuccess;
+    hipLaunchKernelGGL((chol_invalid_kernel<N>), dim3((unsigned)((m * m + 255u) / 256u)), dim3(256), 0, s, w, pitch, n, p);
+    return hipGetLastError();
+}
+
+template <int N> hipError_t launch_zero_upper(uint32_t* w, uint64_t pitch, uint32_t n, hipStream_t s)
+{
+    const uint64_t total = (uint64_t)n * n;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL((zero_upper_kernel<N>), dim3((unsigned)((total + 255u) / 256u)), dim3(256), 0, s, w, pitch, n);
+    return hipGetLastError();
+}
+
+template <int N> hipError_t launch_solve(int backward, const SolveArgs& a, hipStream_t s)
+{
+    if (!a.nrhs || a.r1 <= a.r0) return hipSuccess;
+    const dim3 grid((a.nrhs + 63u) / 64u), block(64);
+    if (backward) hipLaunchKernelGGL((solve_backward_kernel<N>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((solve_forward_kernel<N>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+template hipError_t launch_chol_diag<LF_WORDS>(const FactorArgs&, uint32_t, hipStream_t);
+template hipError_t launch_chol_panel<LF_WORDS>(const FactorArgs&, hipStream_t);
+template hipError_t launch_chol_invalid<LF_WORDS>(uint32_t*, uint64_t, uint32_t, uint32_t, hipStream_t);
+template hipError_t launch_zero_upper<LF_WORDS>(uint32_t*, uint64_t, uint32_t, hipStream_t);
+template hipError_t launch_solve<LF_WORDS>(int, const SolveArgs&, hipStream_t);
+
+} // namespace lf

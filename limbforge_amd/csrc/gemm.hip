@@ -10,6 +10,7 @@
 #include "limbforge_hip.h"
 #include "gemm_kernels.hpp"
 #include "gemm_launch.hpp"
+#include "gemm_plan.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -211,21 +212,8 @@ const std::vector<uint32_t>& primes()
     return p;
 }
 
-// ---- field access for the band analysis ----
-struct Fields {
-    // element (line, k) is at index line*ls + k*ks; fields at word offsets in records of `stride` words
-    const uint32_t* exponent;
-    const uint32_t* sign;
-    const uint32_t* status;
-    uint64_t stride, ls, ks;
-    void get(std::size_t line, std::size_t k, int& e, int& s, uint32_t& st) const
-    {
-        const uint64_t i = (uint64_t(line) * ls + uint64_t(k) * ks) * stride;
-        e = int(exponent[i]);
-        s = int(sign[i]);
-        st = status[i];
-    }
-};
+using lf::Fields;
+using lf::Product;
 
 struct Side {
     std::vector<GemmLine> lines;
@@ -402,20 +390,9 @@ const GemmTables& gemm_tables()
     return t;
 }
 
-// One exact product call on device operands.  left/right: views of op(A) rows and B columns on the
-// device; lf/rf: host field access for the analysis; C: device output planes (m x n, row-major).
-struct Product {
-    int bits;
-    bool syrk, lower_only, subtract;
-    lf::MatView left, right;
-    Fields lf_, rf_;
-    std::size_t m, n, K;
-    lf_operand C;
-    lf_linalg_options opt;
-    hipStream_t stream;
-};
+} // namespace
 
-int run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
+int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
 {
     using Clock = std::chrono::steady_clock;
     const auto since = [](Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); };
@@ -674,7 +651,7 @@ int run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
         r.n = uint32_t(P.n);
         r.out = (uint32_t*)P.C.planes;
         r.out_pitch = P.C.pitch;
-        r.out_stride = P.n;
+        r.out_stride = P.out_stride ? P.out_stride : P.n;
         r.syrk = P.syrk;
         r.lower_only = P.lower_only;
         r.subtract = P.subtract;
@@ -695,7 +672,7 @@ int run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
         x.logk = uint32_t(ceil_log2(P.K + 1));
         x.out = (uint32_t*)P.C.planes;
         x.out_pitch = P.C.pitch;
-        x.out_stride = P.n;
+        x.out_stride = P.out_stride ? P.out_stride : P.n;
         x.syrk = P.syrk;
         x.lower_only = P.lower_only;
         x.subtract = P.subtract;
@@ -777,13 +754,16 @@ int run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     return LF_OK;
 }
 
-int check_options(const lf_linalg_options* o)
+int lf::check_linalg_options(const lf_linalg_options* o)
 {
     if (o->band_bits < 0 || o->band_bits > 1024 || o->max_bands < 0 || o->max_spread < 0 || o->max_spread > 8192)
         return fail(LF_INVALID_ARGUMENT, "linalg options: band_bits in [0,1024], max_bands >= 0, max_spread in [0,8192]");
     if (o->method < 0 || o->method > 1) return fail(LF_INVALID_ARGUMENT, "linalg options: method 0 (residue GEMM) or 1 (limb GEMM)");
     return LF_OK;
 }
+
+namespace {
+using lf::check_linalg_options;
 
 struct HostStage {
     DeviceBuf aos, pa, pb, pc;
@@ -817,7 +797,7 @@ int lf_gemm_host(int bits, int transpose_a, const void* A, const void* B, uint64
     lf_linalg_options o;
     if (options) o = *options;
     else lf_linalg_default_options(&o);
-    if (int r = check_options(&o)) return r;
+    if (int r = check_linalg_options(&o)) return r;
     if (bits < 64 || bits > 1024 || bits % 32) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
     const uint64_t limit = uint64_t(1) << 31;
     if (m >= limit || n >= limit || k >= limit || m * n >= limit || m * k >= limit || k * n >= limit)
@@ -891,7 +871,7 @@ int lf_syrk_host(int bits, const void* A, uint64_t rows, uint64_t cols, void* C,
     lf_linalg_options o;
     if (options) o = *options;
     else lf_linalg_default_options(&o);
-    if (int r = check_options(&o)) return r;
+    if (int r = check_linalg_options(&o)) return r;
     if (bits < 64 || bits > 1024 || bits % 32) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
     if (subtract && !lower_only) return fail(LF_INVALID_ARGUMENT, "syrk: subtract updates the lower triangle only (lower_only)");
     const uint64_t limit = uint64_t(1) << 31;

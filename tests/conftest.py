@@ -20,7 +20,7 @@ def oracle():
     """The CPU oracle (built by `make -C oracle`; built here only when nothing is built)."""
     lib = ROOT / "oracle" / "liblf_oracle.so"
     if not lib.exists():
-        subprocess.run(["make", "-C", str(ROOT / "oracle"), "liblf_oracle.so", "mpfr_pin"], check=True)
+        subprocess.run(["make", "-C", str(ROOT / "oracle"), "liblf_oracle.so", "mpfr_pin", "mpfr_factor_pin"], check=True)
     from oracle import lf_oracle
 
     lf_oracle.lib()

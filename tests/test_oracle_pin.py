@@ -31,3 +31,13 @@ def test_divide_word_closed_form(oracle):
         qr = oracle.lib().lfo_divide_word(high, low, d, inv)
         num = (high << 32) | low
         assert qr & 0xFFFFFFFF == num // d and qr >> 32 == num % d
+
+
+def test_factor_oracle_matches_mpfr_replay(oracle):
+    """lfo_cholesky / lfo_solve (the reference's right-looking host order) against an independent
+    left-looking MPFR program of the documented rounding sequence (oracle/mpfr_factor_pin.c)."""
+    pin = ROOT / "oracle" / "mpfr_factor_pin"
+    assert pin.exists(), "oracle/mpfr_factor_pin is not built (make -C oracle)"
+    p = subprocess.run([str(pin), "3"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert p.returncode == 0, p.stdout[-4000:]
+    assert " 0 failures" in p.stdout
