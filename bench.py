@@ -192,6 +192,95 @@ def gemm_section(reps: int, oracle_samples: int):
     }
 
 
+def to_float(rec: np.ndarray, bits: int) -> float:
+    """A record's value as a double (top two limbs; exponent clamped to double range)."""
+    w = bits // 32
+    s = int(rec[w + 1]) - ((int(rec[w + 1]) >> 31) << 32)
+    if not s or rec[w + 2]:
+        return 0.0
+    e = int(rec[w]) - ((int(rec[w]) >> 31) << 32)
+    top = (int(rec[w - 1]) << 32) | int(rec[w - 2])
+    return float(s) * float(np.ldexp(float(top), max(min(e - 63, 1000), -1000)))
+
+
+def cholesky_section(oracle_check: bool):
+    """256-bit SYRK + blocked Cholesky + solve: A (2048 x 512) single band (exponent round(8 z)), S = A^T A
+    then + I on the diagonal (two roundings, the reference's syrk-then-add idiom), Cholesky n = 512 with
+    block 32, cholesky_solve with 64 right-hand sides (exponent span 20); the scaled residual
+    max |b - S x| / (|S| |x| + |b|) with r rounded once by an exact update GEMM.  L and X are compared
+    bit for bit with the oracle's restatement of the reference's fixed-order host path."""
+    from limbforge_amd import capi
+
+    bits, rows, n, nrhs, block = 256, 2048, 512, 64, 32
+    w = bits // 32
+    rng = np.random.default_rng(20261007 + 4)
+
+    def records(count, exps):
+        r = np.zeros((count, w + 3), dtype=np.uint32)
+        r[:, :w] = rng.integers(0, 1 << 32, size=(count, w), dtype=np.uint64).astype(np.uint32)
+        r[:, w - 1] |= np.uint32(0x80000000)
+        r[:, w] = (np.asarray(exps, dtype=np.int64) & 0xFFFFFFFF).astype(np.uint32)
+        r[:, w + 1] = (np.where(rng.integers(0, 2, size=count) == 1, 1, -1) & 0xFFFFFFFF).astype(np.uint32)
+        return r
+
+    A = records(rows * n, np.round(8 * rng.standard_normal(rows * n)))
+    B = records(n * nrhs, rng.integers(-10, 11, size=n * nrhs))
+    S, srep, syrk_g, syrk_w = capi.syrk(bits, A, rows, n, lower_only=False)
+    one = np.zeros((n, w + 3), np.uint32)
+    one[:, w - 1] = 0x80000000
+    one[:, w + 1] = 1
+    diag = np.arange(n) * (n + 1)
+    S[diag], _, _ = capi.run_host(bits, "add", S[diag], one)
+    res = {"syrk": {"device_ms": round(syrk_g * 1e3, 3), "wall_ms": round(syrk_w * 1e3, 3), "moduli": int(srep.moduli),
+                    "band_pairs": int(srep.npairs)}}
+    chol = []
+    for _ in range(3):
+        t = time.perf_counter()
+        L, info = capi.cholesky(bits, S, n, block=block)
+        chol.append((time.perf_counter() - t, info.timing_gpu_seconds, info.panel_seconds, info.update_seconds))
+    chol.sort()
+    res["cholesky"] = {"wall_ms": round(chol[1][0] * 1e3, 3), "gpu_ms": round(chol[1][1] * 1e3, 3),
+                       "panel_ms": round(chol[1][2] * 1e3, 3), "update_ms": round(chol[1][3] * 1e3, 3),
+                       "pivot": int(info.pivot), "blocks": int(info.blocks)}
+    sol = []
+    for _ in range(3):
+        t = time.perf_counter()
+        X, g, _ = capi.solve(bits, 3, L, n, B, nrhs, block=block)
+        sol.append((time.perf_counter() - t, g))
+    sol.sort()
+    res["solve"] = {"wall_ms": round(sol[1][0] * 1e3, 3), "gpu_ms": round(sol[1][1] * 1e3, 3), "nrhs": nrhs}
+    # scaled residual: r = RN(b - S x) (one rounding), scale = RN(RN(|S| |x|) + |b|), ratio = RN(|r| / scale)
+    r, _, _, _ = capi.gemm(bits, False, S, X, n, nrhs, n, C_in=B, subtract=True)
+
+    def absval(v):
+        v = v.copy()
+        v[:, w + 1] = np.where(v[:, w + 1] != 0, 1, 0).astype(np.uint32)
+        return v
+
+    t, _, _, _ = capi.gemm(bits, False, absval(S), absval(X), n, nrhs, n)
+    scale, _, _ = capi.run_host(bits, "add", t, absval(B))
+    ratio, _, _ = capi.run_host(bits, "div", absval(r), scale)
+    res["scaled_residual"] = max(abs(to_float(q, bits)) for q in ratio)
+    log(f"[bench] cholesky: {res}")
+    if oracle_check:
+        from oracle import lf_oracle
+
+        t = time.perf_counter()
+        WL, piv, _, _ = lf_oracle.cholesky(bits, S, n, block)
+        t_chol = time.perf_counter() - t
+        t = time.perf_counter()
+        WX = lf_oracle.solve(bits, WL, n, B, nrhs, block, 3)
+        t_solve = time.perf_counter() - t
+        res["oracle_check"] = {"factor_mismatches": int((WL != L).any(axis=1).sum()), "solve_mismatches": int((WX != X).any(axis=1).sum()),
+                               "pivot": int(piv)}
+        res["cpu_host_path"] = {"cholesky_s": round(t_chol, 3), "solve_s": round(t_solve, 3), "cores": 1,
+                                "kind": "port", "note": "the oracle's restatement of the reference's fixed-order host path, one thread"}
+        log(f"[bench] cholesky oracle check: {res['oracle_check']}, {res['cpu_host_path']}")
+    res["workload"] = ("256-bit SYRK (A 2048 x 512, single band) + blocked Cholesky n = 512 (block 32) + cholesky_solve "
+                       "with 64 right-hand sides (config 4)")
+    return res
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +291,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-gemm", action="store_true")
     ap.add_argument("--gemm-reps", type=int, default=6)
+    ap.add_argument("--no-cholesky", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})", file=sys.stderr)
@@ -269,6 +359,11 @@ def main(argv=None) -> int:
         log("[bench] 512-bit exact GEMM 512^3")
         gemm = gemm_section(a.gemm_reps, 0 if a.no_parity else 256)
 
+    chol = None
+    if not a.no_cholesky:
+        log("[bench] 256-bit SYRK + Cholesky n=512 + solve (64 right-hand sides)")
+        chol = cholesky_section(not a.no_parity)
+
     base = None
     if not a.no_cpu_baseline:
         log("[bench] timing the CPU baseline")
@@ -294,6 +389,7 @@ def main(argv=None) -> int:
         "cpu_baseline": base,
         "parity": parity,
         "gemm": gemm,
+        "cholesky": chol,
     }
     print(json.dumps(line), flush=True)
     return 0

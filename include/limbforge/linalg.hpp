@@ -47,6 +47,28 @@ struct LinalgReport {
     double analysis_seconds = 0, upload_seconds = 0, gpu_seconds = 0, assembly_seconds = 0, fallback_seconds = 0;
 };
 
+// Blocked Cholesky and triangular solves (the reference's linalg.hpp:70-87).  The result depends on
+// `block` (the documented rounding sequence) and on nothing else; host_macs, solve_host_macs and gpu
+// choose the reference's host/GPU placement of updates, which never changes a bit; here every step runs
+// on the GPU.
+struct FactorOptions {
+    std::size_t block = 32;       // column block nb; 0 selects a single block
+    double host_macs = 5e4;
+    double solve_host_macs = 4e5;
+    bool gpu = true;
+};
+// pivot = n on success; otherwise the first column p whose pivot s_pp = RN(a_pp - sum_k l_pk^2) is zero,
+// negative or carries a status (pivot_sign / pivot_status describe it).  On failure L[i][j] is final for
+// j < p and every lower entry with j >= p is zero with status invalid.
+struct CholeskyInfo {
+    std::size_t pivot = 0;
+    int pivot_sign = 0;
+    word pivot_status = 0;
+    Timing timing{0, 0};
+    std::size_t blocks = 0, gpu_updates = 0, host_updates = 0;
+    double panel_seconds = 0, update_seconds = 0;
+};
+
 // One Linalg per host thread.  C must not overlap A or B.  Calls return after the GPU has finished.
 class Linalg {
 public:
@@ -66,6 +88,25 @@ public:
     // With subtract, C = RN(C - op(A) B), one rounding per entry.
     Timing gemm(int bits, bool transpose_a, const void* A, const void* B, std::size_t m, std::size_t n, std::size_t k, void* C,
                 bool subtract = false);
+    // Lower L (n x n, row-major) with A = L L^T from the lower triangle of A; the strict upper triangle of
+    // L is set to zero.  L may equal A.
+    CholeskyInfo cholesky(int bits, const void* A, std::size_t n, void* L, const FactorOptions& options = {});
+    // X (n x nrhs) solves L X = B, or L^T X = B with transpose (strict upper triangle of L not read).
+    // X may equal B.
+    Timing trsm(int bits, bool transpose, const void* L, std::size_t n, const void* B, std::size_t nrhs, void* X,
+                const FactorOptions& options = {});
+    // X solves (L L^T) X = B: trsm, then trsm with transpose.
+    Timing cholesky_solve(int bits, const void* L, std::size_t n, const void* B, std::size_t nrhs, void* X,
+                          const FactorOptions& options = {});
+    CholeskyInfo cholesky_blocked(int bits, const void* A, std::size_t n, void* L, const FactorOptions& o = {})
+    {
+        return cholesky(bits, A, n, L, o);
+    }
+    Timing cholesky_solve_blocked(int bits, const void* L, std::size_t n, const void* B, std::size_t nrhs, void* X,
+                                  const FactorOptions& o = {})
+    {
+        return cholesky_solve(bits, L, n, B, nrhs, X, o);
+    }
     const LinalgReport& report() const;
     const LinalgOptions& options() const;
     Timing gemm_exact(int bits, bool transpose_a, const void* A, const void* B, std::size_t m, std::size_t n, std::size_t k, void* C,

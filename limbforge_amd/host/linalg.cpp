@@ -119,6 +119,57 @@ Timing Linalg::gemm(int bits, bool transpose_a, const void* A, const void* B, st
     return t;
 }
 
+static lf_factor_options c_factor(const FactorOptions& o)
+{
+    lf_factor_options f;
+    lf_factor_default_options(&f);
+    f.block = o.block;
+    f.host_macs = o.host_macs;
+    f.solve_host_macs = o.solve_host_macs;
+    f.gpu = o.gpu;
+    return f;
+}
+
+CholeskyInfo Linalg::cholesky(int bits, const void* A, std::size_t n, void* L, const FactorOptions& options)
+{
+    check_bits(bits);
+    lf_linalg_options o = impl->c_options();
+    lf_factor_options f = c_factor(options);
+    lf_cholesky_info ci;
+    detail::check(lf_cholesky_host(bits, A, n, L, &f, &o, &ci), "Linalg::cholesky");
+    CholeskyInfo info;
+    info.pivot = ci.pivot;
+    info.pivot_sign = ci.pivot_sign;
+    info.pivot_status = ci.pivot_status;
+    info.timing = {ci.timing_gpu_seconds, ci.timing_wall_seconds};
+    info.blocks = ci.blocks;
+    info.gpu_updates = ci.gpu_updates;
+    info.host_updates = ci.host_updates;
+    info.panel_seconds = ci.panel_seconds;
+    info.update_seconds = ci.update_seconds;
+    return info;
+}
+
+static Timing solve(int bits, int passes, const void* L, std::size_t n, const void* B, std::size_t nrhs, void* X, const FactorOptions& options,
+                    const lf_linalg_options& o, const char* where)
+{
+    if (bits < 64 || bits > 1024 || bits % 32) throw std::invalid_argument("bits must be a multiple of 32 in [64,1024]");
+    lf_factor_options f = c_factor(options);
+    Timing t{0, 0};
+    detail::check(lf_solve_host(bits, passes, L, n, B, nrhs, X, &f, &o, &t.gpu_seconds, &t.wall_seconds), where);
+    return t;
+}
+
+Timing Linalg::trsm(int bits, bool transpose, const void* L, std::size_t n, const void* B, std::size_t nrhs, void* X, const FactorOptions& options)
+{
+    return solve(bits, transpose ? 2 : 1, L, n, B, nrhs, X, options, impl->c_options(), "Linalg::trsm");
+}
+
+Timing Linalg::cholesky_solve(int bits, const void* L, std::size_t n, const void* B, std::size_t nrhs, void* X, const FactorOptions& options)
+{
+    return solve(bits, 3, L, n, B, nrhs, X, options, impl->c_options(), "Linalg::cholesky_solve");
+}
+
 namespace detail {
 void exact_dot_device(int bits, const void* c, bool subtract, const void* a, std::ptrdiff_t sa, const void* b, std::ptrdiff_t sb,
                       std::size_t K, void* out)

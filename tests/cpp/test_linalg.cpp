@@ -10,11 +10,23 @@
 #include <cstring>
 #include <fstream>
 #include <functional>
+#include <random>
 #include <iostream>
 #include <string>
 #include <vector>
 
 using namespace limbforge;
+
+// a random normalized value with exponent in [-span, span]
+template <int Bits> Float<Bits> random_value(std::mt19937_64& g, int span)
+{
+    Float<Bits> x = zero<Bits / 32>();
+    for (auto& w : x.limb) w = uint32_t(g());
+    x.limb[Bits / 32 - 1] |= 0x80000000u;
+    x.exponent = int(g() % uint64_t(2 * span + 1)) - span;
+    x.sign = g() % 2 ? 1 : -1;
+    return x;
+}
 
 static int failures = 0;
 #define REQUIRE(cond, msg)                                                                                             \
@@ -94,6 +106,35 @@ int main(int argc, char** argv)
         la.gemm(64, false, a.data(), b.data(), 1, 1, 3, C.data());
         REQUIRE(std::memcmp(&C[0], &one, sizeof(F)) == 0, "Linalg::gemm known answer");
         REQUIRE(la.report().left_bands.size() == 5, "histogram has max_bands + 1 slots");
+        // Cholesky, trsm and cholesky_solve against the oracle's host order (n = 24, block 8, 256 bits)
+        {
+            using G = Float<256>;
+            const std::size_t n = 24, m = 2 * n + 1, nrhs = 3;
+            std::mt19937_64 g(77);
+            std::vector<G> J(m * n), S(n * n), L(n * n), W(n * n), B(n * nrhs), X(n * nrhs), Y(n * nrhs);
+            for (auto& x : J) x = random_value<256>(g, 4);
+            for (auto& x : B) x = random_value<256>(g, 8);
+            lfo_syrk(256, J.data(), m, n, S.data(), 0, 0);
+            for (std::size_t i = 0; i < n; ++i) S[i * n + i].exponent += 1; // 2 (J^T J)_ii: diagonally heavier
+            FactorOptions fo;
+            fo.block = 8;
+            CholeskyInfo info = la.cholesky(256, S.data(), n, L.data(), fo);
+            int ps = 0;
+            uint32_t pst = 0;
+            const std::size_t piv = lfo_cholesky(256, S.data(), n, W.data(), 8, &ps, &pst);
+            REQUIRE(info.pivot == piv && piv == n && info.blocks == 3, "cholesky pivot " << info.pivot);
+            REQUIRE(std::memcmp(L.data(), W.data(), L.size() * sizeof(G)) == 0, "cholesky factor");
+            for (int passes = 1; passes <= 3; ++passes) {
+                if (passes == 3) la.cholesky_solve(256, L.data(), n, B.data(), nrhs, X.data(), fo);
+                else la.trsm(256, passes == 2, L.data(), n, B.data(), nrhs, X.data(), fo);
+                lfo_solve(256, W.data(), n, B.data(), nrhs, Y.data(), 8, passes);
+                REQUIRE(std::memcmp(X.data(), Y.data(), X.size() * sizeof(G)) == 0, "solve passes " << passes);
+            }
+            // in place: L may equal A
+            std::vector<G> A2 = S;
+            la.cholesky(256, A2.data(), n, A2.data(), fo);
+            REQUIRE(std::memcmp(A2.data(), W.data(), A2.size() * sizeof(G)) == 0, "cholesky in place");
+        }
         // errors
         bool threw = false;
         try {
