@@ -92,7 +92,12 @@ template <int N> __global__ void __launch_bounds__(256) mul_chain_kernel(ChainAr
     if (i >= g.count) return;
     Num<N> x = load_num<N>(g.x, g.px, i);
     const Num<N> f = load_num<N>(g.f, g.pf, operand_index(i, g.f_stride, g.f_period));
-    for (int s = 0; s < g.steps; ++s) x = mul<N>(x, f);
+    if constexpr (use_digits<N>()) {
+        u32 fd[digit_count<N>()]; // the factor's digits, formed once for the whole chain
+        to_digits<N>(f.w, fd);
+        for (int s = 0; s < g.steps; ++s) x = mul_by_digits<N>(x, f, fd);
+    } else
+        for (int s = 0; s < g.steps; ++s) x = mul<N>(x, f);
     store_num<N>(g.out, g.po, i, x);
 }
 

@@ -277,9 +277,67 @@ template <int N> __device__ __forceinline__ Num<N> add(Num<N> a, Num<N> b)
 template <int N> __device__ __forceinline__ Num<N> sub(Num<N> a, Num<N> b) { return add<N>(a, negate<N>(b)); }
 
 // -------------------------------------------------------------------- mul
-// p (2N words) = a * b, schoolbook rows of v_mad_u64_u32 carry chains.
+// From 8 limbs on, products are formed from 29-bit digits: a column of D <= 36 digit products sums to
+// less than 36 * 2^58 < 2^64, so every digit product is one v_mad_u64_u32 accumulating into its column
+// in place, with no carry chain and no register shuffling; one carry pass and a repack to 32-bit words
+// follow.  (32-bit schoolbook rows need carry adds and pair moves around every multiply-add: about 4.7
+// VALU instructions per word product at 32 limbs against 1.3 here.)
+constexpr int DIGIT_BITS = 29;
+constexpr u32 DIGIT_MASK = (1u << DIGIT_BITS) - 1;
+template <int N> constexpr int digit_count() { return (32 * N + DIGIT_BITS - 1) / DIGIT_BITS; }
+template <int N> constexpr bool use_digits() { return N >= 8; }
+
+template <int N> __device__ __forceinline__ void to_digits(const u32 (&w)[N], u32 (&d)[digit_count<N>()])
+{
+#pragma unroll
+    for (int k = 0; k < digit_count<N>(); ++k) {
+        const int bit = DIGIT_BITS * k, wi = bit / 32, s = bit % 32;
+        const u32 lo = w[wi] >> s;
+        const u32 hi = (s && wi + 1 < N) ? (w[wi + 1] << (32 - s)) : 0u;
+        d[k] = (lo | hi) & DIGIT_MASK;
+    }
+}
+
+// p (2N words) = a * b from the digits of a and b (each < 2^(32N)).
+template <int N>
+__device__ __forceinline__ void digit_product(const u32 (&da)[digit_count<N>()], const u32 (&db)[digit_count<N>()], u32 (&p)[2 * N])
+{
+    constexpr int D = digit_count<N>();
+    static_assert(D <= 63, "column sums of 29-bit digit products must fit in 64 bits");
+    u64 col[2 * D];
+#pragma unroll
+    for (int k = 0; k < 2 * D; ++k) col[k] = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) col[i + j] += (u64)da[i] * db[j];
+    u32 dg[2 * D];
+#pragma unroll
+    for (int k = 0; k < 2 * D - 1; ++k) {
+        dg[k] = (u32)col[k] & DIGIT_MASK;
+        col[k + 1] += col[k] >> DIGIT_BITS;
+    }
+    dg[2 * D - 1] = (u32)col[2 * D - 1]; // the product is < 2^(64N) <= 2^(58D): the top digit fits in 29 bits
+#pragma unroll
+    for (int w = 0; w < 2 * N; ++w) {
+        const int bit = 32 * w, k = bit / DIGIT_BITS, s = bit % DIGIT_BITS;
+        u32 v = dg[k] >> s;
+        if (k + 1 < 2 * D) v |= dg[k + 1] << (DIGIT_BITS - s);
+        if (k + 2 < 2 * D && 2 * DIGIT_BITS - s < 32) v |= dg[k + 2] << (2 * DIGIT_BITS - s);
+        p[w] = v;
+    }
+}
+
+// p (2N words) = a * b: digit columns from 8 limbs on, schoolbook rows of v_mad_u64_u32 carry chains below.
 template <int N> __device__ __forceinline__ void mantissa_product(const u32 (&a)[N], const u32 (&b)[N], u32 (&p)[2 * N])
 {
+    if constexpr (use_digits<N>()) {
+        u32 da[digit_count<N>()], db[digit_count<N>()];
+        to_digits<N>(a, da);
+        to_digits<N>(b, db);
+        digit_product<N>(da, db, p);
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 2 * N; ++i) p[i] = 0;
 #pragma unroll
@@ -340,6 +398,17 @@ template <int N> __device__ __forceinline__ Num<N> mul(const Num<N>& a, const Nu
     if (st || !a.s || !b.s) return zero<N>(st);
     u32 p[2 * N];
     mantissa_product<N>(a.w, b.w, p);
+    return round_product<N>(p, (i64)a.e + b.e, a.s * b.s, ST_OK);
+}
+
+// RN(a * b) with the digits of b's mantissa given (a multiplier reused over a chain); needs use_digits<N>().
+template <int N> __device__ __forceinline__ Num<N> mul_by_digits(const Num<N>& a, const Num<N>& b, const u32 (&bd)[digit_count<N>()])
+{
+    const u32 st = a.st | b.st;
+    if (st || !a.s || !b.s) return zero<N>(st);
+    u32 ad[digit_count<N>()], p[2 * N];
+    to_digits<N>(a.w, ad);
+    digit_product<N>(ad, bd, p);
     return round_product<N>(p, (i64)a.e + b.e, a.s * b.s, ST_OK);
 }
 

@@ -48,6 +48,17 @@ template <int N> void run_width(int cases)
         report<N>("add", i, lf::add<N>(da, db), lfo_add(N, a, b));
         report<N>("sub", i, lf::sub<N>(da, db), lfo_sub(N, a, b));
         report<N>("mul", i, lf::mul<N>(da, db), lfo_mul(N, a, b));
+        if constexpr (lf::use_digits<N>()) { // the chain's multiply with the factor's digits formed once
+            uint32_t bd[lf::digit_count<N>()];
+            lf::to_digits<N>(db.w, bd);
+            auto x = da;
+            lfo_num y = a;
+            for (int s = 0; s < 3; ++s) {
+                x = lf::mul_by_digits<N>(x, db, bd);
+                y = lfo_mul(N, y, b);
+            }
+            report<N>("mul chain", i, x, y);
+        }
         report<N>("div", i, lf::div<N>(da, db), lfo_div(N, a, b));
         root = a;
         lfo_root_case(&g, N, i, &root);
