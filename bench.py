@@ -192,6 +192,188 @@ def gemm_section(reps: int, oracle_samples: int):
     }
 
 
+def elementwise_section(reps: int, peak_tps: float, oracle_check: bool, cpu: bool):
+    """Element-wise batches of 2^20 values at 256, 512 and 1024 bits: div, fma (coefficient broadcast with
+    period 64), sqrt of |x| and the composed complex division, resident; operands as the chain's with
+    exponents uniform in [-512, 512].  Device time per launch (HIP events, median of `reps`), the algorithmic
+    word products and HBM bytes per element, a sample of outputs against the oracle, and the MPFR baseline
+    on a 65,536-element sample (oracle/mpfr_elementwise_bench)."""
+    from limbforge_amd import capi
+
+    count = 1 << 20
+    rng = np.random.default_rng(20261007 + 1)
+    stream = capi.Stream()
+    out = {}
+    threads = host_threads()
+    for bits in (256, 512, 1024):
+        n = bits // 32
+        E = 4 * n + 12
+
+        def records(k, positive=False):
+            r = np.zeros((k, n + 3), dtype=np.uint32)
+            r[:, :n] = rng.integers(0, 1 << 32, size=(k, n), dtype=np.uint64).astype(np.uint32)
+            r[:, n - 1] |= np.uint32(0x80000000)
+            r[:, n] = (rng.integers(-512, 513, size=k) & 0xFFFFFFFF).astype(np.uint32)
+            sg = np.ones(k, np.int64) if positive else np.where(rng.integers(0, 2, size=k) == 1, 1, -1)
+            r[:, n + 1] = (sg & 0xFFFFFFFF).astype(np.uint32)
+            return r
+
+        for op in ("div", "fma", "sqrt", "cdiv"):
+            cplx = op == "cdiv"
+            if cplx:
+                a = np.concatenate([records(count), records(count)], axis=1)
+                b = np.concatenate([records(count), records(count)], axis=1)
+            else:
+                a = records(count, positive=op == "sqrt")
+                b = records(64 if op == "fma" else count)
+            c = records(count) if op == "fma" else None
+            A, O = capi.Buffer(bits, count, cplx), capi.Buffer(bits, count, cplx)
+            A.upload(a)
+            B = C_ = None
+            if op != "sqrt":
+                B = capi.Buffer(bits, b.shape[0], cplx)
+                B.upload(b)
+            if c is not None:
+                C_ = capi.Buffer(bits, count)
+                C_.upload(c)
+            code = {"div": "div", "fma": "fma", "sqrt": "sqrt", "cdiv": "complex_div"}[op]
+            bidx = (1, 64) if op == "fma" else None
+            times = []
+            for i in range(reps + 2):
+                e0, e1 = capi.Event(), capi.Event()
+                e0.record(stream)
+                capi.elementwise(bits, code, A, B, C_, O, b_index=bidx, stream=stream.handle)
+                e1.record(stream)
+                stream.synchronize()
+                if i >= 2:
+                    times.append(e0.elapsed_ms(e1))
+            ms = float(np.median(times))
+            wp = {"div": n * (n + 1), "fma": n * n, "sqrt": n * n, "cdiv": 6 * n * n + 2 * n * (n + 1)}[op]
+            nbytes = {"div": 3 * E, "fma": 3 * E, "sqrt": 2 * E, "cdiv": 6 * E}[op]
+            rate = count / (ms * 1e-3)
+            row = {"device_ms": round(ms, 4), "values_per_s": round(rate, 1),
+                   "word_products_per_value": wp, "valu_frac": round(rate * wp / peak_tps, 4),
+                   "hbm_bytes_per_value": nbytes, "hbm_gbps": round(rate * nbytes / 1e9, 1),
+                   "hbm_frac": round(rate * nbytes / 8e12, 4)}
+            if oracle_check:
+                from oracle import lf_oracle
+
+                got = O.download()
+                k = 4096
+                want = lf_oracle.elementwise(bits, capi.OPS[code], a[:k], b if op == "fma" else (b[:k] if B is not None else None),
+                                             c[:k] if c is not None else None, b_index=(1, 64) if op == "fma" else (1, 0))
+                row["oracle_mismatches"] = int((got[:k] != want).any(axis=1).sum())
+                row["oracle_checked"] = k
+            if cpu:
+                exe = ROOT / "oracle" / "mpfr_elementwise_bench"
+                if exe.exists():
+                    try:
+                        p = subprocess.run([str(exe), str(bits), op, "65536", str(threads), "3"], capture_output=True, text=True, timeout=300)
+                        m = json.loads(p.stdout)
+                        row["mpfr"] = {"threads": threads, "serial_values_per_s": round(65536 / m["serial_s"], 1),
+                                       "parallel_values_per_s": round(65536 / m["parallel_s"], 1), "sample": "65,536 values, median of 3"}
+                    except Exception as e:
+                        row["mpfr"] = {"error": str(e)}
+            out[f"{op}_{bits}"] = row
+            log(f"[bench] elementwise {op} {bits}: {row}")
+            del A, O, B, C_
+    return {"workload": "element-wise div / fma (b period 64) / sqrt |x| / complex div, 2^20 values at 256, 512, 1024 bits (config 1)",
+            "peak_word_products_per_s": peak_tps, "ops": out}
+
+
+def mixed_section(peak_tps: float, oracle_check: bool, cpu: bool, reps: int = 3):
+    """10^6 values with precisions 64 + 32 k bits, k in [0, 30] drawn with P(k) ~ 0.85^k, chains of uniform
+    length 1-64 whose steps are fma or mul by a seeded mask, operands as the chain's.  Bucketed by precision
+    (longest chains first inside a bucket); timed as one launch per precision on a stream (workgroup 256,
+    then the autotuned sizes) and as one hipGraph of concurrent bucket nodes with the autotuned sizes."""
+    from limbforge_amd import capi
+
+    count, p = 10 ** 6, 0.15
+    rng = np.random.default_rng(20261007 + 2)
+    probs = (1 - p) ** np.arange(31)
+    probs /= probs.sum()
+    ks = rng.choice(31, size=count, p=probs)
+    length = rng.integers(1, 65, size=count).astype(np.uint32)
+    mask = rng.integers(0, 1 << 63, size=count, dtype=np.uint64) * 2 + rng.integers(0, 2, size=count).astype(np.uint64)
+    buckets, host, work = [], {}, 0
+    for k in np.unique(ks):
+        bits = 64 + 32 * int(k)
+        n = bits // 32
+        idx = np.nonzero(ks == k)[0]
+        idx = idx[np.argsort(-length[idx].astype(np.int64), kind="stable")]
+
+        def records(m):
+            r = np.zeros((m, n + 3), dtype=np.uint32)
+            r[:, :n] = rng.integers(0, 1 << 32, size=(m, n), dtype=np.uint64).astype(np.uint32)
+            r[:, n - 1] |= np.uint32(0x80000000)
+            r[:, n] = (rng.integers(-64, 65, size=m) & 0xFFFFFFFF).astype(np.uint32)
+            r[:, n + 1] = (np.where(rng.integers(0, 2, size=m) == 1, 1, -1) & 0xFFFFFFFF).astype(np.uint32)
+            return r
+
+        x, f, g = records(idx.size), records(idx.size), records(idx.size)
+        host[bits] = (x, f, g, length[idx], mask[idx])
+        buckets.append(capi.MixedBucket(bits, *host[bits]))
+        work += int(length[idx].astype(np.int64).sum()) * n * n
+    tuned = {}
+    for b in buckets:
+        tuned[b.bits] = capi.autotune_mixed(b.bits, 32768)[0]
+    stream = capi.Stream()
+
+    def timed(run):
+        ts = []
+        for i in range(reps + 1):
+            e0, e1 = capi.Event(), capi.Event()
+            e0.record(stream)
+            run()
+            e1.record(stream)
+            stream.synchronize()
+            if i:
+                ts.append(e0.elapsed_ms(e1))
+        return float(np.median(ts))
+
+    plain_ms = timed(lambda: capi.mixed_chains(buckets, block=[256] * len(buckets), stream=stream.handle))
+    tuned_ms = timed(lambda: capi.mixed_chains(buckets, block=[tuned[b.bits] for b in buckets], stream=stream.handle))
+    graph = capi.mixed_graph(buckets, block=[tuned[b.bits] for b in buckets])
+    try:
+        graph_ms = timed(lambda: capi.graph_launch(graph, stream.handle))
+    finally:
+        capi.graph_destroy(graph)
+    res = {
+        "workload": "10^6 values, 64-1024 bits (P(k) ~ 0.85^k), fma/mul chains of length 1-64 (config 2)",
+        "buckets": len(buckets),
+        "word_products": work,
+        "per_precision_launches_ms": round(plain_ms, 4),
+        "per_precision_tuned_ms": round(tuned_ms, 4),
+        "graph_tuned_ms": round(graph_ms, 4),
+        "values_per_s_graph": round(count / (graph_ms * 1e-3), 1),
+        "speedup_graph_vs_per_precision": round(plain_ms / graph_ms, 3),
+        "valu_frac_graph": round(work / (graph_ms * 1e-3) / peak_tps, 4),
+        "autotuned_workgroup": {str(k): v for k, v in sorted(tuned.items())},
+    }
+    if oracle_check or cpu:
+        from oracle import lf_oracle
+
+        bad = checked = 0
+        t_cpu = 0.0
+        sample_work = 0
+        for b in buckets:
+            x, f, g, ln, mk = host[b.bits]
+            sel = np.linspace(0, b.count - 1, num=min(b.count, 400)).astype(np.int64)
+            t0 = time.perf_counter()
+            want = lf_oracle.mixed_chain(b.bits, x[sel], f[sel], g[sel], ln[sel], mk[sel])
+            t_cpu += time.perf_counter() - t0
+            sample_work += int(ln[sel].astype(np.int64).sum()) * (b.bits // 32) ** 2
+            got = b.O.download()[sel]
+            bad += int((got != want).any(axis=1).sum())
+            checked += sel.size
+        res["oracle_check"] = {"values": checked, "mismatches": bad}
+        res["cpu_port"] = {"kind": "port", "cores": 1, "word_products_per_s": round(sample_work / t_cpu, 1),
+                           "values_per_s_at_batch_mix": round(count / (work / (sample_work / t_cpu)), 1),
+                           "sample": f"{checked} values spread over every bucket, the oracle's step sequence on one thread"}
+    log(f"[bench] mixed: {res}")
+    return res
+
+
 def to_float(rec: np.ndarray, bits: int) -> float:
     """A record's value as a double (top two limbs; exponent clamped to double range)."""
     w = bits // 32
@@ -292,6 +474,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-gemm", action="store_true")
     ap.add_argument("--gemm-reps", type=int, default=6)
     ap.add_argument("--no-cholesky", action="store_true")
+    ap.add_argument("--no-elementwise", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--elementwise-reps", type=int, default=5)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})", file=sys.stderr)
@@ -359,6 +544,16 @@ def main(argv=None) -> int:
         log("[bench] 512-bit exact GEMM 512^3")
         gemm = gemm_section(a.gemm_reps, 0 if a.no_parity else 256)
 
+    ew = None
+    if not a.no_elementwise:
+        log("[bench] element-wise div / fma / sqrt / complex div, 2^20 values at 256 / 512 / 1024 bits")
+        ew = elementwise_section(a.elementwise_reps, rate.value, not a.no_parity, not a.no_cpu_baseline)
+
+    mixed = None
+    if not a.no_mixed:
+        log("[bench] mixed-precision chains, 10^6 values, bucketed graph vs per-precision launches")
+        mixed = mixed_section(rate.value, not a.no_parity, not a.no_cpu_baseline)
+
     chol = None
     if not a.no_cholesky:
         log("[bench] 256-bit SYRK + Cholesky n=512 + solve (64 right-hand sides)")
@@ -390,6 +585,8 @@ def main(argv=None) -> int:
         "parity": parity,
         "gemm": gemm,
         "cholesky": chol,
+        "elementwise": ew,
+        "mixed": mixed,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -224,6 +224,29 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
 int lf_solve_host(int bits, int passes, const void* L, uint64_t n, const void* B, uint64_t nrhs, void* X, const lf_factor_options* options,
                   const lf_linalg_options* linalg, double* gpu_seconds, double* wall_seconds);
 
+/* ---- mixed-precision chain batches (no reference counterpart: the reference dispatches one precision
+ * per call, README.md:212) ----
+ * A bucket holds `count` values of one width as limb planes (x, f, g, out with one pitch) and, in device
+ * memory, each value's chain length len[i] (0..64) and step mask: step t computes x <- RN(x * f + g)
+ * (Operation::fma, core.hpp:317-326) when bit t of mask[i] is set and x <- RN(x * f) (Operation::mul)
+ * otherwise, every step rounded once, as a dependent CommandBatch::run sequence would. */
+typedef struct lf_chain_bucket {
+    int bits;
+    lf_operand x, f, g, out;
+    const uint32_t* len;
+    const uint64_t* mask;
+    uint64_t count;
+} lf_chain_bucket;
+/* One launch per bucket, in order, on `stream`.  block[i]: lanes per workgroup of bucket i (64, 128, 256
+ * or 512; NULL or 0 = the autotuned size of that width, else 256). */
+int lf_mixed_chains(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, void* stream);
+/* One hipGraph with a kernel node per bucket and no edges between them (launch with lf_graph_launch,
+ * free with lf_graph_destroy).  The graph keeps the buckets' device pointers. */
+int lf_mixed_graph_create(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, void** graph_exec);
+/* Launch-shape autotuner: times the bucket kernel of `bits` for each workgroup size on `sample` synthetic
+ * values (32-step chains, half fma) and keeps the fastest for this device and width. */
+int lf_autotune_mixed(int bits, uint64_t sample, unsigned* block, double* best_ms);
+
 /* ---- measurement support (no reference counterpart) ---- */
 /* Sustained v_mad_u64_u32 rate of the device (32x32->64 word products per second), measured with
  * `blocks` workgroups of 256 lanes, each lane running `iters` rounds of eight independent chains. */

@@ -49,8 +49,8 @@ def _run(cmd: list[str]) -> tuple[list[str], int, str]:
 def _hip_jobs() -> list[tuple[Path, list[str]]]:
     jobs = []
     # sources compiled once per limb count (LF_WORDS), and once
-    per_width = ("elementwise_w", "gemm_w", "factor_w")
-    once = ("layout", "capi", "calibrate", "cast", "gemm", "factor")
+    per_width = ("elementwise_w", "gemm_w", "factor_w", "mixed_w")
+    once = ("layout", "capi", "calibrate", "cast", "gemm", "factor", "mixed")
     for n in WIDTHS:
         for name in per_width:
             out = BUILD / f"{name}{n}.o"

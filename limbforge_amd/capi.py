@@ -428,3 +428,80 @@ def solve(bits, passes, Lf, n, B, nrhs, block=32, options=None, linalg=None):
     check(L.lf_solve_host(bits, int(passes), Lf.ctypes.data, n, B.ctypes.data, nrhs, X.ctypes.data, C.byref(fo),
                           C.byref(linalg) if linalg is not None else None, C.byref(g), C.byref(w)))
     return X, g.value, w.value
+
+
+# ---------------------------------------------------------------- mixed-precision chain batches
+class ChainBucket(C.Structure):
+    _fields_ = [("bits", C.c_int), ("x", Operand), ("f", Operand), ("g", Operand), ("out", Operand),
+                ("len", C.c_void_p), ("mask", C.c_void_p), ("count", C.c_uint64)]
+
+
+def _mixed_sigs(L):
+    if getattr(L, "_mixed", False):
+        return
+    P = C.POINTER
+    L.lf_mixed_chains.argtypes = [P(ChainBucket), C.c_int, P(C.c_uint), C.c_void_p]
+    L.lf_mixed_graph_create.argtypes = [P(ChainBucket), C.c_int, P(C.c_uint), P(C.c_void_p)]
+    L.lf_autotune_mixed.argtypes = [C.c_int, C.c_uint64, P(C.c_uint), P(C.c_double)]
+    L._mixed = True
+
+
+class MixedBucket:
+    """The values of one precision of a mixed batch, resident: x, f, g and out planes, chain lengths and
+    step masks (bit t set: step t is fma, else mul)."""
+
+    def __init__(self, bits: int, x: np.ndarray, f: np.ndarray, g: np.ndarray, length: np.ndarray, mask: np.ndarray):
+        self.bits, self.count = bits, x.shape[0]
+        self.X, self.F, self.G, self.O = (Buffer(bits, self.count) for _ in range(4))
+        self.X.upload(x)
+        self.F.upload(f)
+        self.G.upload(g)
+        self.len = DeviceMemory(4 * self.count)
+        self.len.upload(np.ascontiguousarray(length, dtype=np.uint32))
+        self.mask = DeviceMemory(8 * self.count)
+        self.mask.upload(np.ascontiguousarray(mask, dtype=np.uint64))
+
+    def struct(self) -> ChainBucket:
+        return ChainBucket(self.bits, self.X.operand, self.F.operand, self.G.operand, self.O.operand, self.len.ptr,
+                           self.mask.ptr, self.count)
+
+
+def _bucket_array(buckets, block):
+    arr = (ChainBucket * len(buckets))(*[b.struct() for b in buckets])
+    blk = None if block is None else (C.c_uint * len(buckets))(*[int(x) for x in block])
+    return arr, blk
+
+
+def mixed_chains(buckets, block=None, stream=None) -> None:
+    """One launch per bucket, in order, on `stream` (lf_mixed_chains)."""
+    L = lib()
+    _mixed_sigs(L)
+    arr, blk = _bucket_array(buckets, block)
+    check(L.lf_mixed_chains(arr, len(buckets), blk, stream))
+
+
+def mixed_graph(buckets, block=None) -> int:
+    """A graph with one kernel node per bucket (lf_mixed_graph_create); launch with graph_launch."""
+    L = lib()
+    _mixed_sigs(L)
+    arr, blk = _bucket_array(buckets, block)
+    g = C.c_void_p()
+    check(L.lf_mixed_graph_create(arr, len(buckets), blk, C.byref(g)))
+    return g.value
+
+
+def graph_launch(graph, stream=None) -> None:
+    check(lib().lf_graph_launch(graph, stream))
+
+
+def graph_destroy(graph) -> None:
+    check(lib().lf_graph_destroy(graph))
+
+
+def autotune_mixed(bits: int, sample: int = 32768):
+    """lf_autotune_mixed: the fastest workgroup size of the bucket kernel at `bits` (cached per device)."""
+    L = lib()
+    _mixed_sigs(L)
+    b, ms = C.c_uint(), C.c_double()
+    check(L.lf_autotune_mixed(bits, sample, C.byref(b), C.byref(ms)))
+    return b.value, ms.value

@@ -296,10 +296,29 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
     FACTOR_HIP(hipMemsetAsync(derr.p, 0, 4, s), "hipMemsetAsync");
     double gpu = 0;
     RegionFields lf_, rf_;
+    hipEvent_t e0, e1;
+    FACTOR_HIP(hipEventCreate(&e0), "hipEventCreate");
+    FACTOR_HIP(hipEventCreate(&e1), "hipEventCreate");
+    struct EventGuard {
+        hipEvent_t a, b;
+        ~EventGuard()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{e0, e1};
+    // the in-block kernel between two events; its time is collected after the stream's next synchronization
     auto block_solve = [&](int backward, uint64_t r0, uint64_t r1) {
         lf::SolveArgs a{F, fpitch, Xp, xpitch, uint32_t(n), uint32_t(nrhs), uint32_t(r0), uint32_t(r1), (uint32_t*)dscratch.p, swords,
                         (int32_t*)derr.p};
-        return tables().solve[N](backward, a, s);
+        hipError_t e = hipEventRecord(e0, s);
+        if (e == hipSuccess) e = tables().solve[N](backward, a, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        return e;
+    };
+    auto collect = [&]() {
+        float t = 0;
+        if (hipEventElapsedTime(&t, e0, e1) == hipSuccess) gpu += t * 1e-3;
     };
     float ms = 0;
     if (passes & 1)
@@ -311,12 +330,17 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
             if (int r = lf_.download(F, fpitch, N, r1 * n + r0, n, n - r1, K, s)) return r;
             if (int r = rf_.download(Xp, xpitch, N, r0 * nrhs, nrhs, K, nrhs, s)) return r;
             FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
+            collect();
             const lf::MatView left{F + (r1 * n + r0), fpitch, n, 1}, right{Xp + r0 * nrhs, xpitch, 1, nrhs};
             if (int r = update(bits, lo, false, left, lf_.fields(K, 1), right, rf_.fields(1, nrhs), n - r1, nrhs, K, Xp + r1 * nrhs, xpitch,
                                nrhs, s, ms))
                 return r;
             gpu += ms * 1e-3;
         }
+    if (passes == 3) { // the forward pass's last block
+        FACTOR_HIP(hipEventSynchronize(e1), "device execution (solve)");
+        collect();
+    }
     if (passes & 2)
         for (uint64_t r0 = (n - 1) / nb * nb;; r0 -= nb) {
             const uint64_t r1 = std::min(n, r0 + nb), K = r1 - r0;
@@ -326,6 +350,7 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
             if (int r = lf_.download(F, fpitch, N, r0 * n, n, K, r0, s)) return r;
             if (int r = rf_.download(Xp, xpitch, N, r0 * nrhs, nrhs, K, nrhs, s)) return r;
             FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
+            collect();
             const lf::MatView left{F + r0 * n, fpitch, 1, n}, right{Xp + r0 * nrhs, xpitch, 1, nrhs};
             if (int r = update(bits, lo, false, left, lf_.fields(1, r0), right, rf_.fields(1, nrhs), r0, nrhs, K, Xp, xpitch, nrhs, s, ms))
                 return r;
@@ -334,6 +359,7 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
     int32_t err = 0;
     FACTOR_HIP(hipMemcpyAsync(&err, derr.p, 4, hipMemcpyDeviceToHost, s), "download status");
     FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
+    collect(); // the last block of the last pass
     if (err) return fail(LF_LOGIC_ERROR, "solve: exact-dot scratch overflow");
     if (gpu_seconds) *gpu_seconds = gpu;
     if (wall_seconds) *wall_seconds = since(start);

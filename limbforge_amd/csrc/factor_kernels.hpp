@@ -173,17 +173,88 @@ __device__ __noinline__ Num<N> dot_sub_clusters(const Num<N>& c, const VecView& 
     return pack<N, EW>(ext, scale1 - 32 * DW, sign1, ST_OK);
 }
 
+// ---------------------------------------------------------------- one exact dot per wavefront
 // D(c; x, y; K) = RN(c - sum_{k<K} x_k y_k) with one rounding (the contract of the reference's
 // exact_dot_add with subtract, linalg.hpp:268-318): zero with the OR of the statuses of c and of every
 // x_k, y_k; c itself when no product is nonzero.
-template <int N> __device__ Num<N> dot_sub(const Num<N>& c, const VecView& x, const VecView& y, u32 K, const DotScratch& sc)
+//
+// One wavefront computes one D.  Lane l forms the exact products of terms k = l, l+64, ... and writes
+// each into its LDS row already bit-aligned to the window (2N+1 words), with its word offset and sign;
+// lane w then sums window column w over the rows into a signed 64-bit column sum (no carries between
+// columns); lane 0 resolves the carries, negates a negative window and rounds once.  A window of
+// W = 2N + 24 words holds terms spread over up to ~700 bits; wider spreads take the cluster path on
+// lane 0.  Every lane of the wave calls it with the same arguments; the result is valid in lane 0.
+constexpr int WAVE = 64;
+
+template <int N> struct WaveLds {
+    u32 word[WAVE + 1][2 * N + 1]; // row WAVE holds c
+    int32_t off[WAVE + 1];
+    int32_t sgn[WAVE + 1];
+    i64 col[factor_window(N)];
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ long long wave_max(long long v)
+{
+#pragma unroll
+    for (int m = 32; m; m >>= 1) {
+        const long long o = __shfl_xor(v, m);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ long long wave_min(long long v)
+{
+#pragma unroll
+    for (int m = 32; m; m >>= 1) {
+        const long long o = __shfl_xor(v, m);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int m = 32; m; m >>= 1) v += __shfl_xor(v, m);
+    return v;
+}
+__device__ __forceinline__ int wave_or(int v)
+{
+#pragma unroll
+    for (int m = 32; m; m >>= 1) v |= __shfl_xor(v, m);
+    return v;
+}
+
+// row r of the wave's LDS rows = src (S words) * 2^(rel mod 32), word offset rel / 32, sign sg
+template <int N, int S> __device__ __forceinline__ void put_row(WaveLds<N>& L, int r, const u32 (&src)[S], i64 rel, int sg)
+{
+    const u32 b = (u32)(rel & 31);
+#pragma unroll
+    for (int t = 0; t <= 2 * N; ++t) {
+        const u32 hi = t < S ? src[t] : 0u, lo = (t > 0 && t - 1 < S) ? src[t - 1] : 0u;
+        L.word[r][t] = b ? ((hi << b) | (lo >> (32u - b))) : hi;
+    }
+    L.off[r] = (int32_t)(rel >> 5);
+    L.sgn[r] = sg;
+}
+
+template <int N>
+__device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView& y, u32 K, u32 lane, WaveLds<N>& L, const DotScratch& sc)
 {
     constexpr int W = factor_window(N);
+    constexpr int CPL = (W + WAVE - 1) / WAVE; // window columns per lane
     constexpr i64 NONE = -((i64)1 << 62);
-    u32 st = c.st;
+    // pass 1: statuses, scale range and count of the nonzero products
+    u32 st = 0;
     i64 hi = NONE, lo = -NONE;
-    u32 count = 0;
-    for (u32 k = 0; k < K; ++k) {
+    int count = 0;
+    for (u32 k = lane; k < K; k += WAVE) {
         int ex, sx, ey, sy;
         u32 tx, ty;
         vec_meta<N>(x, k, ex, sx, tx);
@@ -196,8 +267,12 @@ template <int N> __device__ Num<N> dot_sub(const Num<N>& c, const VecView& x, co
             ++count;
         }
     }
+    st = (u32)wave_or((int)st) | c.st;
     if (st) return zero<N>(st);
+    count = wave_sum(count);
     if (!count) return c;
+    hi = wave_max(hi);
+    lo = wave_min(lo);
     if (c.s) {
         const i64 s = (i64)c.e - (32 * N - 1);
         hi = s > hi ? s : hi;
@@ -205,36 +280,79 @@ template <int N> __device__ Num<N> dot_sub(const Num<N>& c, const VecView& x, co
         ++count;
     }
     int logc = 0;
-    while ((1u << logc) < count) ++logc;
-    if (hi - lo + 64 * N + logc + 1 > 32 * W) return dot_sub_clusters<N>(c, x, y, K, sc);
-    u32 acc[W];
+    while ((1 << logc) < count) ++logc;
+    if (hi - lo + 64 * N + logc + 1 > 32 * W) {
+        Num<N> r = zero<N>(ST_OK);
+        if (lane == 0) r = dot_sub_clusters<N>(c, x, y, K, sc);
+        return r;
+    }
+    // pass 2: rows of exact products, column sums
+    i64 S[CPL];
 #pragma unroll
-    for (int w = 0; w < W; ++w) acc[w] = 0;
-    for (u32 k = 0; k < K; ++k) {
-        const Num<N> a = vec_load<N>(x, k), b = vec_load<N>(y, k);
-        if (!a.s || !b.s) continue;
-        u32 p[2 * N], w[W];
-        mantissa_product<N>(a.w, b.w, p);
-        place<W, 2 * N>(w, p, (i64)a.e + b.e - 2 * (32 * N - 1) - lo);
-        accumulate_signed<W>(acc, w, -a.s * b.s);
+    for (int t = 0; t < CPL; ++t) S[t] = 0;
+    for (u32 kb = 0; kb < K; kb += WAVE) {
+        const u32 k = kb + lane;
+        int sg = 0;
+        if (k < K) {
+            const Num<N> a = vec_load<N>(x, k), b = vec_load<N>(y, k);
+            sg = -a.s * b.s;
+            if (sg) {
+                u32 p[2 * N];
+                mantissa_product<N>(a.w, b.w, p);
+                put_row<N, 2 * N>(L, (int)lane, p, (i64)a.e + b.e - 2 * (32 * N - 1) - lo, sg);
+            }
+        }
+        if (!sg) L.sgn[lane] = 0;
+        const bool with_c = kb == 0 && c.s != 0;
+        if (with_c && lane == 0) put_row<N, N>(L, WAVE, c.w, (i64)c.e - (32 * N - 1) - lo, c.s);
+        wave_sync();
+        const int rows = (int)(K - kb < (u32)WAVE ? K - kb : (u32)WAVE);
+        for (int r = 0; r <= rows; ++r) {
+            const int row = r == rows ? WAVE : r;
+            if (row == WAVE && !with_c) break;
+            const int g = L.sgn[row];
+            if (!g) continue;
+            const int o = L.off[row];
+#pragma unroll
+            for (int t = 0; t < CPL; ++t) {
+                const int d = (int)lane + WAVE * t - o;
+                if (d >= 0 && d <= 2 * N) {
+                    const i64 v = (i64)L.word[row][d];
+                    S[t] += g > 0 ? v : -v;
+                }
+            }
+        }
+        wave_sync();
     }
-    if (c.s) {
-        u32 w[W];
-        place<W, N>(w, c.w, (i64)c.e - (32 * N - 1) - lo);
-        accumulate_signed<W>(acc, w, c.s);
-    }
-    int sign = 1;
-    if (acc[W - 1] >> 31) {
-        sign = -1;
-        u32 cy = 1;
+#pragma unroll
+    for (int t = 0; t < CPL; ++t)
+        if ((int)lane + WAVE * t < W) L.col[lane + WAVE * t] = S[t];
+    wave_sync();
+    Num<N> r = zero<N>(ST_OK);
+    if (lane == 0) {
+        u32 acc[W];
+        i64 cy = 0;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const u64 v = (u64)(u32)~acc[w] + cy;
+            const i64 v = L.col[w] + cy;
             acc[w] = (u32)v;
-            cy = (u32)(v >> 32);
+            cy = v >> 32;
         }
+        int sign = 1;
+        if (acc[W - 1] >> 31) {
+            sign = -1;
+            u32 cc = 1;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const u64 v = (u64)(u32)~acc[w] + cc;
+                acc[w] = (u32)v;
+                cc = (u32)(v >> 32);
+            }
+        }
+        r = pack<N, W>(acc, lo, sign, ST_OK);
     }
-    return pack<N, W>(acc, lo, sign, ST_OK);
+    wave_sync(); // the rows and columns are reused by the wave's next dot
+    return r;
 }
 
 // ---------------------------------------------------------------- Cholesky
@@ -243,54 +361,67 @@ template <int N> __device__ __forceinline__ Num<N> element(const FactorArgs& f, 
     return load_out<N>(f.w, f.pitch, (u64)i * f.n + j);
 }
 
-// The diagonal block [k0, k1) of a panel, column by column, in one workgroup: the pivot of column j on
-// thread 0, then the entries below it in the block on every thread.  Stops at a failing pivot.
-template <int N> __global__ void __launch_bounds__(256) chol_diag_kernel(FactorArgs f)
+// Waves of the diagonal-block workgroup: as many as the LDS rows of one wave each allow in 48 KB.
+template <int N> constexpr int diag_waves()
 {
+    constexpr int w = (int)(49152 / sizeof(WaveLds<N>));
+    return w < 1 ? 1 : (w > 16 ? 16 : w);
+}
+
+// The diagonal block [k0, k1) of a panel, column by column, in one workgroup of diag_waves<N>()
+// wavefronts: the pivot of column j on wave 0 (exact dot, then sqrt on lane 0), then the entries below
+// it in the block, one row per wave at a time.  Stops at a failing pivot.
+template <int N> __global__ void __launch_bounds__(1024) chol_diag_kernel(FactorArgs f)
+{
+    constexpr int DW = diag_waves<N>();
+    __shared__ WaveLds<N> lds[DW];
     __shared__ int stop;
-    const u32 t = threadIdx.x, T = blockDim.x;
-    const DotScratch sc{f.scratch + (u64)t * 2 * f.swords, f.swords, &f.info->error};
+    const u32 wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    WaveLds<N>& L = lds[wave];
+    const DotScratch sc{f.scratch + (u64)wave * 2 * f.swords, f.swords, &f.info->error};
     for (u32 j = f.k0; j < f.k1; ++j) {
         const VecView rj{f.w, f.pitch, (i64)j * f.n + f.k0, 1};
-        if (t == 0) {
-            const Num<N> s = dot_sub<N>(element<N>(f, j, j), rj, rj, j - f.k0, sc);
-            if (s.st || s.s <= 0) {
-                f.info->pivot = (int32_t)j;
-                f.info->sign = s.st ? 0 : s.s;
-                f.info->status = s.st;
-                stop = 1;
-            } else {
-                store_out<N>(f.w, f.pitch, f.n, j, j, false, sqrt<N>(s));
-                stop = 0;
+        if (wave == 0) {
+            const Num<N> s = wave_dot_sub<N>(element<N>(f, j, j), rj, rj, j - f.k0, lane, L, sc);
+            if (lane == 0) {
+                if (s.st || s.s <= 0) {
+                    f.info->pivot = (int32_t)j;
+                    f.info->sign = s.st ? 0 : s.s;
+                    f.info->status = s.st;
+                    stop = 1;
+                } else {
+                    store_out<N>(f.w, f.pitch, f.n, j, j, false, sqrt<N>(s));
+                    stop = 0;
+                }
             }
         }
         __syncthreads();
         if (stop) return;
-        const Num<N> d = element<N>(f, j, j);
-        for (u32 i = j + 1 + t; i < f.k1; i += T) {
+        for (u32 i = j + 1 + wave; i < f.k1; i += DW) {
             const VecView ri{f.w, f.pitch, (i64)i * f.n + f.k0, 1};
-            const Num<N> v = dot_sub<N>(element<N>(f, i, j), ri, rj, j - f.k0, sc);
-            store_out<N>(f.w, f.pitch, f.n, i, j, false, div<N>(v, d));
+            const Num<N> v = wave_dot_sub<N>(element<N>(f, i, j), ri, rj, j - f.k0, lane, L, sc);
+            if (lane == 0) store_out<N>(f.w, f.pitch, f.n, i, j, false, div<N>(v, element<N>(f, j, j)));
         }
         __syncthreads();
     }
 }
 
-// The rows below the diagonal block, one thread per row i >= k1: columns [k0, p) in order, p the failing
-// pivot of the block or k1.
+// The rows below the diagonal block, one wavefront per row i >= k1: columns [k0, p) in order, p the
+// failing pivot of the block or k1.
 template <int N> __global__ void __launch_bounds__(64) chol_panel_kernel(FactorArgs f)
 {
-    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ WaveLds<N> L;
+    const u32 r = blockIdx.x, lane = threadIdx.x;
     const u32 i = f.k1 + r;
-    if (i >= f.n) return;
     const int32_t piv = f.info->pivot;
     const u32 pe = piv >= 0 ? (u32)piv : f.k1;
     const DotScratch sc{f.scratch + (u64)r * 2 * f.swords, f.swords, &f.info->error};
     const VecView ri{f.w, f.pitch, (i64)i * f.n + f.k0, 1};
     for (u32 j = f.k0; j < pe; ++j) {
         const VecView rj{f.w, f.pitch, (i64)j * f.n + f.k0, 1};
-        const Num<N> v = dot_sub<N>(element<N>(f, i, j), ri, rj, j - f.k0, sc);
-        store_out<N>(f.w, f.pitch, f.n, i, j, false, div<N>(v, element<N>(f, j, j)));
+        const Num<N> v = wave_dot_sub<N>(element<N>(f, i, j), ri, rj, j - f.k0, lane, L, sc);
+        if (lane == 0) store_out<N>(f.w, f.pitch, f.n, i, j, false, div<N>(v, element<N>(f, j, j)));
+        __syncthreads(); // entry (i, j) is a term of the row's next dot
     }
 }
 
@@ -314,34 +445,36 @@ template <int N> __global__ void __launch_bounds__(256) zero_upper_kernel(u32* w
 }
 
 // ---------------------------------------------------------------- triangular solves
-// Forward block [r0, r1) of L X = B, one thread per right-hand side.
+// Forward block [r0, r1) of L X = B, one wavefront per right-hand side.
 template <int N> __global__ void __launch_bounds__(64) solve_forward_kernel(SolveArgs a)
 {
-    const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.nrhs) return;
+    __shared__ WaveLds<N> L;
+    const u32 c = blockIdx.x, lane = threadIdx.x;
     const DotScratch sc{a.scratch + (u64)c * 2 * a.swords, a.swords, a.error};
     const VecView xc{a.x, a.xpitch, (i64)a.r0 * a.nrhs + c, (i64)a.nrhs};
     for (u32 i = a.r0; i < a.r1; ++i) {
         const VecView li{a.f, a.fpitch, (i64)i * a.n + a.r0, 1};
-        const Num<N> v = dot_sub<N>(load_out<N>(a.x, a.xpitch, (u64)i * a.nrhs + c), li, xc, i - a.r0, sc);
-        store_out<N>(a.x, a.xpitch, a.nrhs, i, c, false, div<N>(v, load_out<N>(a.f, a.fpitch, (u64)i * a.n + i)));
+        const Num<N> v = wave_dot_sub<N>(load_out<N>(a.x, a.xpitch, (u64)i * a.nrhs + c), li, xc, i - a.r0, lane, L, sc);
+        if (lane == 0) store_out<N>(a.x, a.xpitch, a.nrhs, i, c, false, div<N>(v, load_out<N>(a.f, a.fpitch, (u64)i * a.n + i)));
+        __syncthreads(); // x_i is a term of the next row's dot
     }
 }
 
-// Backward block [r0, r1) of L^T X = B (column i of L as the vector), one thread per right-hand side.
+// Backward block [r0, r1) of L^T X = B (column i of L as the vector), one wavefront per right-hand side.
 template <int N> __global__ void __launch_bounds__(64) solve_backward_kernel(SolveArgs a)
 {
-    const u32 c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.nrhs) return;
+    __shared__ WaveLds<N> L;
+    const u32 c = blockIdx.x, lane = threadIdx.x;
     const DotScratch sc{a.scratch + (u64)c * 2 * a.swords, a.swords, a.error};
     for (u32 i = a.r1; i-- > a.r0;) {
         Num<N> v = load_out<N>(a.x, a.xpitch, (u64)i * a.nrhs + c);
         if (i + 1 < a.r1) {
             const VecView col{a.f, a.fpitch, (i64)(i + 1) * a.n + i, (i64)a.n};
             const VecView xc{a.x, a.xpitch, (i64)(i + 1) * a.nrhs + c, (i64)a.nrhs};
-            v = dot_sub<N>(v, col, xc, a.r1 - i - 1, sc);
+            v = wave_dot_sub<N>(v, col, xc, a.r1 - i - 1, lane, L, sc);
         }
-        store_out<N>(a.x, a.xpitch, a.nrhs, i, c, false, div<N>(v, load_out<N>(a.f, a.fpitch, (u64)i * a.n + i)));
+        if (lane == 0) store_out<N>(a.x, a.xpitch, a.nrhs, i, c, false, div<N>(v, load_out<N>(a.f, a.fpitch, (u64)i * a.n + i)));
+        __syncthreads();
     }
 }
 

@@ -10,15 +10,16 @@ namespace lf {
 
 template <int N> hipError_t launch_chol_diag(const FactorArgs& f, uint32_t threads, hipStream_t s)
 {
+    (void)threads;
     if (f.k1 <= f.k0) return hipSuccess;
-    hipLaunchKernelGGL((chol_diag_kernel<N>), dim3(1), dim3(threads), 0, s, f);
+    hipLaunchKernelGGL((chol_diag_kernel<N>), dim3(1), dim3(diag_waves<N>() * WAVE), 0, s, f);
     return hipGetLastError();
 }
 
 template <int N> hipError_t launch_chol_panel(const FactorArgs& f, hipStream_t s)
 {
     if (f.k1 >= f.n) return hipSuccess;
-    hipLaunchKernelGGL((chol_panel_kernel<N>), dim3((f.n - f.k1 + 63u) / 64u), dim3(64), 0, s, f);
+    hipLaunchKernelGGL((chol_panel_kernel<N>), dim3(f.n - f.k1), dim3(WAVE), 0, s, f);
     return hipGetLastError();
 }
 
@@ -41,7 +42,7 @@ template <int N> hipError_t launch_zero_upper(uint32_t* w, uint64_t pitch, uint3
 template <int N> hipError_t launch_solve(int backward, const SolveArgs& a, hipStream_t s)
 {
     if (!a.nrhs || a.r1 <= a.r0) return hipSuccess;
-    const dim3 grid((a.nrhs + 63u) / 64u), block(64);
+    const dim3 grid(a.nrhs), block(WAVE);
     if (backward) hipLaunchKernelGGL((solve_backward_kernel<N>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((solve_forward_kernel<N>), grid, block, 0, s, a);
     return hipGetLastError();

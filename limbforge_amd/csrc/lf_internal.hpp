@@ -31,9 +31,27 @@ struct ChainArgs {
     int steps;
 };
 
+// One precision bucket of the mixed-precision chains: value i runs len[i] steps, step t being
+// x <- RN(x * f + g) when bit t of mask[i] is set and x <- RN(x * f) otherwise; x, f, g and out are
+// limb planes of `count` values with one pitch.
+struct MixedArgs {
+    const uint32_t* x;
+    const uint32_t* f;
+    const uint32_t* g;
+    uint32_t* out;
+    uint64_t pitch;
+    const uint32_t* len;
+    const uint64_t* mask;
+    uint32_t count;
+};
+
 // Element-wise operation `op` (numbering of limbforge::Operation) on N-word values.
 template <int N> hipError_t launch_elementwise(int op, const ElemArgs& g, hipStream_t s);
 // x <- RN(x * f) `steps` times with x held in registers (one launch for the whole chain).
 template <int N> hipError_t launch_mul_chain(const ChainArgs& g, hipStream_t s);
+// The mixed chains of one bucket with `block` lanes per workgroup (64, 128, 256 or 512).
+template <int N> hipError_t launch_mixed_chain(const MixedArgs& g, unsigned block, hipStream_t s);
+// The kernel entry of launch_mixed_chain, for graph kernel nodes (arguments: one MixedArgs).
+template <int N> const void* mixed_chain_entry();
 
 } // namespace lf

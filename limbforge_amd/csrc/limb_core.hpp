@@ -535,10 +535,11 @@ template <int N> __device__ __forceinline__ Num<N> div(const Num<N>& a, const Nu
 }
 
 // ------------------------------------------------------------------- sqrt
-// RN(sqrt(a)): restoring integer square root of Z = mantissa * 2^(32N-1+odd), rounded up exactly when
-// Z - Q^2 > Q (contract of core.hpp:216-235; docs/numerics.md "Square and square root").  Only the
-// top N+1 words of Z can be nonzero.  The radicand is consumed two bits at a time from a shift register.
-template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
+// RN(sqrt(a)), bit-serial: restoring integer square root of Z = mantissa * 2^(32N-1+odd), rounded up
+// exactly when Z - Q^2 > Q (contract of core.hpp:216-235; docs/numerics.md "Square and square root").
+// Only the top N+1 words of Z can be nonzero.  The radicand is consumed two bits at a time from a shift
+// register.  The fallback of sqrt below.
+template <int N> __device__ __noinline__ Num<N> sqrt_restoring(const Num<N>& a)
 {
     if (a.st || !a.s) return zero<N>(a.st);
     if (a.s < 0) return zero<N>(ST_INVALID);
@@ -620,6 +621,130 @@ template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
     }
     r.e = e;
     return r;
+}
+
+// RN(sqrt(a)).  The mantissa m (N words, top bit set) and the exponent parity define the integer
+// X = m * 2^(32N-1+odd) < 2^(64N), whose integer square root r has exactly 32N bits; the result is r, plus
+// one ulp exactly when X - r^2 > r (the contract of core.hpp:216-235: no ties are possible).  r comes from
+// Newton's iteration y <- (y + X/y) / 2 in N-word arithmetic started from a double-precision seed, and is
+// then made exact by integer checks r^2 <= X < (r+1)^2 (a squaring and a compare each), so the result
+// does not depend on how close the iteration gets.  Should the checks not settle within a few steps, the
+// bit-serial restoring method decides.
+template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
+{
+    if (a.st || !a.s) return zero<N>(a.st);
+    if (a.s < 0) return zero<N>(ST_INVALID);
+    int e = a.e / 2;
+    if (a.e < 0 && (a.e % 2)) --e;
+    const int odd = a.e - 2 * e;
+    // X: words 0..N-2 zero, words N-1..2N-1 = m << (31 + odd) in N+1 words
+    u32 X[2 * N];
+#pragma unroll
+    for (int i = 0; i < N - 1; ++i) X[i] = 0;
+#pragma unroll
+    for (int i = 0; i <= N; ++i) {
+        const u32 hi = i < N ? a.w[i] : 0u, lo = i ? a.w[i - 1] : 0u;
+        X[N - 1 + i] = __builtin_amdgcn_alignbit(hi, lo, odd ? 0u : 1u);
+    }
+    // seed: sqrt of the top 64 mantissa bits in double precision, as an N-word value of exponent 32N-1
+    const u64 top = ((u64)a.w[N - 1] << 32) | a.w[N - 2];
+    const double u = (double)top * (odd ? 1.0 : 0.5);   // in [2^62, 2^64)
+    const u64 s0 = (u64)(__builtin_sqrt(u) * 2147483648.0) << 1; // sqrt(u) * 2^32, in [2^63, 2^64)
+    Num<N> v, y;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        v.w[i] = a.w[i];
+        y.w[i] = i == N - 1 ? (u32)(s0 >> 32) : (i == N - 2 ? (u32)s0 : 0u);
+    }
+    v.e = 64 * N - 2 + odd; // value X
+    v.s = y.s = 1;
+    v.st = y.st = ST_OK;
+    y.e = 32 * N - 1;
+    // Newton steps: the seed's ~50 correct bits double per step
+    constexpr int STEPS = 32 * N <= 50 ? 1 : 32 * N <= 100 ? 2 : 32 * N <= 200 ? 3 : 32 * N <= 400 ? 4 : 32 * N <= 800 ? 5 : 6;
+#pragma unroll 1
+    for (int it = 0; it < STEPS; ++it) {
+        y = add<N>(y, div<N>(v, y));
+        y.e -= 1;
+    }
+    u32 r[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) r[i] = y.e == 32 * N - 1 ? y.w[i] : (y.e > 32 * N - 1 ? 0xffffffffu : (i == N - 1 ? 0x80000000u : 0u));
+    // exact correction: r^2 <= X < (r+1)^2
+    u32 rem[2 * N];
+    bool done = false;
+#pragma unroll 1
+    for (int guard = 0; guard < 6 && !done; ++guard) {
+        u32 sq[2 * N];
+        mantissa_product<N>(r, r, sq);
+        u32 bw = 0;
+#pragma unroll
+        for (int i = 0; i < 2 * N; ++i) {
+            const u64 t = (u64)X[i] - sq[i] - bw;
+            rem[i] = (u32)t;
+            bw = (u32)(t >> 63);
+        }
+        if (bw) { // r^2 > X
+            u32 b = 1;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const u64 t = (u64)r[i] - b;
+                r[i] = (u32)t;
+                b = (u32)(t >> 63);
+            }
+            continue;
+        }
+        // (r+1)^2 <= X  <=>  X - r^2 >= 2r + 1
+        u32 t2[N + 1];
+#pragma unroll
+        for (int i = 0; i <= N; ++i) {
+            const u32 hi = i < N ? r[i] : 0u, lo = i ? r[i - 1] : 0u;
+            t2[i] = __builtin_amdgcn_alignbit(hi, lo, 31u);
+        }
+        t2[0] |= 1u;
+        bool high = false;
+#pragma unroll
+        for (int i = N + 1; i < 2 * N; ++i) high = high || rem[i] != 0;
+        u32 rl[N + 1];
+#pragma unroll
+        for (int i = 0; i <= N; ++i) rl[i] = rem[i];
+        if (high || compare_words<N + 1>(rl, t2) >= 0) {
+            u32 c = 1;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const u64 t = (u64)r[i] + c;
+                r[i] = (u32)t;
+                c = (u32)(t >> 32);
+            }
+            continue;
+        }
+        done = true;
+    }
+    if (!done) return sqrt_restoring<N>(a);
+    // round up iff X - r^2 > r (the remainder is below 2r + 1 < 2^(32N+1))
+    u32 rr[N + 1], rl[N + 1];
+#pragma unroll
+    for (int i = 0; i <= N; ++i) {
+        rr[i] = i < N ? r[i] : 0u;
+        rl[i] = rem[i];
+    }
+    const bool gt = compare_words<N + 1>(rl, rr) > 0;
+    Num<N> res;
+    res.s = 1;
+    res.st = ST_OK;
+    u32 c = gt ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const u64 t = (u64)r[i] + c;
+        res.w[i] = (u32)t;
+        c = (u32)(t >> 32);
+    }
+    if (c) {
+        res.w[N - 1] = 0x80000000u;
+        e += 1;
+    }
+    res.e = e;
+    return res;
 }
 
 // -------------------------------------------------------------------- fma

@@ -1,0 +1,229 @@
+// mixed.hip — mixed-precision chain batches behind lf_mixed_chains / lf_mixed_graph_create, and the
+// launch-shape autotuner lf_autotune_mixed.
+//
+// A batch is a list of precision buckets (all values of one width, already resident as limb planes).
+// lf_mixed_chains issues one launch per bucket in order on a stream: the plain "one launch per
+// precision" form.  lf_mixed_graph_create builds one hipGraph whose kernel nodes are the buckets'
+// launches with no edges between them, so one graph launch runs every precision concurrently and small
+// buckets fill the CUs that large ones leave idle.  The autotuner times the bucket kernel of a width on a
+// synthetic sample for each workgroup size and keeps the fastest per device and width.
+#include "lf_internal.hpp"
+#include "limbforge_hip.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <random>
+#include <string>
+#include <vector>
+
+void lf_set_last_error(const std::string& message);
+
+namespace {
+
+int fail(int code, const std::string& m)
+{
+    lf_set_last_error(m);
+    return code;
+}
+
+#define MIXED_HIP(call, where)                                                                                         \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) return fail(LF_RUNTIME_ERROR, std::string(where) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+typedef hipError_t (*mixed_fn)(const lf::MixedArgs&, unsigned, hipStream_t);
+typedef const void* (*entry_fn)();
+struct MixedTables;
+template <int N> void fill_mixed(MixedTables& t);
+struct MixedTables {
+    mixed_fn launch[33] = {};
+    entry_fn entry[33] = {};
+    MixedTables() { fill_mixed<2>(*this); }
+};
+template <int N> void fill_mixed(MixedTables& t)
+{
+    t.launch[N] = lf::launch_mixed_chain<N>;
+    t.entry[N] = lf::mixed_chain_entry<N>;
+    if constexpr (N < 32) fill_mixed<N + 1>(t);
+}
+const MixedTables& tables()
+{
+    static const MixedTables t;
+    return t;
+}
+
+bool valid_bits(int bits) { return bits >= 64 && bits <= 1024 && bits % 32 == 0; }
+bool valid_block(unsigned b) { return b == 64 || b == 128 || b == 256 || b == 512; }
+
+int check_bucket(const lf_chain_bucket& b)
+{
+    if (!valid_bits(b.bits)) return fail(LF_INVALID_ARGUMENT, "bucket bits must be a multiple of 32 in [64,1024]");
+    if (b.count > 0xffffffffull) return fail(LF_INVALID_ARGUMENT, "bucket exceeds 32-bit indexing");
+    if (b.count && (!b.x.planes || !b.f.planes || !b.g.planes || !b.out.planes || !b.len || !b.mask))
+        return fail(LF_INVALID_ARGUMENT, "null bucket operand");
+    if (b.count && (b.x.pitch < b.count || b.x.pitch != b.f.pitch || b.x.pitch != b.g.pitch || b.x.pitch != b.out.pitch))
+        return fail(LF_INVALID_ARGUMENT, "bucket operands need one pitch of at least count");
+    return LF_OK;
+}
+
+lf::MixedArgs args_of(const lf_chain_bucket& b)
+{
+    return {(const uint32_t*)b.x.planes, (const uint32_t*)b.f.planes, (const uint32_t*)b.g.planes, (uint32_t*)b.out.planes, b.x.pitch,
+            b.len, b.mask, uint32_t(b.count)};
+}
+
+std::mutex tuned_mutex;
+std::map<std::pair<int, int>, unsigned> tuned; // (device, bits) -> workgroup size
+
+unsigned block_for(const unsigned* block, int i, int bits)
+{
+    if (block && block[i]) return block[i];
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lock(tuned_mutex);
+    auto it = tuned.find({dev, bits});
+    return it == tuned.end() ? 256u : it->second;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+} // namespace
+
+extern "C" {
+
+int lf_mixed_chains(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, void* stream)
+{
+    if (nbuckets < 0 || (nbuckets && !buckets)) return fail(LF_INVALID_ARGUMENT, "bad bucket list");
+    for (int i = 0; i < nbuckets; ++i) {
+        if (int r = check_bucket(buckets[i])) return r;
+        if (block && block[i] && !valid_block(block[i])) return fail(LF_INVALID_ARGUMENT, "workgroup size must be 64, 128, 256 or 512");
+    }
+    for (int i = 0; i < nbuckets; ++i) {
+        const lf_chain_bucket& b = buckets[i];
+        MIXED_HIP(tables().launch[b.bits / 32](args_of(b), block_for(block, i, b.bits), (hipStream_t)stream), "mixed chain launch");
+    }
+    return LF_OK;
+}
+
+int lf_mixed_graph_create(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, void** graph_exec)
+{
+    if (!graph_exec) return fail(LF_INVALID_ARGUMENT, "null pointer");
+    *graph_exec = nullptr;
+    if (nbuckets < 0 || (nbuckets && !buckets)) return fail(LF_INVALID_ARGUMENT, "bad bucket list");
+    for (int i = 0; i < nbuckets; ++i) {
+        if (int r = check_bucket(buckets[i])) return r;
+        if (block && block[i] && !valid_block(block[i])) return fail(LF_INVALID_ARGUMENT, "workgroup size must be 64, 128, 256 or 512");
+    }
+    hipGraph_t g = nullptr;
+    MIXED_HIP(hipGraphCreate(&g, 0), "hipGraphCreate");
+    struct GraphGuard {
+        hipGraph_t g;
+        ~GraphGuard() { (void)hipGraphDestroy(g); }
+    } guard{g};
+    std::vector<lf::MixedArgs> args(nbuckets);
+    for (int i = 0; i < nbuckets; ++i) {
+        const lf_chain_bucket& b = buckets[i];
+        if (!b.count) continue;
+        args[i] = args_of(b);
+        const unsigned blk = block_for(block, i, b.bits);
+        void* params[] = {&args[i]};
+        hipKernelNodeParams p;
+        std::memset(&p, 0, sizeof p);
+        p.func = const_cast<void*>(tables().entry[b.bits / 32]());
+        p.gridDim = dim3(unsigned((b.count + blk - 1) / blk));
+        p.blockDim = dim3(blk);
+        p.sharedMemBytes = 0;
+        p.kernelParams = params;
+        p.extra = nullptr;
+        hipGraphNode_t node;
+        MIXED_HIP(hipGraphAddKernelNode(&node, g, nullptr, 0, &p), "hipGraphAddKernelNode");
+    }
+    hipGraphExec_t x = nullptr;
+    MIXED_HIP(hipGraphInstantiate(&x, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    *graph_exec = x;
+    return LF_OK;
+}
+
+int lf_autotune_mixed(int bits, uint64_t sample, unsigned* block, double* best_ms)
+{
+    if (!valid_bits(bits)) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    if (!block) return fail(LF_INVALID_ARGUMENT, "null pointer");
+    if (sample < 64 || sample > (uint64_t(1) << 24)) return fail(LF_INVALID_ARGUMENT, "autotune sample must be in [64, 2^24]");
+    const int N = bits / 32;
+    const uint64_t pitch = lf_plane_pitch(sample);
+    // synthetic sample: random normalized operands (exponents in [-64, 64]), 32-step chains, half fma
+    std::mt19937_64 rng(20261007u + unsigned(bits));
+    std::vector<uint32_t> planes(3 * (N + 3) * pitch, 0);
+    for (int o = 0; o < 3; ++o) {
+        uint32_t* p = planes.data() + o * (N + 3) * pitch;
+        for (uint64_t i = 0; i < sample; ++i) {
+            for (int k = 0; k < N; ++k) p[k * pitch + i] = uint32_t(rng());
+            p[(N - 1) * pitch + i] |= 0x80000000u;
+            p[N * pitch + i] = uint32_t(int(rng() % 129) - 64);
+            p[(N + 1) * pitch + i] = rng() & 1 ? 1u : uint32_t(-1);
+        }
+    }
+    std::vector<uint32_t> len(sample, 32);
+    std::vector<uint64_t> mask(sample);
+    for (auto& m : mask) m = rng();
+    DevBuf dx, dout, dlen, dmask;
+    const size_t pbytes = (N + 3) * pitch * 4;
+    MIXED_HIP(hipMalloc(&dx.p, 3 * pbytes), "hipMalloc");
+    MIXED_HIP(hipMalloc(&dout.p, pbytes), "hipMalloc");
+    MIXED_HIP(hipMalloc(&dlen.p, sample * 4), "hipMalloc");
+    MIXED_HIP(hipMalloc(&dmask.p, sample * 8), "hipMalloc");
+    MIXED_HIP(hipMemcpy(dx.p, planes.data(), 3 * pbytes, hipMemcpyHostToDevice), "upload");
+    MIXED_HIP(hipMemcpy(dlen.p, len.data(), sample * 4, hipMemcpyHostToDevice), "upload");
+    MIXED_HIP(hipMemcpy(dmask.p, mask.data(), sample * 8, hipMemcpyHostToDevice), "upload");
+    const char* base = (const char*)dx.p;
+    lf::MixedArgs a{(const uint32_t*)base, (const uint32_t*)(base + pbytes), (const uint32_t*)(base + 2 * pbytes), (uint32_t*)dout.p, pitch,
+                    (const uint32_t*)dlen.p, (const uint64_t*)dmask.p, uint32_t(sample)};
+    hipStream_t s = nullptr;
+    MIXED_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    unsigned best = 256;
+    float best_t = 1e30f;
+    for (unsigned blk : {64u, 128u, 256u, 512u}) {
+        float t_min = 1e30f;
+        for (int rep = 0; rep < 4 && err == hipSuccess; ++rep) { // the first run warms
+            err = hipEventRecord(e0, s);
+            if (err == hipSuccess) err = tables().launch[N](a, blk, s);
+            if (err == hipSuccess) err = hipEventRecord(e1, s);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float t = 0;
+            if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+            if (rep && t < t_min) t_min = t;
+        }
+        if (err != hipSuccess) break;
+        if (t_min < best_t) {
+            best_t = t_min;
+            best = blk;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    if (err != hipSuccess) return fail(LF_RUNTIME_ERROR, std::string("autotune: ") + hipGetErrorString(err));
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    {
+        std::lock_guard<std::mutex> lock(tuned_mutex);
+        tuned[{dev, bits}] = best;
+    }
+    *block = best;
+    if (best_ms) *best_ms = best_t;
+    return LF_OK;
+}
+
+} // extern "C"

@@ -754,6 +754,23 @@ void lfo_exact_dot_add(int bits, const void* c, int subtract, const void* a, ptr
     lfo_store(n, &r, out);
 }
 
+int lfo_mixed_chain(int bits, const void* x, const void* f, const void* g, const uint32_t* len, const uint64_t* mask,
+                    void* out, size_t count)
+{
+    if (!valid_bits(bits)) return -1;
+    const int n = bits / 32;
+    const size_t E = 4 * (size_t)n + 12;
+    for (size_t i = 0; i < count; ++i) {
+        lfo_num v, a, c;
+        lfo_load(n, (const unsigned char*)x + i * E, &v);
+        lfo_load(n, (const unsigned char*)f + i * E, &a);
+        lfo_load(n, (const unsigned char*)g + i * E, &c);
+        for (uint32_t t = 0; t < len[i]; ++t) v = ((mask[i] >> t) & 1u) ? lfo_fma(n, v, a, c) : lfo_mul(n, v, a);
+        lfo_store(n, &v, (unsigned char*)out + i * E);
+    }
+    return 0;
+}
+
 /* --------------------------------------------------------- dense products */
 int lfo_gemm(int bits, int transpose_a, const void* A, const void* B, size_t m, size_t n, size_t k,
              void* C, int subtract)

@@ -92,6 +92,10 @@ int lfo_gemm(int bits, int transpose_a, const void* A, const void* B, size_t m, 
 int lfo_syrk(int bits, const void* A, size_t rows, size_t cols, void* C, int lower_only, int subtract);
 /* Blocked Cholesky / solves in the reference's fixed host order (linalg.mm:411-486, gpu=false).
  * Returns the failing pivot (n on success). passes: 1 forward, 2 backward (L^T), 3 both. */
+/* Mixed chains of one width: value i runs len[i] steps, step t being x <- RN(x*f + g) when bit t of
+ * mask[i] is set and x <- RN(x*f) otherwise (Operation::fma / mul, one rounding each). */
+int lfo_mixed_chain(int bits, const void* x, const void* f, const void* g, const uint32_t* len, const uint64_t* mask,
+                    void* out, size_t count);
 size_t lfo_cholesky(int bits, const void* A, size_t n, void* L, size_t block, int* pivot_sign, uint32_t* pivot_status);
 int lfo_solve(int bits, const void* L, size_t n, const void* B, size_t nrhs, void* X, size_t block, int passes);
 

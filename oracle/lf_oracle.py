@@ -42,6 +42,7 @@ def lib() -> C.CDLL:
         L.lfo_cholesky.argtypes = [i32, vp, sz, vp, sz, C.POINTER(i32), C.POINTER(C.c_uint32)]
         L.lfo_cholesky.restype = sz
         L.lfo_solve.argtypes = [i32, vp, sz, vp, sz, vp, sz, i32]
+        L.lfo_mixed_chain.argtypes = [i32, vp, vp, vp, vp, vp, vp, sz]
         L.lfo_solve.restype = i32
         L.lfo_divide_word.argtypes = [C.c_uint32] * 4
         L.lfo_divide_word.restype = C.c_uint64
@@ -69,6 +70,18 @@ def mul_chain(bits, x, f, steps):
     f = np.ascontiguousarray(f, dtype=np.uint32)
     out = np.zeros_like(x)
     assert lib().lfo_mul_chain(bits, x.ctypes.data, f.ctypes.data, out.ctypes.data, x.shape[0], steps) == 0
+    return out
+
+
+def mixed_chain(bits, x, f, g, length, mask):
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    f = np.ascontiguousarray(f, dtype=np.uint32)
+    g = np.ascontiguousarray(g, dtype=np.uint32)
+    length = np.ascontiguousarray(length, dtype=np.uint32)
+    mask = np.ascontiguousarray(mask, dtype=np.uint64)
+    out = np.zeros_like(x)
+    assert lib().lfo_mixed_chain(bits, x.ctypes.data, f.ctypes.data, g.ctypes.data, length.ctypes.data, mask.ctypes.data,
+                                 out.ctypes.data, x.shape[0]) == 0
     return out
 
 

@@ -8,6 +8,7 @@
 #define __host__
 #define __global__
 #define __forceinline__ inline
+#define __noinline__ __attribute__((noinline))
 static inline uint32_t lf_emul_alignbit(uint32_t hi, uint32_t lo, uint32_t s)
 {
     return (uint32_t)((((uint64_t)hi << 32) | lo) >> (s & 31u));
