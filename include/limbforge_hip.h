@@ -237,8 +237,8 @@ typedef struct lf_chain_bucket {
     const uint64_t* mask;
     uint64_t count;
 } lf_chain_bucket;
-/* One launch per bucket, in order, on `stream`.  block[i]: lanes per workgroup of bucket i (64, 128, 256
- * or 512; NULL or 0 = the autotuned size of that width, else 256). */
+/* One launch per bucket, in order, on `stream`.  block[i]: lanes per workgroup of bucket i (64, 128 or
+ * 256; NULL or 0 = the autotuned size of that width, else 256). */
 int lf_mixed_chains(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, void* stream);
 /* One hipGraph with a kernel node per bucket and no edges between them (launch with lf_graph_launch,
  * free with lf_graph_destroy).  The graph keeps the buckets' device pointers. */

@@ -92,25 +92,36 @@ const FactorTables& tables()
 
 bool valid_bits(int bits) { return bits >= 64 && bits <= 1024 && bits % 32 == 0; }
 
-// The exponent, sign and status words of a rows x cols region of an n-column plane matrix, from element
-// first, into host arrays (row-major, rows x cols).
+// Pinned host copies of the exponent, sign and status planes of whole rows [first_row, first_row + rows)
+// of a matrix with `cols` columns: three contiguous copies, for the band analysis of an update product.
 struct RegionFields {
-    std::vector<uint32_t> e, s, st;
-    int download(const uint32_t* planes, uint64_t pitch, int N, uint64_t first, uint64_t row_stride, std::size_t rows, std::size_t cols,
-                 hipStream_t stream)
+    uint32_t* host = nullptr;
+    std::size_t cap = 0, plane = 0;
+    ~RegionFields()
     {
-        e.assign(rows * cols, 0);
-        s.assign(rows * cols, 0);
-        st.assign(rows * cols, 0);
-        if (!rows || !cols) return LF_OK;
-        std::vector<uint32_t>* dst[3] = {&e, &s, &st};
-        for (int f = 0; f < 3; ++f)
-            FACTOR_HIP(hipMemcpy2DAsync(dst[f]->data(), cols * 4, planes + (uint64_t)(N + f) * pitch + first, row_stride * 4, cols * 4, rows,
-                                        hipMemcpyDeviceToHost, stream),
+        if (host) (void)hipHostFree(host);
+    }
+    int download(const uint32_t* planes, uint64_t pitch, int N, uint64_t first_row, uint64_t rows, uint64_t cols, hipStream_t stream)
+    {
+        plane = rows * cols;
+        if (3 * plane > cap) {
+            if (host) (void)hipHostFree(host);
+            host = nullptr;
+            cap = 0;
+            FACTOR_HIP(hipHostMalloc((void**)&host, std::max<std::size_t>(3 * plane, 1) * 4, hipHostMallocDefault), "hipHostMalloc (fields)");
+            cap = 3 * plane;
+        }
+        for (int f = 0; f < 3 && plane; ++f)
+            FACTOR_HIP(hipMemcpyAsync(host + f * plane, planes + (uint64_t)(N + f) * pitch + first_row * cols, plane * 4, hipMemcpyDeviceToHost,
+                                      stream),
                        "download fields");
         return LF_OK;
     }
-    lf::Fields fields(uint64_t ls, uint64_t ks) const { return {e.data(), s.data(), st.data(), 1, ls, ks}; }
+    // element (line, k) at col0 + line*ls + k*ks of the downloaded rows
+    lf::Fields fields(uint64_t col0, uint64_t ls, uint64_t ks) const
+    {
+        return {host + col0, host + plane + col0, host + 2 * plane + col0, 1, ls, ks};
+    }
 };
 
 // Copies the N+3 planes of `count` elements between operands of possibly different pitch.
@@ -234,7 +245,7 @@ int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, 
         FACTOR_HIP(tables().panel[N](f, s), "panel launch");
         FACTOR_HIP(hipEventRecord(e1, s), "hipEventRecord");
         FACTOR_HIP(hipMemcpyAsync(&hinfo, dinfo.p, sizeof hinfo, hipMemcpyDeviceToHost, s), "download pivot");
-        if (int r = panel.download(W, pitch, N, k1 * n + k0, n, n - k1, w, s)) return r;
+        if (int r = panel.download(W, pitch, N, k1, n - k1, n, s)) return r;
         FACTOR_HIP(hipStreamSynchronize(s), "device execution (panel)");
         float ms = 0;
         FACTOR_HIP(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
@@ -253,8 +264,8 @@ int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, 
         if (k1 < n) {
             t = Clock::now();
             const lf::MatView v{W + (k1 * n + k0), pitch, n, 1};
-            if (int r = update(bits, lo, true, v, panel.fields(w, 1), v, panel.fields(w, 1), n - k1, n - k1, w, W + (k1 * n + k1), pitch, n,
-                               s, ms))
+            if (int r = update(bits, lo, true, v, panel.fields(k0, n, 1), v, panel.fields(k0, n, 1), n - k1, n - k1, w, W + (k1 * n + k1), pitch,
+                               n, s, ms))
                 return r;
             ci.timing_gpu_seconds += ms * 1e-3;
             ci.update_seconds += since(t);
@@ -327,13 +338,13 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
             FACTOR_HIP(block_solve(0, r0, r1), "solve launch");
             if (r1 == n) break;
             // rows [r1, n) of X -= L[r1:, r0:r1] X[r0:r1, :]
-            if (int r = lf_.download(F, fpitch, N, r1 * n + r0, n, n - r1, K, s)) return r;
-            if (int r = rf_.download(Xp, xpitch, N, r0 * nrhs, nrhs, K, nrhs, s)) return r;
+            if (int r = lf_.download(F, fpitch, N, r1, n - r1, n, s)) return r;
+            if (int r = rf_.download(Xp, xpitch, N, r0, K, nrhs, s)) return r;
             FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
             collect();
             const lf::MatView left{F + (r1 * n + r0), fpitch, n, 1}, right{Xp + r0 * nrhs, xpitch, 1, nrhs};
-            if (int r = update(bits, lo, false, left, lf_.fields(K, 1), right, rf_.fields(1, nrhs), n - r1, nrhs, K, Xp + r1 * nrhs, xpitch,
-                               nrhs, s, ms))
+            if (int r = update(bits, lo, false, left, lf_.fields(r0, n, 1), right, rf_.fields(0, 1, nrhs), n - r1, nrhs, K, Xp + r1 * nrhs,
+                               xpitch, nrhs, s, ms))
                 return r;
             gpu += ms * 1e-3;
         }
@@ -347,12 +358,12 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
             FACTOR_HIP(block_solve(1, r0, r1), "solve launch");
             if (!r0) break;
             // rows [0, r0) of X -= L[r0:r1, :r0]^T X[r0:r1, :]
-            if (int r = lf_.download(F, fpitch, N, r0 * n, n, K, r0, s)) return r;
-            if (int r = rf_.download(Xp, xpitch, N, r0 * nrhs, nrhs, K, nrhs, s)) return r;
+            if (int r = lf_.download(F, fpitch, N, r0, K, n, s)) return r;
+            if (int r = rf_.download(Xp, xpitch, N, r0, K, nrhs, s)) return r;
             FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
             collect();
             const lf::MatView left{F + r0 * n, fpitch, 1, n}, right{Xp + r0 * nrhs, xpitch, 1, nrhs};
-            if (int r = update(bits, lo, false, left, lf_.fields(1, r0), right, rf_.fields(1, nrhs), r0, nrhs, K, Xp, xpitch, nrhs, s, ms))
+            if (int r = update(bits, lo, false, left, lf_.fields(0, 1, n), right, rf_.fields(0, 1, nrhs), r0, nrhs, K, Xp, xpitch, nrhs, s, ms))
                 return r;
             gpu += ms * 1e-3;
         }

@@ -371,7 +371,7 @@ template <int N> constexpr int diag_waves()
 // The diagonal block [k0, k1) of a panel, column by column, in one workgroup of diag_waves<N>()
 // wavefronts: the pivot of column j on wave 0 (exact dot, then sqrt on lane 0), then the entries below
 // it in the block, one row per wave at a time.  Stops at a failing pivot.
-template <int N> __global__ void __launch_bounds__(1024) chol_diag_kernel(FactorArgs f)
+template <int N> __global__ void __launch_bounds__(diag_waves<N>() * WAVE) chol_diag_kernel(FactorArgs f)
 {
     constexpr int DW = diag_waves<N>();
     __shared__ WaveLds<N> lds[DW];

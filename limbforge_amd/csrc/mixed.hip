@@ -56,7 +56,7 @@ const MixedTables& tables()
 }
 
 bool valid_bits(int bits) { return bits >= 64 && bits <= 1024 && bits % 32 == 0; }
-bool valid_block(unsigned b) { return b == 64 || b == 128 || b == 256 || b == 512; }
+bool valid_block(unsigned b) { return b == 64 || b == 128 || b == 256; }
 
 int check_bucket(const lf_chain_bucket& b)
 {
@@ -105,7 +105,7 @@ int lf_mixed_chains(const lf_chain_bucket* buckets, int nbuckets, const unsigned
     if (nbuckets < 0 || (nbuckets && !buckets)) return fail(LF_INVALID_ARGUMENT, "bad bucket list");
     for (int i = 0; i < nbuckets; ++i) {
         if (int r = check_bucket(buckets[i])) return r;
-        if (block && block[i] && !valid_block(block[i])) return fail(LF_INVALID_ARGUMENT, "workgroup size must be 64, 128, 256 or 512");
+        if (block && block[i] && !valid_block(block[i])) return fail(LF_INVALID_ARGUMENT, "workgroup size must be 64, 128 or 256");
     }
     for (int i = 0; i < nbuckets; ++i) {
         const lf_chain_bucket& b = buckets[i];
@@ -121,7 +121,7 @@ int lf_mixed_graph_create(const lf_chain_bucket* buckets, int nbuckets, const un
     if (nbuckets < 0 || (nbuckets && !buckets)) return fail(LF_INVALID_ARGUMENT, "bad bucket list");
     for (int i = 0; i < nbuckets; ++i) {
         if (int r = check_bucket(buckets[i])) return r;
-        if (block && block[i] && !valid_block(block[i])) return fail(LF_INVALID_ARGUMENT, "workgroup size must be 64, 128, 256 or 512");
+        if (block && block[i] && !valid_block(block[i])) return fail(LF_INVALID_ARGUMENT, "workgroup size must be 64, 128 or 256");
     }
     hipGraph_t g = nullptr;
     MIXED_HIP(hipGraphCreate(&g, 0), "hipGraphCreate");
@@ -194,7 +194,7 @@ int lf_autotune_mixed(int bits, uint64_t sample, unsigned* block, double* best_m
     if (err == hipSuccess) err = hipEventCreate(&e1);
     unsigned best = 256;
     float best_t = 1e30f;
-    for (unsigned blk : {64u, 128u, 256u, 512u}) {
+    for (unsigned blk : {64u, 128u, 256u}) {
         float t_min = 1e30f;
         for (int rep = 0; rep < 4 && err == hipSuccess; ++rep) { // the first run warms
             err = hipEventRecord(e0, s);

@@ -14,7 +14,7 @@
 
 namespace lf {
 
-template <int N> __global__ void __launch_bounds__(512) mixed_chain_kernel(MixedArgs a)
+template <int N> __global__ void __launch_bounds__(256) mixed_chain_kernel(MixedArgs a)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.count) return;
@@ -35,7 +35,7 @@ template <int N> __global__ void __launch_bounds__(512) mixed_chain_kernel(Mixed
 template <int N> hipError_t launch_mixed_chain(const MixedArgs& g, unsigned block, hipStream_t s)
 {
     if (!g.count) return hipSuccess;
-    if (block != 64 && block != 128 && block != 256 && block != 512) return hipErrorInvalidValue;
+    if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
     hipLaunchKernelGGL((mixed_chain_kernel<N>), dim3((g.count + block - 1) / block), dim3(block), 0, s, g);
     return hipGetLastError();
 }
