@@ -178,19 +178,20 @@ __device__ __noinline__ Num<N> dot_sub_clusters(const Num<N>& c, const VecView& 
 // exact_dot_add with subtract, linalg.hpp:268-318): zero with the OR of the statuses of c and of every
 // x_k, y_k; c itself when no product is nonzero.
 //
-// One wavefront computes one D.  Lane l forms the exact products of terms k = l, l+64, ... and writes
-// each into its LDS row already bit-aligned to the window (2N+1 words), with its word offset and sign;
-// lane w then sums window column w over the rows into a signed 64-bit column sum (no carries between
-// columns); lane 0 resolves the carries, negates a negative window and rounds once.  A window of
-// W = 2N + 24 words holds terms spread over up to ~700 bits; wider spreads take the cluster path on
-// lane 0.  Every lane of the wave calls it with the same arguments; the result is valid in lane 0.
+// One wavefront computes one D.  In chunks of 32 terms, lane l < 32 forms the exact product of term
+// k = chunk + l and writes it into LDS row l as a dense W-word two's-complement window (the product shifted
+// to its offset from the smallest term, negated for a subtracted product); lane w then sums window column
+// w over the rows as an unsigned 64-bit column sum, independent loads with no carries between columns;
+// lane 0 resolves the carries (mod 2^(32W), which holds the signed total), negates a negative window and
+// rounds once.  A window of W = 2N + 24 words holds terms spread over up to ~700 bits; wider spreads take
+// the cluster path on lane 0.  Every lane of the wave calls it with the same arguments; the result is
+// valid in lane 0.
 constexpr int WAVE = 64;
+constexpr int DOT_ROWS = 32; // terms per chunk
 
 template <int N> struct WaveLds {
-    u32 word[WAVE + 1][2 * N + 1]; // row WAVE holds c
-    int32_t off[WAVE + 1];
-    int32_t sgn[WAVE + 1];
-    i64 col[factor_window(N)];
+    u32 row[DOT_ROWS + 1][factor_window(N)]; // row DOT_ROWS holds c
+    u64 col[factor_window(N)];
 };
 
 __device__ __forceinline__ void wave_sync()
@@ -231,17 +232,36 @@ __device__ __forceinline__ int wave_or(int v)
     return v;
 }
 
-// row r of the wave's LDS rows = src (S words) * 2^(rel mod 32), word offset rel / 32, sign sg
+// row r = sg * src (S words) * 2^rel as a W-word two's-complement window (sg in {-1, 0, 1}; rel >= 0 and the
+// term fits: rel + 32S <= 32W - 1)
 template <int N, int S> __device__ __forceinline__ void put_row(WaveLds<N>& L, int r, const u32 (&src)[S], i64 rel, int sg)
 {
+    constexpr int W = factor_window(N);
+    const int o = (int)(rel >> 5);
     const u32 b = (u32)(rel & 31);
+    u32 q[S + 1];
 #pragma unroll
-    for (int t = 0; t <= 2 * N; ++t) {
-        const u32 hi = t < S ? src[t] : 0u, lo = (t > 0 && t - 1 < S) ? src[t - 1] : 0u;
-        L.word[r][t] = b ? ((hi << b) | (lo >> (32u - b))) : hi;
+    for (int t = 0; t <= S; ++t) {
+        const u32 hi = t < S ? src[t] : 0u, lo = t > 0 ? src[t - 1] : 0u;
+        q[t] = sg ? (b ? ((hi << b) | (lo >> (32u - b))) : hi) : 0u;
     }
-    L.off[r] = (int32_t)(rel >> 5);
-    L.sgn[r] = sg;
+    bool nonzero = false;
+    if (sg < 0) { // two's complement of q within S+1 words; words above are all ones when q != 0
+        u32 c = 1;
+#pragma unroll
+        for (int t = 0; t <= S; ++t) {
+            nonzero = nonzero || q[t] != 0;
+            const u64 v = (u64)(u32)~q[t] + c;
+            q[t] = (u32)v;
+            c = (u32)(v >> 32);
+        }
+    }
+    const u32 fill = nonzero ? 0xffffffffu : 0u;
+#pragma unroll
+    for (int w = 0; w < W; ++w) L.row[r][w] = w < o ? 0u : fill;
+#pragma unroll
+    for (int t = 0; t <= S; ++t)
+        if (o + t < W) L.row[r][o + t] = q[t];
 }
 
 template <int N>
@@ -287,39 +307,29 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
         return r;
     }
     // pass 2: rows of exact products, column sums
-    i64 S[CPL];
+    u64 S[CPL];
 #pragma unroll
     for (int t = 0; t < CPL; ++t) S[t] = 0;
-    for (u32 kb = 0; kb < K; kb += WAVE) {
-        const u32 k = kb + lane;
-        int sg = 0;
-        if (k < K) {
+    for (u32 kb = 0; kb < K; kb += DOT_ROWS) {
+        const u32 rows = K - kb < (u32)DOT_ROWS ? K - kb : (u32)DOT_ROWS;
+        if (lane < rows) {
+            const u32 k = kb + lane;
             const Num<N> a = vec_load<N>(x, k), b = vec_load<N>(y, k);
-            sg = -a.s * b.s;
-            if (sg) {
-                u32 p[2 * N];
-                mantissa_product<N>(a.w, b.w, p);
-                put_row<N, 2 * N>(L, (int)lane, p, (i64)a.e + b.e - 2 * (32 * N - 1) - lo, sg);
-            }
+            u32 p[2 * N];
+            mantissa_product<N>(a.w, b.w, p);
+            put_row<N, 2 * N>(L, (int)lane, p, (a.s && b.s) ? (i64)a.e + b.e - 2 * (32 * N - 1) - lo : 0, -a.s * b.s);
         }
-        if (!sg) L.sgn[lane] = 0;
         const bool with_c = kb == 0 && c.s != 0;
-        if (with_c && lane == 0) put_row<N, N>(L, WAVE, c.w, (i64)c.e - (32 * N - 1) - lo, c.s);
+        if (with_c && lane == WAVE - 1) put_row<N, N>(L, DOT_ROWS, c.w, (i64)c.e - (32 * N - 1) - lo, c.s);
         wave_sync();
-        const int rows = (int)(K - kb < (u32)WAVE ? K - kb : (u32)WAVE);
-        for (int r = 0; r <= rows; ++r) {
-            const int row = r == rows ? WAVE : r;
-            if (row == WAVE && !with_c) break;
-            const int g = L.sgn[row];
-            if (!g) continue;
-            const int o = L.off[row];
 #pragma unroll
-            for (int t = 0; t < CPL; ++t) {
-                const int d = (int)lane + WAVE * t - o;
-                if (d >= 0 && d <= 2 * N) {
-                    const i64 v = (i64)L.word[row][d];
-                    S[t] += g > 0 ? v : -v;
-                }
+        for (int t = 0; t < CPL; ++t) {
+            const int w = (int)lane + WAVE * t;
+            if (w < W) {
+                u64 sum = with_c ? (u64)L.row[DOT_ROWS][w] : 0u;
+#pragma unroll 8
+                for (u32 r = 0; r < rows; ++r) sum += L.row[r][w];
+                S[t] += sum;
             }
         }
         wave_sync();
@@ -331,10 +341,10 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
     Num<N> r = zero<N>(ST_OK);
     if (lane == 0) {
         u32 acc[W];
-        i64 cy = 0;
+        u64 cy = 0;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const i64 v = L.col[w] + cy;
+            const u64 v = L.col[w] + cy;
             acc[w] = (u32)v;
             cy = v >> 32;
         }

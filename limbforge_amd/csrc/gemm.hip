@@ -243,6 +243,8 @@ Side analyze(const Fields& f, std::size_t lines, std::size_t K, const lf_linalg_
     s.lines.assign(lines, GemmLine{});
     s.bands.assign(lines, {});
     const long G = o.band_bits;
+    // a thread per 2^16 entries at most: thread start-up costs more than a small analysis
+    threads = unsigned(std::min<std::size_t>(threads, std::max<std::size_t>(1, lines * std::max<std::size_t>(K, 1) >> 16)));
     parallel_for(threads, lines, [&](std::size_t first, std::size_t last) {
         for (std::size_t i = first; i < last; ++i) {
             GemmLine& L = s.lines[i];
@@ -351,6 +353,46 @@ struct DeviceBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         return hipMalloc(&p, std::max<std::size_t>(bytes, 16));
+    }
+};
+
+// Device work buffers of the planner, kept per host thread and device and grown on demand, so that a
+// sequence of products (the updates of a factorization) does not allocate and free on every call.  A
+// product synchronizes its stream before returning, so the next call on the thread may reuse them.
+struct Workspace {
+    std::vector<std::pair<void*, std::size_t>> slots;
+    ~Workspace()
+    {
+        for (auto& s : slots)
+            if (s.first) (void)hipFree(s.first);
+    }
+};
+struct PoolBuf {
+    int slot;
+    void* p = nullptr;
+    hipError_t alloc(std::size_t bytes)
+    {
+        thread_local std::vector<std::pair<int, Workspace>> pools; // (device, buffers)
+        int dev = 0;
+        if (hipError_t e = hipGetDevice(&dev)) return e;
+        Workspace* ws = nullptr;
+        for (auto& d : pools)
+            if (d.first == dev) ws = &d.second;
+        if (!ws) {
+            pools.emplace_back(dev, Workspace{});
+            ws = &pools.back().second;
+        }
+        if (ws->slots.size() <= std::size_t(slot)) ws->slots.resize(slot + 1, {nullptr, 0});
+        auto& s = ws->slots[slot];
+        bytes = std::max<std::size_t>(bytes, 16);
+        if (s.second < bytes) {
+            if (s.first) (void)hipFree(s.first);
+            s = {nullptr, 0};
+            if (hipError_t e = hipMalloc(&s.first, bytes)) return e;
+            s.second = bytes;
+        }
+        p = s.first;
+        return hipSuccess;
     }
 };
 
@@ -473,7 +515,7 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     }
     t0 = Clock::now();
     // line tables
-    DeviceBuf dl, dr;
+    PoolBuf dl{0}, dr{1};
     GEMM_HIP(dl.alloc(sizeof(GemmLine) * std::max<std::size_t>(P.m, 1)), "hipMalloc (lines)");
     GEMM_HIP(hipMemcpyAsync(dl.p, ls.lines.data(), sizeof(GemmLine) * P.m, hipMemcpyHostToDevice, P.stream), "upload lines");
     const GemmLine* dright = (const GemmLine*)dl.p;
@@ -506,7 +548,7 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     // residue path
     const std::size_t Kp = std::max<std::size_t>((std::max<std::size_t>(P.K, 1) + 63) / 64 * 64, 64);
     const std::size_t Mp = (Me + 63) / 64 * 64, Np = (Ne + 63) / 64 * 64;
-    DeviceBuf dconst, dig_l1, dig_l0, dig_r1, dig_r0, dres, dlist[3], derr;
+    PoolBuf dconst{2}, dig_l1{3}, dig_l0{4}, dig_r1{5}, dig_r0{6}, dres{7}, dlist[3] = {{8}, {9}, {10}}, derr{11};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     GEMM_HIP(hipEventCreate(&e0), "hipEventCreate");
     GEMM_HIP(hipEventCreate(&e1), "hipEventCreate");
@@ -655,7 +697,22 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
         r.syrk = P.syrk;
         r.lower_only = P.lower_only;
         r.subtract = P.subtract;
-        GEMM_HIP(gemm_tables().rebuild[N](wx_class, r, P.stream), "rebuild launch");
+        // band pairs per output and the stage of the pair CRTs, in row chunks of at most 1 GiB
+        uint32_t pi = 0, pj = 0;
+        for (auto& Ln : ls.lines) pi = std::max<uint32_t>(pi, Ln.cls == 0 ? uint32_t(Ln.nb) : 0u);
+        for (auto& Ln : rs.lines) pj = std::max<uint32_t>(pj, Ln.cls == 0 ? uint32_t(Ln.nb) : 0u);
+        r.pj = std::max<uint32_t>(pj, 1);
+        r.pairs = any_pair ? pi * pj : 0;
+        const uint64_t per_row = uint64_t(r.pairs) * P.n * (WX + 1) * 4;
+        const uint64_t chunk = per_row ? std::max<uint64_t>(1, (uint64_t(1) << 30) / per_row) : P.m;
+        PoolBuf dstage{12};
+        if (r.pairs) GEMM_HIP(dstage.alloc(per_row * std::min<uint64_t>(chunk, P.m)), "hipMalloc (CRT stage)");
+        r.stage = (uint32_t*)dstage.p;
+        for (uint64_t row0 = 0; row0 < P.m; row0 += chunk) {
+            r.row0 = uint32_t(row0);
+            r.rows = uint32_t(std::min<uint64_t>(chunk, P.m - row0));
+            GEMM_HIP(gemm_tables().rebuild[N](wx_class, r, P.stream), "rebuild launch");
+        }
     }
     for (int c = 0; c < 3; ++c) {
         if (lists[c].empty()) continue;

@@ -76,6 +76,10 @@ struct RebuildArgs {
     uint32_t* out;            // output planes (element i*out_stride + j)
     uint64_t out_pitch, out_stride;
     uint32_t syrk, lower_only, subtract;
+    // two-stage rebuild over output rows [row0, row0 + rows): the CRT of band pair p = b * pj + c of output
+    // (i, j) goes to stage word w at (w * pairs + p) * (rows * n) + (i - row0) * n + j (w = WX: the sign)
+    uint32_t* stage;
+    uint32_t row0, rows, pairs, pj;
 };
 
 // Exact dot products of fallback outputs (and the plain limb GEMM): one lane per listed output.

@@ -90,13 +90,22 @@ __device__ __forceinline__ int crt_rebuild(const RebuildArgs& g, u32 row, u32 co
 #pragma unroll
     for (int w = 0; w < WX; ++w) colsum[w] = 0;
     const u64 plane = (u64)g.Mp * g.Np, at = (u64)row * g.Np + col;
-    for (u32 q = 0; q < g.L; ++q) {
-        const u32 m = g.moduli[q], cm = g.recip[q];
-        const u32 r = g.res[q * plane + at];
-        const u32 y = mulmod16(r, g.cinv[q], m, cm);
-        const u32* C = g.crt + q * WX;
+    // residues in batches of 8 moduli, loaded before they are used so that the loads overlap
+    constexpr u32 RB = 8;
+    for (u32 q0 = 0; q0 < g.L; q0 += RB) {
+        u32 rr[RB];
 #pragma unroll
-        for (int w = 0; w < WX; ++w) colsum[w] += (u64)y * C[w];
+        for (u32 u = 0; u < RB; ++u) rr[u] = q0 + u < g.L ? (u32)g.res[(u64)(q0 + u) * plane + at] : 0u;
+#pragma unroll
+        for (u32 u = 0; u < RB; ++u) {
+            const u32 q = q0 + u;
+            if (q >= g.L) break;
+            const u32 m = g.moduli[q], cm = g.recip[q];
+            const u32 y = mulmod16(rr[u], g.cinv[q], m, cm);
+            const u32* C = g.crt + q * WX;
+#pragma unroll
+            for (int w = 0; w < WX; ++w) colsum[w] += (u64)y * C[w];
+        }
     }
     u64 carry = 0;
 #pragma unroll
@@ -178,13 +187,56 @@ template <int N> __device__ __forceinline__ Num<N> load_out(const u32* base, u64
     return r;
 }
 
-// WX: CRT integer words (M' normalised to bit 32(WX-1)-1); ACC: multi-band accumulator words.
-template <int N, int WX, int ACC> __global__ void __launch_bounds__(256) rebuild_kernel(RebuildArgs g)
+// Stage 1 of the rebuild: one lane per (band pair, output) of rows [row0, row0 + rows) reconstructs the
+// pair's exact integer by CRT, reduced to the symmetric range, into the stage (WX words and a sign word).
+// Only this stage touches the residues; the register budget holds the column sums and nothing else.
+template <int WX> __global__ void __launch_bounds__(256) crt_pairs_kernel(RebuildArgs g)
 {
-    const u32 nblocks = (g.n + 255u) / 256u;
-    const u32 i = blockIdx.x / nblocks;
-    const u32 j = (blockIdx.x - i * nblocks) * 256u + threadIdx.x;
-    if (i >= g.m || j >= g.n) return;
+    const u64 span = (u64)g.rows * g.n;
+    const u64 idx = (u64)blockIdx.x * 256u + threadIdx.x;
+    if (idx >= span) return;
+    const u32 i = g.row0 + (u32)(idx / g.n), j = (u32)(idx % g.n);
+    if (g.syrk && j > i) return;
+    const GemmLine* Li = g.left + i;
+    const GemmLine* Lj = g.right + j;
+    if (Li->cls != 0 || Lj->cls != 0) return;
+    const u32 p = blockIdx.y, b = p / g.pj, c = p % g.pj;
+    if ((int)b >= Li->nb || (int)c >= Lj->nb) return;
+    u32 rr = (u32)Li->member[b], rc = (u32)Lj->member[c];
+    if (g.syrk && rc > rr) { // the extended SYRK product is symmetric: read its lower triangle
+        const u32 t0 = rr;
+        rr = rc;
+        rc = t0;
+    }
+    u32 X[WX];
+    const int sg = crt_rebuild<WX>(g, rr, rc, X);
+    u32* dst = g.stage + (u64)p * span + idx;
+    const u64 wstride = (u64)g.pairs * span;
+#pragma unroll
+    for (int w = 0; w < WX; ++w) dst[w * wstride] = X[w];
+    dst[WX * wstride] = (u32)sg;
+}
+
+template <int WX> __device__ __forceinline__ int read_stage(const RebuildArgs& g, u32 b, u32 c, u32 i, u32 j, u32 (&X)[WX])
+{
+    const u64 span = (u64)g.rows * g.n;
+    const u32* src = g.stage + (u64)(b * g.pj + c) * span + (u64)(i - g.row0) * g.n + j;
+    const u64 wstride = (u64)g.pairs * span;
+#pragma unroll
+    for (int w = 0; w < WX; ++w) X[w] = src[w * wstride];
+    return (int)src[WX * wstride];
+}
+
+// WX: CRT integer words (M' normalised to bit 32(WX-1)-1); ACC: multi-band accumulator words (kept in LDS,
+// one row of ACC+1 words per lane; 1 = no multi-band outputs in this instantiation).
+constexpr u32 REBUILD_BLOCK = 128;
+template <int N, int WX, int ACC> __global__ void __launch_bounds__(REBUILD_BLOCK) rebuild_kernel(RebuildArgs g)
+{
+    __shared__ u32 rebuild_acc[ACC > 1 ? REBUILD_BLOCK * (ACC + 1) : 1];
+    const u32 nblocks = (g.n + REBUILD_BLOCK - 1) / REBUILD_BLOCK;
+    const u32 i = g.row0 + blockIdx.x / nblocks;
+    const u32 j = (blockIdx.x % nblocks) * REBUILD_BLOCK + threadIdx.x;
+    if (i >= g.row0 + g.rows || i >= g.m || j >= g.n) return;
     if (g.syrk && j > i) return; // SYRK computes the lower triangle (mirrored when full)
     const bool mirror = g.syrk && !g.lower_only;
     const GemmLine* Li = g.left + i;
@@ -209,49 +261,72 @@ template <int N, int WX, int ACC> __global__ void __launch_bounds__(256) rebuild
     Num<N> result;
     if (nbi == 1 && nbj == 1) {
         u32 X[WX];
-        u32 r0 = (u32)Li->member[0], c0 = (u32)Lj->member[0];
-        if (g.syrk && c0 > r0) { // the extended SYRK product is symmetric: read its lower triangle
-            const u32 t0 = r0;
-            r0 = c0;
-            c0 = t0;
-        }
-        const int sg = crt_rebuild<WX>(g, r0, c0, X);
+        const int sg = read_stage<WX>(g, 0, 0, i, j, X);
         if (!g.subtract) result = pack<N, WX>(X, base - g.t, sg, ST_OK);
         else result = round_sum2<N, WX, N>(X, base - g.t, -sg, old.w, old_scale, old.s);
     } else if constexpr (ACC <= WX) {
         result = zero<N>(ST_INVALID); // not reached: the planner sends multi-band lines of this class elsewhere
     } else {
+        // exact sum over the band pairs in this lane's LDS accumulator (two's complement, ACC words): each
+        // pair's integer is added at its word offset with a runtime address, so no register barrel shifter
         const int tb = (g.t + 31) & ~31;
-        u32 acc[ACC];
-#pragma unroll
+        u32* acc = rebuild_acc + threadIdx.x * (ACC + 1);
         for (int w = 0; w < ACC; ++w) acc[w] = 0;
         for (int b = 0; b < nbi; ++b)
             for (int c = 0; c < nbj; ++c) {
-                u32 X[WX], P[ACC];
-                u32 rr = (u32)Li->member[b], rc = (u32)Lj->member[c];
-                if (g.syrk && rc > rr) {
-                    const u32 t0 = rr;
-                    rr = rc;
-                    rc = t0;
-                }
-                const int sg = crt_rebuild<WX>(g, rr, rc, X);
+                u32 X[WX];
+                const int sg = read_stage<WX>(g, (u32)b, (u32)c, i, j, X);
                 const i64 shift = (i64)(Li->lo[b] - Li->low) + (Lj->lo[c] - Lj->low) + tb - g.t;
-                place<ACC, WX>(P, X, shift);
-                accumulate_signed<ACC>(acc, P, sg);
+                const int o = (int)(shift >> 5);
+                const u32 bs = (u32)(shift & 31);
+                if (sg > 0) {
+                    u32 cy = 0;
+#pragma unroll
+                    for (int t = 0; t <= WX; ++t) {
+                        const u32 hi = t < WX ? X[t] : 0u, lo = t ? X[t - 1] : 0u;
+                        const u32 v = bs ? __builtin_amdgcn_alignbit(hi, lo, 32u - bs) : hi;
+                        const u64 z = (u64)acc[o + t] + v + cy;
+                        acc[o + t] = (u32)z;
+                        cy = (u32)(z >> 32);
+                    }
+                    for (int w = o + WX + 1; cy && w < ACC; ++w) {
+                        const u64 z = (u64)acc[w] + cy;
+                        acc[w] = (u32)z;
+                        cy = (u32)(z >> 32);
+                    }
+                } else {
+                    u32 bw = 0;
+#pragma unroll
+                    for (int t = 0; t <= WX; ++t) {
+                        const u32 hi = t < WX ? X[t] : 0u, lo = t ? X[t - 1] : 0u;
+                        const u32 v = bs ? __builtin_amdgcn_alignbit(hi, lo, 32u - bs) : hi;
+                        const u64 z = (u64)acc[o + t] - v - bw;
+                        acc[o + t] = (u32)z;
+                        bw = (u32)(z >> 63);
+                    }
+                    for (int w = o + WX + 1; bw && w < ACC; ++w) {
+                        const u64 z = (u64)acc[w] - bw;
+                        acc[w] = (u32)z;
+                        bw = (u32)(z >> 63);
+                    }
+                }
             }
+        u32 A[ACC];
+#pragma unroll
+        for (int w = 0; w < ACC; ++w) A[w] = acc[w];
         int sign = 1;
-        if (acc[ACC - 1] >> 31) {
+        if (A[ACC - 1] >> 31) {
             sign = -1;
             u32 cy = 1;
 #pragma unroll
             for (int w = 0; w < ACC; ++w) {
-                const u64 v = (u64)(u32)~acc[w] + cy;
-                acc[w] = (u32)v;
+                const u64 v = (u64)(u32)~A[w] + cy;
+                A[w] = (u32)v;
                 cy = (u32)(v >> 32);
             }
         }
-        if (!g.subtract) result = pack<N, ACC>(acc, base - tb, sign, ST_OK);
-        else result = round_sum2<N, ACC, N>(acc, base - tb, -sign, old.w, old_scale, old.s);
+        if (!g.subtract) result = pack<N, ACC>(A, base - tb, sign, ST_OK);
+        else result = round_sum2<N, ACC, N>(A, base - tb, -sign, old.w, old_scale, old.s);
     }
     store_out<N>(g.out, g.out_pitch, g.out_stride, i, j, mirror, result);
 }
