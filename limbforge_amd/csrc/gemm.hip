@@ -29,9 +29,10 @@ namespace lf {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int PT = 64;        // output tile per workgroup (rows and columns)
+constexpr int PT = 128;       // output tile per workgroup (rows and columns)
 constexpr int PK = 32;        // K per MFMA step
 constexpr int LROW = PK + 16; // LDS row stride in bytes (padding against bank conflicts)
+constexpr u32 XCDS = 8;       // MI355X: 8 XCDs, workgroups dealt to them round-robin by linear id
 
 __device__ __forceinline__ u32 smod(int g, u32 m, u32 c, u32 p31)
 {
@@ -40,61 +41,112 @@ __device__ __forceinline__ u32 smod(int g, u32 m, u32 c, u32 p31)
     return r >= p31 ? r - p31 : r + m - p31;
 }
 
-// One 64x64 output tile of one modulus: four waves, each a 32x32 sub-tile with three int32 MFMA
-// accumulators (G11, G00, Gmid).  Each 32-deep K step stages both digit planes of the A rows and the B
-// columns in LDS (every thread moves 16 bytes of each), then issues four v_mfma_i32_32x32x32_i8.
-__global__ void __launch_bounds__(256) product_kernel(ProductArgs g)
+// One 32x32 block of residues: (65536 G11 + 256 Gmid + G00) mod m, written as uint16.
+__device__ __forceinline__ void store_block(uint16_t* out, u32 Np, u32 row, u32 col, u32 lane, const v16i& c11, const v16i& c00,
+                                            const v16i& cmid, u32 m, u32 cm, u32 p31, u32 p16)
 {
-    const u32 q = blockIdx.z;
-    const u32 row0 = blockIdx.y * PT, col0 = blockIdx.x * PT;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        // 32x32 accumulator layout: lane l, element i -> row 8(i/4) + 4(l/32) + i%4, column l%32
+        const u32 rr = row + 8u * (u32)(i / 4) + 4u * (lane >> 5) + (u32)(i % 4);
+        const u32 cc = col + (lane & 31u);
+        const u32 r11 = smod(c11[i], m, cm, p31), r00 = smod(c00[i], m, cm, p31), rm = smod(cmid[i], m, cm, p31);
+        u32 v = reduce32(r11 * p16, m, cm) + reduce32(rm << 8, m, cm);
+        v = (v >= m ? v - m : v) + r00;
+        out[(u64)rr * Np + cc] = (uint16_t)(v >= m ? v - m : v);
+    }
+}
+
+// One 128x128 output tile of one modulus: four waves, each a 64x64 sub-tile of 2x2 MFMA blocks with three
+// int32 accumulators each (G11, G00, Gmid).  Per 32-deep K step the workgroup stages both digit planes of
+// its 128 A rows and 128 B columns in LDS (double-buffered: one barrier per step) while the next step's
+// tiles are loaded into registers; each wave then reads 8 fragments and issues 16 v_mfma_i32_32x32x32_i8
+// (a 64x64 wave tile reads half the LDS bytes per MFMA of a 32x32 one).  The 1-D grid is dealt to the XCDs
+// so that all tiles of a modulus run on one XCD (modulus q on XCD q mod 8): a modulus's digit planes are
+// fetched into one L2 instead of eight.
+__global__ void __launch_bounds__(256, 2) product_kernel(ProductArgs g)
+{
+    const u32 tiles_x = g.Np / PT, tiles = tiles_x * (g.Mp / PT);
+    const u32 xcd = blockIdx.x % XCDS, slot = blockIdx.x / XCDS;
+    const u32 q = xcd + XCDS * (slot / tiles), tile = slot % tiles;
+    if (q >= g.L) return;
+    const u32 row0 = (tile / tiles_x) * PT, col0 = (tile % tiles_x) * PT;
     if (g.lower && col0 > row0 + PT - 1) return; // tile strictly above the diagonal
-    __shared__ __attribute__((aligned(16))) int8_t sa[2][PT * LROW];
-    __shared__ __attribute__((aligned(16))) int8_t sb[2][PT * LROW];
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][2][PT * LROW]; // [buffer][plane]
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][2][PT * LROW];
     const u64 aplane = (u64)g.Mp * g.Kp, bplane = (u64)g.Np * g.Kp;
     const u32 t = threadIdx.x;
     const u32 lane = t & 63u, wave = t >> 6;
-    const u32 wr = (wave >> 1) * 32u, wc = (wave & 1u) * 32u;
-    const u32 lp = t >> 7, lr = (t & 127u) >> 1, lk = (t & 1u) * 16u;
-    const int8_t* A = (lp ? g.a0 : g.a1) + q * aplane + (u64)(row0 + lr) * g.Kp + lk;
-    const int8_t* B = (lp ? g.b0 : g.b1) + q * bplane + (u64)(col0 + lr) * g.Kp + lk;
-    v16i c11 = {}, c00 = {}, cmid = {};
+    const u32 wr = (wave >> 1) * 64u, wc = (wave & 1u) * 64u;
+    // staging: thread t moves the 32 bytes of k of row (t & 127) of plane (t >> 7), for A and for B
+    const u32 lp = t >> 7, lr = t & 127u;
+    const int8_t* A = (lp ? g.a0 : g.a1) + q * aplane + (u64)(row0 + lr) * g.Kp;
+    const int8_t* B = (lp ? g.b0 : g.b1) + q * bplane + (u64)(col0 + lr) * g.Kp;
+    v16i c11[2][2] = {}, c00[2][2] = {}, cmid[2][2] = {};
     const u32 r = lane & 31u, kg = (lane >> 5) * 16u;
-    for (u32 k0 = 0; k0 < g.Kp; k0 += PK) {
-        const v4i va = *(const v4i*)(A + k0);
-        const v4i vb = *(const v4i*)(B + k0);
+    v4i pa0 = *(const v4i*)(A), pa1 = *(const v4i*)(A + 16), pb0 = *(const v4i*)(B), pb1 = *(const v4i*)(B + 16);
+    *(v4i*)(&sa[0][lp][lr * LROW]) = pa0;
+    *(v4i*)(&sa[0][lp][lr * LROW + 16]) = pa1;
+    *(v4i*)(&sb[0][lp][lr * LROW]) = pb0;
+    *(v4i*)(&sb[0][lp][lr * LROW + 16]) = pb1;
+    __syncthreads();
+    const u32 steps = g.Kp / PK;
+    for (u32 s = 0; s < steps; ++s) {
+        const u32 cur = s & 1u;
+        const bool more = s + 1 < steps;
+        if (more) { // next step's tiles, in flight while this step computes
+            const u32 k1 = (s + 1) * PK;
+            pa0 = *(const v4i*)(A + k1);
+            pa1 = *(const v4i*)(A + k1 + 16);
+            pb0 = *(const v4i*)(B + k1);
+            pb1 = *(const v4i*)(B + k1 + 16);
+        }
+        v4i a1[2], a0[2], b1[2], b0[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            a1[h] = *(const v4i*)(&sa[cur][0][(wr + 32u * h + r) * LROW + kg]);
+            a0[h] = *(const v4i*)(&sa[cur][1][(wr + 32u * h + r) * LROW + kg]);
+            b1[h] = *(const v4i*)(&sb[cur][0][(wc + 32u * h + r) * LROW + kg]);
+            b0[h] = *(const v4i*)(&sb[cur][1][(wc + 32u * h + r) * LROW + kg]);
+        }
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) {
+                c11[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1[mi], b1[ni], c11[mi][ni], 0, 0, 0);
+                c00[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0[mi], b0[ni], c00[mi][ni], 0, 0, 0);
+                cmid[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1[mi], b0[ni], cmid[mi][ni], 0, 0, 0);
+                cmid[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0[mi], b1[ni], cmid[mi][ni], 0, 0, 0);
+            }
+        if (more) {
+            const u32 nxt = cur ^ 1u;
+            *(v4i*)(&sa[nxt][lp][lr * LROW]) = pa0;
+            *(v4i*)(&sa[nxt][lp][lr * LROW + 16]) = pa1;
+            *(v4i*)(&sb[nxt][lp][lr * LROW]) = pb0;
+            *(v4i*)(&sb[nxt][lp][lr * LROW + 16]) = pb1;
+        }
         __syncthreads();
-        *(v4i*)(&sa[lp][lr * LROW + lk]) = va;
-        *(v4i*)(&sb[lp][lr * LROW + lk]) = vb;
-        __syncthreads();
-        const v4i a1 = *(const v4i*)(&sa[0][(wr + r) * LROW + kg]);
-        const v4i a0 = *(const v4i*)(&sa[1][(wr + r) * LROW + kg]);
-        const v4i b1 = *(const v4i*)(&sb[0][(wc + r) * LROW + kg]);
-        const v4i b0 = *(const v4i*)(&sb[1][(wc + r) * LROW + kg]);
-        c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
-        c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
-        cmid = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, cmid, 0, 0, 0);
-        cmid = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, cmid, 0, 0, 0);
     }
     const u32 m = g.moduli[q], cm = g.recip[q];
     const u32 p31 = reduce32(0x80000000u, m, cm);
     const u32 p16 = reduce32(65536u, m, cm);
     uint16_t* out = g.res + (u64)q * g.Mp * g.Np;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        // 32x32 accumulator layout: lane l, element i -> row 8(i/4) + 4(l/32) + i%4, column l%32
-        const u32 rr = row0 + wr + 8u * (u32)(i / 4) + 4u * (lane >> 5) + (u32)(i % 4);
-        const u32 cc = col0 + wc + (lane & 31u);
-        const u32 r11 = smod(c11[i], m, cm, p31), r00 = smod(c00[i], m, cm, p31), rm = smod(cmid[i], m, cm, p31);
-        u32 v = reduce32(r11 * p16, m, cm) + reduce32(rm << 8, m, cm);
-        v = (v >= m ? v - m : v) + r00;
-        out[(u64)rr * g.Np + cc] = (uint16_t)(v >= m ? v - m : v);
-    }
+    // one block at a time (the scheduling barriers keep the accumulators in AGPRs until their turn)
+    store_block(out, g.Np, row0 + wr, col0 + wc, lane, c11[0][0], c00[0][0], cmid[0][0], m, cm, p31, p16);
+    __builtin_amdgcn_sched_barrier(0);
+    store_block(out, g.Np, row0 + wr, col0 + wc + 32u, lane, c11[0][1], c00[0][1], cmid[0][1], m, cm, p31, p16);
+    __builtin_amdgcn_sched_barrier(0);
+    store_block(out, g.Np, row0 + wr + 32u, col0 + wc, lane, c11[1][0], c00[1][0], cmid[1][0], m, cm, p31, p16);
+    __builtin_amdgcn_sched_barrier(0);
+    store_block(out, g.Np, row0 + wr + 32u, col0 + wc + 32u, lane, c11[1][1], c00[1][1], cmid[1][1], m, cm, p31, p16);
 }
 
 hipError_t launch_product(const ProductArgs& g, hipStream_t s)
 {
     if (!g.L || !g.Mp || !g.Np) return hipSuccess;
-    hipLaunchKernelGGL(product_kernel, dim3(g.Np / PT, g.Mp / PT, g.L), dim3(256), 0, s, g);
+    const u32 tiles = (g.Np / PT) * (g.Mp / PT);
+    const u32 per_xcd = (g.L + XCDS - 1) / XCDS; // moduli per XCD (the last ones may idle)
+    hipLaunchKernelGGL(product_kernel, dim3(XCDS * tiles * per_xcd), dim3(256), 0, s, g);
     return hipGetLastError();
 }
 
@@ -547,7 +599,7 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     const uint64_t fallback_outputs = (lists[0].size() + lists[1].size() + lists[2].size()) / 2;
     // residue path
     const std::size_t Kp = std::max<std::size_t>((std::max<std::size_t>(P.K, 1) + 63) / 64 * 64, 64);
-    const std::size_t Mp = (Me + 63) / 64 * 64, Np = (Ne + 63) / 64 * 64;
+    const std::size_t Mp = (Me + PT - 1) / PT * PT, Np = (Ne + PT - 1) / PT * PT; // product tiles
     PoolBuf dconst{2}, dig_l1{3}, dig_l0{4}, dig_r1{5}, dig_r0{6}, dres{7}, dlist[3] = {{8}, {9}, {10}}, derr{11};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     GEMM_HIP(hipEventCreate(&e0), "hipEventCreate");
