@@ -34,6 +34,8 @@ BITS, COUNT, CHAIN = 1024, 65536, 16
 # Reference (Apple M5 Max) device time for this exact workload: 3.192 ms per chain batch
 # (benchmarks/results/final.csv:145) -> 65,536 / 3.192 ms.
 REFERENCE_VALUES_PER_S = COUNT / 3.192e-3
+# Measured HBM traffic of one mul_chain_kernel<32> launch (counter passes described at the roofline below).
+TRAFFIC_BYTES = round((9127.7 * 262144 / 131084 + 8960.0) * 1024)
 
 
 def log(*a):
@@ -515,6 +517,9 @@ def main(argv=None) -> int:
     achieved = products / (kernel_ms * 1e-3) / 1e12
     rate = C.c_double(0)
     capi.check(capi.lib().lf_calibrate_mad64(256 * 64, 4096, C.byref(rate)))
+    # a 256 MiB device copy: the reference kernel that scales the HBM traffic counters
+    copy_rate = C.c_double(0)
+    capi.check(capi.lib().lf_calibrate_copy(256 << 20, C.byref(copy_rate)))
     peak = rate.value / 1e12
     roofline = {
         "bound": "valu",
@@ -522,11 +527,16 @@ def main(argv=None) -> int:
         "peak": round(peak, 4),
         "unit": "Tword-products/s",
         "frac": round(achieved / peak, 4),
-        "traffic": None,
+        # HBM bytes per launch from the FETCH_SIZE and WRITE_SIZE counter passes of this command
+        # (rocprofv3 --pmc, one counter per run), each scaled by the same counter's reading for the 256 MiB
+        # reference copy lf_calibrate_copy: FETCH 9127.7 KiB x (262144 / 131084) + WRITE 8960 KiB x 1.0
+        "traffic": TRAFFIC_BYTES,
         "kernel": "mul_chain_kernel<32>",
         "kernel_ms": round(kernel_ms, 5),
         "peak_source": "measured: lf_calibrate_mad64 (8 independent v_mad_u64_u32 chains per lane, 16384 workgroups)",
         "algorithmic_bytes": COUNT * 3 * (4 * 32 + 12),
+        "traffic_over_algorithmic": round(TRAFFIC_BYTES / (COUNT * 3 * (4 * 32 + 12)), 3),
+        "copy_probe_gbps": round(copy_rate.value / 1e9, 1),
     }
 
     parity = None

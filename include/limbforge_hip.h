@@ -251,6 +251,10 @@ int lf_autotune_mixed(int bits, uint64_t sample, unsigned* block, double* best_m
 /* Sustained v_mad_u64_u32 rate of the device (32x32->64 word products per second), measured with
  * `blocks` workgroups of 256 lanes, each lane running `iters` rounds of eight independent chains. */
 int lf_calibrate_mad64(uint64_t blocks, int iters, double* products_per_second);
+/* One copy of `bytes` (a multiple of 16) between two fresh device buffers; the rate counts bytes read plus
+ * bytes written.  Profilers' FETCH_SIZE / WRITE_SIZE of this kernel against its known size scale those
+ * counters for the traffic of the other kernels. */
+int lf_calibrate_copy(uint64_t bytes, double* bytes_per_second);
 
 #ifdef __cplusplus
 }

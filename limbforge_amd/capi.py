@@ -83,6 +83,7 @@ def lib() -> C.CDLL:
         "lf_elementwise": (i32, [i32, i32, P(Operand), P(Operand), P(Operand), P(Operand), u64, P(Broadcast), P(Broadcast), vp]),
         "lf_mul_chain": (i32, [i32, P(Operand), P(Operand), P(Operand), u64, P(Broadcast), i32, vp]),
         "lf_calibrate_mad64": (i32, [u64, i32, P(C.c_double)]),
+        "lf_calibrate_copy": (i32, [u64, P(C.c_double)]),
         "lf_cast": (i32, [i32, i32, i32, P(Operand), P(Operand), u64, vp]),
         "lf_cast_host": (i32, [i32, i32, i32, vp, vp, u64, P(C.c_double), P(C.c_double)]),
         "lf_run_host": (i32, [i32, i32, vp, vp, vp, vp, u64, P(Broadcast), P(Broadcast), P(C.c_double), P(C.c_double)]),

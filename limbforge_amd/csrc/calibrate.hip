@@ -49,3 +49,43 @@ extern "C" int lf_calibrate_mad64(uint64_t blocks, int iters, double* products_p
     *products_per_second = (double)blocks * 256.0 * 8.0 * (double)iters / (ms * 1e-3);
     return LF_OK;
 }
+
+// Reference copy for HBM traffic counters: `bytes` (a multiple of 16) copied once from one fresh buffer to
+// another, 16 bytes per lane per iteration of a grid-stride loop.  A profiler's FETCH_SIZE / WRITE_SIZE for
+// this kernel, against the known size, gives the scale of those counters.
+namespace {
+__global__ void __launch_bounds__(256) copy_probe(const uint4* src, uint4* dst, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) dst[i] = src[i];
+}
+} // namespace
+
+extern "C" int lf_calibrate_copy(uint64_t bytes, double* bytes_per_second)
+{
+    if (!bytes || bytes % 16 || !bytes_per_second) return LF_INVALID_ARGUMENT;
+    void *src = nullptr, *dst = nullptr;
+    if (hipMalloc(&src, bytes) != hipSuccess) return LF_RUNTIME_ERROR;
+    if (hipMalloc(&dst, bytes) != hipSuccess) {
+        (void)hipFree(src);
+        return LF_RUNTIME_ERROR;
+    }
+    (void)hipMemset(src, 1, bytes);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const uint64_t n = bytes / 16;
+    const unsigned blocks = 256u * 8u;
+    (void)hipEventRecord(e0, nullptr);
+    hipLaunchKernelGGL(copy_probe, dim3(blocks), dim3(256), 0, nullptr, (const uint4*)src, (uint4*)dst, n);
+    (void)hipEventRecord(e1, nullptr);
+    hipError_t err = hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (err != hipSuccess) return LF_RUNTIME_ERROR;
+    *bytes_per_second = 2.0 * (double)bytes / (ms * 1e-3);
+    return LF_OK;
+}
