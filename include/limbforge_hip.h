@@ -240,8 +240,9 @@ typedef struct lf_chain_bucket {
 /* One launch per bucket, in order, on `stream`.  block[i]: lanes per workgroup of bucket i (64, 128 or
  * 256; NULL or 0 = the autotuned size of that width, else 256). */
 int lf_mixed_chains(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, void* stream);
-/* One hipGraph with a kernel node per bucket and no edges between them (launch with lf_graph_launch,
- * free with lf_graph_destroy).  The graph keeps the buckets' device pointers. */
+/* One hipGraph with one persistent kernel node per group of eight widths (2-8, 9-16, 17-24, 25-32 limbs) and
+ * no edges between them; each node runs all of its group's buckets (launch with lf_graph_launch, free with
+ * lf_graph_destroy).  The graph keeps the buckets' device pointers. */
 int lf_mixed_graph_create(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, void** graph_exec);
 /* Launch-shape autotuner: times the bucket kernel of `bits` for each workgroup size on `sample` synthetic
  * values (32-step chains, half fma) and keeps the fastest for this device and width. */

@@ -58,6 +58,10 @@ def _hip_jobs() -> list[tuple[Path, list[str]]]:
     for name in once:
         out = BUILD / f"{name}.o"
         jobs.append((out, [HIPCC, *HIP_FLAGS, "-c", str(CSRC / f"{name}.hip"), "-o", str(out)]))
+    # the mixed-precision group kernels: one translation unit per group of eight widths
+    for grp in range(4):
+        out = BUILD / f"mixed_group{grp}.o"
+        jobs.append((out, [HIPCC, *HIP_FLAGS, f"-DLF_GROUP={grp}", "-c", str(CSRC / "mixed_group.hip"), "-o", str(out)]))
     return jobs
 
 
@@ -91,7 +95,7 @@ def build_hip(jobs: int, incremental: bool = False) -> Path:
     every = _hip_jobs()
     work = [j for j in every if not incremental or _stale(*j)]
     # the heaviest translation units first
-    work.sort(key=lambda j: -int(j[0].stem.split("_w")[-1]) if "_w" in j[0].stem else -100)
+    work.sort(key=lambda j: -int(j[0].stem.split("_w")[-1]) if "_w" in j[0].stem else (-200 if "group" in j[0].stem else -100))
     failures = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for cmd, rc, log in ex.map(lambda j: _run(j[1]), work):

@@ -21,6 +21,8 @@ namespace {
 thread_local std::string last_error;
 }
 
+void lf_mixed_graph_release(void* graph_exec);
+
 void lf_set_last_error(const std::string& message) { last_error = message; }
 
 namespace {
@@ -294,6 +296,7 @@ int lf_graph_launch(void* graph_exec, void* stream)
 int lf_graph_destroy(void* graph_exec)
 {
     if (graph_exec) LF_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec), "hipGraphExecDestroy");
+    lf_mixed_graph_release(graph_exec); // the bucket tables of a mixed-precision graph, if it is one
     return LF_OK;
 }
 

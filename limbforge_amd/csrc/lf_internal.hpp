@@ -53,5 +53,9 @@ template <int N> hipError_t launch_mul_chain(const ChainArgs& g, hipStream_t s);
 template <int N> hipError_t launch_mixed_chain(const MixedArgs& g, unsigned block, hipStream_t s);
 // The kernel entry of launch_mixed_chain, for graph kernel nodes (arguments: one MixedArgs).
 template <int N> const void* mixed_chain_entry();
+// The persistent kernel of width group G (0: 2-8 limbs, 1: 9-16, 2: 17-24, 3: 25-32); arguments: a device
+// table of MixedArgs, the limb count of each, and the first workgroup of each bucket (plus the grid size).
+template <int G> const void* mixed_group_entry();
+constexpr int mixed_group_of(int N) { return N <= 8 ? 0 : (N - 1) / 8; }
 
 } // namespace lf

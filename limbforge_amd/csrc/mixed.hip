@@ -3,10 +3,12 @@
 //
 // A batch is a list of precision buckets (all values of one width, already resident as limb planes).
 // lf_mixed_chains issues one launch per bucket in order on a stream: the plain "one launch per
-// precision" form.  lf_mixed_graph_create builds one hipGraph whose kernel nodes are the buckets'
-// launches with no edges between them, so one graph launch runs every precision concurrently and small
-// buckets fill the CUs that large ones leave idle.  The autotuner times the bucket kernel of a width on a
-// synthetic sample for each workgroup size and keeps the fastest per device and width.
+// precision" form.  lf_mixed_graph_create builds one hipGraph of at most four persistent kernel nodes,
+// one per group of eight widths (mixed_group.hip), with no edges between them: each group kernel runs all
+// of its buckets in one launch (workgroups find their bucket in a device table), so one graph launch runs
+// every precision concurrently within the runtime's hardware queues.  The autotuner times the bucket
+// kernel of a width on a synthetic sample for each workgroup size and keeps the fastest per device and
+// width.
 #include "lf_internal.hpp"
 #include "limbforge_hip.h"
 
@@ -96,7 +98,28 @@ struct DevBuf {
     }
 };
 
+typedef const void* (*group_fn)();
+const group_fn group_entries[4] = {lf::mixed_group_entry<0>, lf::mixed_group_entry<1>, lf::mixed_group_entry<2>, lf::mixed_group_entry<3>};
+
+// device tables of the graphs made by lf_mixed_graph_create, freed by lf_graph_destroy
+std::mutex tables_mutex;
+std::map<void*, void*> graph_tables;
+
 } // namespace
+
+// called by lf_graph_destroy (capi.hip) after the executable graph is destroyed
+void lf_mixed_graph_release(void* graph_exec)
+{
+    void* p = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(tables_mutex);
+        auto it = graph_tables.find(graph_exec);
+        if (it == graph_tables.end()) return;
+        p = it->second;
+        graph_tables.erase(it);
+    }
+    (void)hipFree(p);
+}
 
 extern "C" {
 
@@ -123,24 +146,66 @@ int lf_mixed_graph_create(const lf_chain_bucket* buckets, int nbuckets, const un
         if (int r = check_bucket(buckets[i])) return r;
         if (block && block[i] && !valid_block(block[i])) return fail(LF_INVALID_ARGUMENT, "workgroup size must be 64, 128 or 256");
     }
+    // per group: the buckets' arguments, limb counts and first workgroups, in one device table
+    struct Group {
+        std::vector<lf::MixedArgs> args;
+        std::vector<int32_t> words;
+        std::vector<uint32_t> first{0};
+        unsigned block = 0;
+        int widest = 0;
+    } grp[4];
+    for (int i = 0; i < nbuckets; ++i) {
+        const lf_chain_bucket& b = buckets[i];
+        if (!b.count) continue;
+        const int N = b.bits / 32, k = lf::mixed_group_of(N);
+        if (N > grp[k].widest) {
+            grp[k].widest = N;
+            grp[k].block = block_for(block, i, b.bits);
+        }
+    }
+    for (int i = 0; i < nbuckets; ++i) {
+        const lf_chain_bucket& b = buckets[i];
+        if (!b.count) continue;
+        Group& G = grp[lf::mixed_group_of(b.bits / 32)];
+        G.args.push_back(args_of(b));
+        G.words.push_back(b.bits / 32);
+        G.first.push_back(G.first.back() + uint32_t((b.count + G.block - 1) / G.block));
+    }
+    std::size_t bytes = 0, off[4][3] = {};
+    for (int k = 0; k < 4; ++k) {
+        auto align = [&](std::size_t n) { bytes = (bytes + 255) / 256 * 256; const std::size_t o = bytes; bytes += n; return o; };
+        off[k][0] = align(grp[k].args.size() * sizeof(lf::MixedArgs));
+        off[k][1] = align(grp[k].words.size() * sizeof(int32_t));
+        off[k][2] = align(grp[k].first.size() * sizeof(uint32_t));
+    }
+    std::vector<unsigned char> host(std::max<std::size_t>(bytes, 1), 0);
+    for (int k = 0; k < 4; ++k) {
+        std::memcpy(host.data() + off[k][0], grp[k].args.data(), grp[k].args.size() * sizeof(lf::MixedArgs));
+        std::memcpy(host.data() + off[k][1], grp[k].words.data(), grp[k].words.size() * sizeof(int32_t));
+        std::memcpy(host.data() + off[k][2], grp[k].first.data(), grp[k].first.size() * sizeof(uint32_t));
+    }
+    DevBuf table;
+    MIXED_HIP(hipMalloc(&table.p, host.size()), "hipMalloc (bucket tables)");
+    MIXED_HIP(hipMemcpy(table.p, host.data(), host.size(), hipMemcpyHostToDevice), "upload bucket tables");
     hipGraph_t g = nullptr;
     MIXED_HIP(hipGraphCreate(&g, 0), "hipGraphCreate");
     struct GraphGuard {
         hipGraph_t g;
         ~GraphGuard() { (void)hipGraphDestroy(g); }
     } guard{g};
-    std::vector<lf::MixedArgs> args(nbuckets);
-    for (int i = 0; i < nbuckets; ++i) {
-        const lf_chain_bucket& b = buckets[i];
-        if (!b.count) continue;
-        args[i] = args_of(b);
-        const unsigned blk = block_for(block, i, b.bits);
-        void* params[] = {&args[i]};
+    for (int k = 0; k < 4; ++k) {
+        if (grp[k].args.empty()) continue;
+        const char* base = (const char*)table.p;
+        const lf::MixedArgs* a = (const lf::MixedArgs*)(base + off[k][0]);
+        const int32_t* w = (const int32_t*)(base + off[k][1]);
+        const uint32_t* f = (const uint32_t*)(base + off[k][2]);
+        uint32_t nb = uint32_t(grp[k].args.size());
+        void* params[] = {&a, &w, &f, &nb};
         hipKernelNodeParams p;
         std::memset(&p, 0, sizeof p);
-        p.func = const_cast<void*>(tables().entry[b.bits / 32]());
-        p.gridDim = dim3(unsigned((b.count + blk - 1) / blk));
-        p.blockDim = dim3(blk);
+        p.func = const_cast<void*>(group_entries[k]());
+        p.gridDim = dim3(grp[k].first.back());
+        p.blockDim = dim3(grp[k].block);
         p.sharedMemBytes = 0;
         p.kernelParams = params;
         p.extra = nullptr;
@@ -149,6 +214,11 @@ int lf_mixed_graph_create(const lf_chain_bucket* buckets, int nbuckets, const un
     }
     hipGraphExec_t x = nullptr;
     MIXED_HIP(hipGraphInstantiate(&x, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    {
+        std::lock_guard<std::mutex> lock(tables_mutex);
+        graph_tables[(void*)x] = table.p;
+    }
+    table.p = nullptr; // owned by the graph now
     *graph_exec = x;
     return LF_OK;
 }
