@@ -783,10 +783,13 @@ template <int W> __device__ __forceinline__ void accumulate_unit(u32 (&acc)[W], 
     }
 }
 
-// RN(a*b + c) with one rounding (the mpfr_fma contract of core.hpp:317-326).  Both terms go into one
-// two's-complement window of 3N+6 words whose top lies two bits above the larger term's msb; bits of
-// the smaller term that fall below the window are collapsed into one sticky unit (the argument of
-// core.hpp:251-253: such a term lies far below the larger one, so no cancellation can expose it).
+// RN(a*b + c) with one rounding (the mpfr_fma contract of core.hpp:317-326).  With P = a*b exact
+// (2N words) and C = c (N words), three cases, each giving the one correctly rounded result:
+//   c below half an ulp of P's last bit   only c's sign matters: P*2^32 +/- 1 (a sticky unit) is rounded;
+//   P below a quarter ulp of c           RN(c + p) = c (the nearest neighbours of c are at least half an
+//                                         ulp of c away, a quarter below a power of two);
+//   otherwise                            both terms span at most 96N bits: exact sum in a (3N+2)-word
+//                                         window whose bit 0 is the lower of the two last bits, rounded once.
 template <int N> __device__ __forceinline__ Num<N> fma(const Num<N>& a, const Num<N>& b, const Num<N>& c)
 {
     const u32 st = a.st | b.st | c.st;
@@ -800,41 +803,64 @@ template <int N> __device__ __forceinline__ Num<N> fma(const Num<N>& a, const Nu
     const i64 p_msb = (i64)a.e + b.e + (top ? 1 : 0);
     const i64 p_lsb = (i64)a.e + b.e - 2 * (32 * N - 1);
     const i64 c_msb = c.e, c_lsb = (i64)c.e - (32 * N - 1);
-    const i64 mx = p_msb > c_msb ? p_msb : c_msb;
-    constexpr int FW = 3 * N + 6;
-    // window bit i has weight 2^(L + i); the larger msb sits at bit 32FW-4, leaving a carry bit
-    // and the sign bit above it, and bit 0 is kept free for the sticky unit.
-    const i64 L = mx - (32 * FW - 4);
-    u32 wp[FW], wc[FW];
-    const bool sp = place<FW, 2 * N>(wp, p, p_lsb - L);
-    const bool sc = place<FW, N>(wc, c.w, c_lsb - L);
-    // acc = ps*P + cs*C in two's complement
-    u32 acc[FW];
+    if (c_msb < p_lsb - 1) {
+        // |c| < 2^(p_lsb-1): magnitude P*2^p_lsb + t*delta, 0 < delta < half of P's last unit
+        const int t = ps * c.s;
+        u32 x[2 * N + 1];
+        x[0] = t > 0 ? 1u : 0xffffffffu;
+        u32 bw = t > 0 ? 0u : 1u;
 #pragma unroll
-    for (int i = 0; i < FW; ++i) acc[i] = 0;
-    accumulate_signed<FW>(acc, wp, ps);
-    accumulate_signed<FW>(acc, wc, c.s);
-    i64 scale = L;
-    if (sp || sc) {
-        // Only the smaller term can lose bits below the window; the exact value is then
-        // acc * 2^L + t * delta with 0 < delta < 2^L and t that term's sign: append one sticky unit.
-        const int t = sp ? ps : c.s;
-        shl_bits(acc, 1);
-        accumulate_unit<FW>(acc, t);
-        scale = L - 1;
+        for (int i = 0; i < 2 * N; ++i) {
+            const u64 v = (u64)p[i] - bw;
+            x[i + 1] = (u32)v;
+            bw = (u32)(v >> 63);
+        }
+        return pack<N, 2 * N + 1>(x, p_lsb - 32, ps, ST_OK);
     }
-    int sign = 1;
-    if (acc[FW - 1] >> 31) {
-        sign = -1;
-        u32 cy = 1;
+    if (p_msb < c_lsb - 2) return c; // |p| < 2^(c_lsb-2)
+    constexpr int W = 3 * N + 2;
+    const bool p_low = p_lsb <= c_lsb; // the term at window bit 0: P, else C
+    const i64 L = p_low ? p_lsb : c_lsb;
+    const i64 off = p_low ? c_lsb - p_lsb : p_lsb - c_lsb; // the other term's offset, below 96N
+    u32 base[W], sh[W];
 #pragma unroll
-        for (int i = 0; i < FW; ++i) {
-            u64 v = (u64)(u32)~acc[i] + cy;
-            acc[i] = (u32)v;
+    for (int i = 0; i < W; ++i) {
+        const u32 pw = i < 2 * N ? p[i] : 0u, cw = i < N ? c.w[i] : 0u;
+        base[i] = p_low ? pw : cw;
+        sh[i] = p_low ? cw : pw;
+    }
+    shl_bits<W>(sh, (int)(off & 31));
+    shl_words<W>(sh, (int)(off >> 5));
+    const int sb = p_low ? ps : c.s, ss = p_low ? c.s : ps;
+    int sign = sb;
+    if (sb == ss) {
+        u32 cy = 0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            const u64 v = (u64)base[i] + sh[i] + cy;
+            base[i] = (u32)v;
             cy = (u32)(v >> 32);
         }
+    } else {
+        u32 bw = 0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            const u64 v = (u64)base[i] - sh[i] - bw;
+            base[i] = (u32)v;
+            bw = (u32)(v >> 63);
+        }
+        if (base[W - 1] >> 31) { // the shifted term was larger: negate
+            sign = ss;
+            u32 cy = 1;
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                const u64 v = (u64)(u32)~base[i] + cy;
+                base[i] = (u32)v;
+                cy = (u32)(v >> 32);
+            }
+        }
     }
-    return pack<N, FW>(acc, scale, sign, ST_OK);
+    return pack<N, W>(base, L, sign, ST_OK);
 }
 
 // Position of the highest set bit of x, or -1 (a scan over compile-time indices).

@@ -69,6 +69,26 @@ template <int N> void run_width(int cases)
         auto dc = to_dev<N>(c);
         report<N>("fma", i, lf::fma<N>(da, db, dc), lfo_fma(N, a, b, c));
         report<N>("fms", i, lf::fms<N>(da, db, dc), lfo_fms(N, a, b, c));
+        // fma at the borders of its near and far cases: c's top bit around P's last bit, P's top bit around
+        // c's last bit, c a power of two, both signs
+        if (i % 4 == 0 && a.sign && b.sign) {
+            const long pe = (long)a.exponent + b.exponent, plsb = pe - 2 * (32L * N - 1);
+            const int shifts[] = {-3, -2, -1, 0, 1};
+            for (int k = 0; k < 5; ++k)
+                for (int side = 0; side < 2; ++side) {
+                    lfo_num cc = c;
+                    if (!cc.sign) cc = lfo_random_number(&g, N, 10);
+                    // side 0: c_msb = p_lsb + shift; side 1: c_lsb = p_msb + 2 - shift (p_msb ~ pe or pe+1)
+                    cc.exponent = side == 0 ? (int)(plsb + shifts[k]) : (int)(pe + 2 + shifts[k] + 32L * N - 1);
+                    if (k % 2) {
+                        for (int w = 0; w < N; ++w) cc.limb[w] = 0;
+                        cc.limb[N - 1] = 0x80000000u;
+                    }
+                    if (cc.exponent > 1000000000 || cc.exponent < -1000000000) continue;
+                    cc.sign = (k + side) % 2 ? 1 : -1;
+                    report<N>("fma border", i, lf::fma<N>(da, db, to_dev<N>(cc)), lfo_fma(N, a, b, cc));
+                }
+        }
         if (i % 4 == 0) {
             lfo_num x[2], y[2], r[2];
             for (int p = 0; p < 2; ++p) { x[p] = lfo_random_number(&g, N, 1000); y[p] = lfo_random_number(&g, N, 1000); }
