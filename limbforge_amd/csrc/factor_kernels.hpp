@@ -270,23 +270,37 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
     constexpr int W = factor_window(N);
     constexpr int CPL = (W + WAVE - 1) / WAVE; // window columns per lane
     constexpr i64 NONE = -((i64)1 << 62);
-    // pass 1: statuses, scale range and count of the nonzero products
+    // pass 1: statuses, scale range and count of the nonzero products.  With one chunk of terms (K <= 32)
+    // each lane loads its term once, here, and keeps it for pass 2.
+    const bool one_chunk = K <= (u32)DOT_ROWS;
+    Num<N> ta = zero<N>(ST_OK), tb = zero<N>(ST_OK);
     u32 st = 0;
     i64 hi = NONE, lo = -NONE;
     int count = 0;
-    for (u32 k = lane; k < K; k += WAVE) {
-        int ex, sx, ey, sy;
-        u32 tx, ty;
-        vec_meta<N>(x, k, ex, sx, tx);
-        vec_meta<N>(y, k, ey, sy, ty);
-        st |= tx | ty;
-        if (sx && sy) {
-            const i64 s = (i64)ex + ey - 2 * (32 * N - 1);
-            hi = s > hi ? s : hi;
-            lo = s < lo ? s : lo;
-            ++count;
+    if (one_chunk) {
+        if (lane < K) {
+            ta = vec_load<N>(x, lane);
+            tb = vec_load<N>(y, lane);
+            st = ta.st | tb.st;
+            if (ta.s && tb.s) {
+                hi = lo = (i64)ta.e + tb.e - 2 * (32 * N - 1);
+                count = 1;
+            }
         }
-    }
+    } else
+        for (u32 k = lane; k < K; k += WAVE) {
+            int ex, sx, ey, sy;
+            u32 tx, ty;
+            vec_meta<N>(x, k, ex, sx, tx);
+            vec_meta<N>(y, k, ey, sy, ty);
+            st |= tx | ty;
+            if (sx && sy) {
+                const i64 s = (i64)ex + ey - 2 * (32 * N - 1);
+                hi = s > hi ? s : hi;
+                lo = s < lo ? s : lo;
+                ++count;
+            }
+        }
     st = (u32)wave_or((int)st) | c.st;
     if (st) return zero<N>(st);
     count = wave_sum(count);
@@ -314,7 +328,7 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
         const u32 rows = K - kb < (u32)DOT_ROWS ? K - kb : (u32)DOT_ROWS;
         if (lane < rows) {
             const u32 k = kb + lane;
-            const Num<N> a = vec_load<N>(x, k), b = vec_load<N>(y, k);
+            const Num<N> a = one_chunk ? ta : vec_load<N>(x, k), b = one_chunk ? tb : vec_load<N>(y, k);
             u32 p[2 * N];
             mantissa_product<N>(a.w, b.w, p);
             put_row<N, 2 * N>(L, (int)lane, p, (a.s && b.s) ? (i64)a.e + b.e - 2 * (32 * N - 1) - lo : 0, -a.s * b.s);

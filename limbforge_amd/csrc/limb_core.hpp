@@ -623,12 +623,45 @@ template <int N> __device__ __noinline__ Num<N> sqrt_restoring(const Num<N>& a)
     return r;
 }
 
+template <int N, int M> __device__ __forceinline__ Num<N> cast(const Num<M>& x);
+
+// y ~ sqrt(v) for v > 0, accurate to about N words: Newton's step y <- (y + v/y) / 2 at N words on top of
+// the result for v rounded to N/2 + 1 words (so each level doubles the precision at a quarter of the cost
+// of a full-width step), down to two words seeded from the double-precision square root of v's top 64 bits.
+template <int N> __device__ Num<N> sqrt_newton(const Num<N>& v)
+{
+    Num<N> y;
+    if constexpr (N <= 2) {
+        const int ev = v.e >= 0 ? v.e / 2 : -((1 - v.e) / 2); // floor(e / 2)
+        const int odd = v.e - 2 * ev;
+        const u64 top = ((u64)v.w[N - 1] << 32) | (N > 1 ? v.w[N - 2] : 0u);
+        const double u = (double)top * (odd ? 1.0 : 0.5);            // in [2^62, 2^64)
+        const u64 s0 = (u64)(__builtin_sqrt(u) * 2147483648.0) << 1; // sqrt(u) * 2^32, in [2^63, 2^64)
+#pragma unroll
+        for (int i = 0; i < N; ++i) y.w[i] = i == N - 1 ? (u32)(s0 >> 32) : (i == N - 2 ? (u32)s0 : 0u);
+        y.e = ev;
+        y.s = 1;
+        y.st = ST_OK;
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            y = add<N>(y, div<N>(v, y));
+            y.e -= 1;
+        }
+    } else {
+        constexpr int H = N / 2 + 1;
+        y = cast<N, H>(sqrt_newton<H>(cast<H, N>(v)));
+        y = add<N>(y, div<N>(v, y));
+        y.e -= 1;
+    }
+    return y;
+}
+
 // RN(sqrt(a)).  The mantissa m (N words, top bit set) and the exponent parity define the integer
 // X = m * 2^(32N-1+odd) < 2^(64N), whose integer square root r has exactly 32N bits; the result is r, plus
 // one ulp exactly when X - r^2 > r (the contract of core.hpp:216-235: no ties are possible).  r comes from
 // Newton's iteration y <- (y + X/y) / 2 in N-word arithmetic started from a double-precision seed, and is
 // then made exact by integer checks r^2 <= X < (r+1)^2 (a squaring and a compare each), so the result
-// does not depend on how close the iteration gets.  Should the checks not settle within a few steps, the
+// does not depend on how close the iteration gets.  The iteration runs at increasing widths (sqrt_newton).  Should the checks not settle within a few steps, the
 // bit-serial restoring method decides.
 template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
 {
@@ -646,27 +679,13 @@ template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
         const u32 hi = i < N ? a.w[i] : 0u, lo = i ? a.w[i - 1] : 0u;
         X[N - 1 + i] = __builtin_amdgcn_alignbit(hi, lo, odd ? 0u : 1u);
     }
-    // seed: sqrt of the top 64 mantissa bits in double precision, as an N-word value of exponent 32N-1
-    const u64 top = ((u64)a.w[N - 1] << 32) | a.w[N - 2];
-    const double u = (double)top * (odd ? 1.0 : 0.5);   // in [2^62, 2^64)
-    const u64 s0 = (u64)(__builtin_sqrt(u) * 2147483648.0) << 1; // sqrt(u) * 2^32, in [2^63, 2^64)
-    Num<N> v, y;
+    Num<N> v;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        v.w[i] = a.w[i];
-        y.w[i] = i == N - 1 ? (u32)(s0 >> 32) : (i == N - 2 ? (u32)s0 : 0u);
-    }
+    for (int i = 0; i < N; ++i) v.w[i] = a.w[i];
     v.e = 64 * N - 2 + odd; // value X
-    v.s = y.s = 1;
-    v.st = y.st = ST_OK;
-    y.e = 32 * N - 1;
-    // Newton steps: the seed's ~50 correct bits double per step
-    constexpr int STEPS = 32 * N <= 50 ? 1 : 32 * N <= 100 ? 2 : 32 * N <= 200 ? 3 : 32 * N <= 400 ? 4 : 32 * N <= 800 ? 5 : 6;
-#pragma unroll 1
-    for (int it = 0; it < STEPS; ++it) {
-        y = add<N>(y, div<N>(v, y));
-        y.e -= 1;
-    }
+    v.s = 1;
+    v.st = ST_OK;
+    const Num<N> y = sqrt_newton<N>(v);
     u32 r[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) r[i] = y.e == 32 * N - 1 ? y.w[i] : (y.e > 32 * N - 1 ? 0xffffffffu : (i == N - 1 ? 0x80000000u : 0u));
