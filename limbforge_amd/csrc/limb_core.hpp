@@ -298,11 +298,28 @@ template <int N> __device__ __forceinline__ void to_digits(const u32 (&w)[N], u3
     }
 }
 
-// p (2N words) = a * b from the digits of a and b (each < 2^(32N)).
-template <int N>
-__device__ __forceinline__ void digit_product(const u32 (&da)[digit_count<N>()], const u32 (&db)[digit_count<N>()], u32 (&p)[2 * N])
+// Short products (the truncated "high half" product of MPFR's mpfr_mulhigh_n, after Krandick & Johnson,
+// "Efficient multiprecision floating point multiplication with optimal directional rounding", 1993):
+// the digit columns below C0 = short_cut<N>() are left out.  Column k sums at most k+1 products below
+// 2^58, so the omitted part is delta < C0 * 2^(29*C0 + 29) / (1 - 2^-29), and C0 is the largest count
+// with C0 < 2^(32N - 61 - 29*C0), which keeps delta < 2^(32N-32), the weight of word N-1 of the product.
+template <int N> constexpr int short_cut()
 {
-    constexpr int D = digit_count<N>();
+    int best = 0;
+    for (int c = 1; c <= digit_count<N>(); ++c) {
+        const int e = 32 * N - 61 - DIGIT_BITS * c;
+        if (e >= 0 && (e >= 30 || c < (1 << e))) best = c;
+    }
+    return best;
+}
+
+// p (2N words) = a * b from the digits of a and b (each < 2^(32N)) when full; otherwise the short
+// product P' = a * b - delta, 0 <= delta < 2^(32N-32), from the columns from short_cut<N>() up.
+template <int N>
+__device__ __forceinline__ void digit_columns(const u32 (&da)[digit_count<N>()], const u32 (&db)[digit_count<N>()], u32 (&p)[2 * N],
+                                              bool full)
+{
+    constexpr int D = digit_count<N>(), C0 = short_cut<N>();
     static_assert(D <= 63, "column sums of 29-bit digit products must fit in 64 bits");
     u64 col[2 * D];
 #pragma unroll
@@ -310,10 +327,26 @@ __device__ __forceinline__ void digit_product(const u32 (&da)[digit_count<N>()],
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
-        for (int j = 0; j < D; ++j) col[i + j] += (u64)da[i] * db[j];
+        for (int j = 0; j < D; ++j)
+            if (i + j >= C0) col[i + j] += (u64)da[i] * db[j];
     u32 dg[2 * D];
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < 2 * D - 1; ++k) {
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+                if (i + j < C0) col[i + j] += (u64)da[i] * db[j];
+#pragma unroll
+        for (int k = 0; k < C0; ++k) {
+            dg[k] = (u32)col[k] & DIGIT_MASK;
+            col[k + 1] += col[k] >> DIGIT_BITS;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < C0; ++k) dg[k] = 0;
+    }
+#pragma unroll
+    for (int k = C0; k < 2 * D - 1; ++k) {
         dg[k] = (u32)col[k] & DIGIT_MASK;
         col[k + 1] += col[k] >> DIGIT_BITS;
     }
@@ -326,6 +359,40 @@ __device__ __forceinline__ void digit_product(const u32 (&da)[digit_count<N>()],
         if (k + 2 < 2 * D && 2 * DIGIT_BITS - s < 32) v |= dg[k + 2] << (2 * DIGIT_BITS - s);
         p[w] = v;
     }
+}
+
+// p (2N words) = a * b from the digits of a and b (each < 2^(32N)).
+template <int N>
+__device__ __forceinline__ void digit_product(const u32 (&da)[digit_count<N>()], const u32 (&db)[digit_count<N>()], u32 (&p)[2 * N])
+{
+    digit_columns<N>(da, db, p, true);
+}
+
+// Whether round_product of a short product P' of normalized operands is RN(a*b).  With A = P' div
+// 2^(32N-32) and B = P' mod 2^(32N-32), a*b = A * 2^(32N-32) + (B + delta), B + delta < 2^(32N-31).
+// The round bit r is bit 32N-1 (top bit of P set) or 32N-2, so the low r-(32N-32) bits of A (31 or 30)
+// lie between it and the omitted part: unless they are all ones, no carry reaches r and mantissa and
+// round bit are those of P'; the sticky bit is too, unless those bits and B are all zero.
+template <int N> __device__ __forceinline__ bool short_decides(const u32 (&p)[2 * N])
+{
+    const u32 low = (p[2 * N - 1] >> 31) ? 0x7fffffffu : 0x3fffffffu;
+    const u32 gl = p[N - 1] & low;
+    u32 any = gl;
+#pragma unroll
+    for (int i = 0; i < N - 1; ++i) any |= p[i];
+    return gl != low && any != 0;
+}
+
+// p = the product a * b as far as round_product needs it: the short product when it decides the
+// rounding, else (or with full set) the exact product.
+template <int N>
+__device__ __forceinline__ void rounding_product(const u32 (&da)[digit_count<N>()], const u32 (&db)[digit_count<N>()], u32 (&p)[2 * N],
+                                                 bool full)
+{
+    // two calls rather than a retry loop: in a loop the compiler hoists the low columns out of the
+    // loop as invariant and forms them on every call
+    digit_columns<N>(da, db, p, full);
+    if (!full && !short_decides<N>(p)) digit_columns<N>(da, db, p, true);
 }
 
 // p (2N words) = a * b: digit columns from 8 limbs on, schoolbook rows of v_mad_u64_u32 carry chains below.
@@ -397,7 +464,13 @@ template <int N> __device__ __forceinline__ Num<N> mul(const Num<N>& a, const Nu
     const u32 st = a.st | b.st;
     if (st || !a.s || !b.s) return zero<N>(st);
     u32 p[2 * N];
-    mantissa_product<N>(a.w, b.w, p);
+    if constexpr (use_digits<N>()) {
+        u32 da[digit_count<N>()], db[digit_count<N>()];
+        to_digits<N>(a.w, da);
+        to_digits<N>(b.w, db);
+        rounding_product<N>(da, db, p, false);
+    } else
+        mantissa_product<N>(a.w, b.w, p);
     return round_product<N>(p, (i64)a.e + b.e, a.s * b.s, ST_OK);
 }
 
@@ -408,7 +481,7 @@ template <int N> __device__ __forceinline__ Num<N> mul_by_digits(const Num<N>& a
     if (st || !a.s || !b.s) return zero<N>(st);
     u32 ad[digit_count<N>()], p[2 * N];
     to_digits<N>(a.w, ad);
-    digit_product<N>(ad, bd, p);
+    rounding_product<N>(ad, bd, p, false);
     return round_product<N>(p, (i64)a.e + b.e, a.s * b.s, ST_OK);
 }
 
@@ -809,18 +882,14 @@ template <int W> __device__ __forceinline__ void accumulate_unit(u32 (&acc)[W], 
 //                                         ulp of c away, a quarter below a power of two);
 //   otherwise                            both terms span at most 96N bits: exact sum in a (3N+2)-word
 //                                         window whose bit 0 is the lower of the two last bits, rounded once.
-template <int N> __device__ __forceinline__ Num<N> fma(const Num<N>& a, const Num<N>& b, const Num<N>& c)
+// The cases after P is formed: p = the 2N-word product of two nonzero status-free operands whose
+// exponents sum to eab, ps = the product's sign.
+template <int N> __device__ __forceinline__ Num<N> fma_tail(const u32 (&p)[2 * N], i64 eab, int ps, const Num<N>& c)
 {
-    const u32 st = a.st | b.st | c.st;
-    if (st) return zero<N>(st);
-    const int ps = a.s * b.s;
-    if (!ps) return c.s ? c : zero<N>(ST_OK); // RN(c) = c
-    u32 p[2 * N];
-    mantissa_product<N>(a.w, b.w, p);
-    if (!c.s) return round_product<N>(p, (i64)a.e + b.e, ps, ST_OK);
+    if (!c.s) return round_product<N>(p, eab, ps, ST_OK);
     const bool top = (p[2 * N - 1] >> 31) != 0;
-    const i64 p_msb = (i64)a.e + b.e + (top ? 1 : 0);
-    const i64 p_lsb = (i64)a.e + b.e - 2 * (32 * N - 1);
+    const i64 p_msb = eab + (top ? 1 : 0);
+    const i64 p_lsb = eab - 2 * (32 * N - 1);
     const i64 c_msb = c.e, c_lsb = (i64)c.e - (32 * N - 1);
     if (c_msb < p_lsb - 1) {
         // |c| < 2^(p_lsb-1): magnitude P*2^p_lsb + t*delta, 0 < delta < half of P's last unit
@@ -880,6 +949,33 @@ template <int N> __device__ __forceinline__ Num<N> fma(const Num<N>& a, const Nu
         }
     }
     return pack<N, W>(base, L, sign, ST_OK);
+}
+
+template <int N> __device__ __forceinline__ Num<N> fma(const Num<N>& a, const Num<N>& b, const Num<N>& c)
+{
+    const u32 st = a.st | b.st | c.st;
+    if (st) return zero<N>(st);
+    const int ps = a.s * b.s;
+    if (!ps) return c.s ? c : zero<N>(ST_OK); // RN(c) = c
+    u32 p[2 * N];
+    mantissa_product<N>(a.w, b.w, p);
+    return fma_tail<N>(p, (i64)a.e + b.e, ps, c);
+}
+
+// fma with the digits of b's mantissa given (a multiplier reused over a chain); needs use_digits<N>().
+// With c = 0 this is RN(a * b), the result of mul_by_digits: a chain whose steps mix mul and fma runs
+// every step through here with c = 0 on its mul steps, so the lanes of a wave share one product per step.
+template <int N>
+__device__ __forceinline__ Num<N> fma_by_digits(const Num<N>& a, const Num<N>& b, const u32 (&bd)[digit_count<N>()], const Num<N>& c)
+{
+    const u32 st = a.st | b.st | c.st;
+    if (st) return zero<N>(st);
+    const int ps = a.s * b.s;
+    if (!ps) return c.s ? c : zero<N>(ST_OK);
+    u32 ad[digit_count<N>()], p[2 * N];
+    to_digits<N>(a.w, ad);
+    rounding_product<N>(ad, bd, p, c.s != 0); // with c = 0 only the rounding of P is needed
+    return fma_tail<N>(p, (i64)a.e + b.e, ps, c);
 }
 
 // Position of the highest set bit of x, or -1 (a scan over compile-time indices).

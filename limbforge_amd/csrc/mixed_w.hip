@@ -26,9 +26,9 @@ template <int N> __global__ void __launch_bounds__(256) mixed_chain_kernel(Mixed
     if constexpr (use_digits<N>()) {
         u32 fd[digit_count<N>()]; // the factor's digits, formed once for the chain's multiplications
         to_digits<N>(f.w, fd);
-        for (u32 t = 0; t < len; ++t) x = ((mask >> t) & 1u) ? fma<N>(x, f, g) : mul_by_digits<N>(x, f, fd);
+        for (u32 t = 0; t < len; ++t) x = fma_by_digits<N>(x, f, fd, ((mask >> t) & 1u) ? g : zero<N>(ST_OK));
     } else
-        for (u32 t = 0; t < len; ++t) x = ((mask >> t) & 1u) ? fma<N>(x, f, g) : mul<N>(x, f);
+        for (u32 t = 0; t < len; ++t) x = fma<N>(x, f, ((mask >> t) & 1u) ? g : zero<N>(ST_OK));
     store_num<N>(a.out, a.pitch, i, x);
 }
 

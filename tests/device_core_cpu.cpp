@@ -48,6 +48,22 @@ template <int N> void run_width(int cases)
         report<N>("add", i, lf::add<N>(da, db), lfo_add(N, a, b));
         report<N>("sub", i, lf::sub<N>(da, db), lfo_sub(N, a, b));
         report<N>("mul", i, lf::mul<N>(da, db), lfo_mul(N, a, b));
+        if (a.sign && b.sign) { // structured mantissas: the short product's undecided cases (runs of zeros
+                                // or ones next to the round bit) must fall back to the exact product
+            lfo_num s = a, t = b;
+            const int za = (int)(lfo_mt_next(&g) % N), zb = (int)(lfo_mt_next(&g) % N);
+            const uint32_t fill[4] = {0u, 0xffffffffu, 0u, 0x80000000u};
+            for (int w = 0; w < za; ++w) s.limb[w] = fill[i % 4];
+            for (int w = 0; w < zb; ++w) t.limb[w] = fill[(i / 4) % 4];
+            if (i % 5 == 1) for (int w = 0; w < N; ++w) s.limb[w] = 0xffffffffu;
+            if (i % 7 == 2) { for (int w = 0; w < N; ++w) t.limb[w] = 0; t.limb[N - 1] = 0x80000000u; }
+            report<N>("mul structured", i, lf::mul<N>(to_dev<N>(s), to_dev<N>(t)), lfo_mul(N, s, t));
+            if constexpr (lf::use_digits<N>()) {
+                uint32_t td[lf::digit_count<N>()];
+                lf::to_digits<N>(to_dev<N>(t).w, td);
+                report<N>("mul structured digits", i, lf::mul_by_digits<N>(to_dev<N>(s), to_dev<N>(t), td), lfo_mul(N, s, t));
+            }
+        }
         if constexpr (lf::use_digits<N>()) { // the chain's multiply with the factor's digits formed once
             uint32_t bd[lf::digit_count<N>()];
             lf::to_digits<N>(db.w, bd);
@@ -58,6 +74,26 @@ template <int N> void run_width(int cases)
                 y = lfo_mul(N, y, b);
             }
             report<N>("mul chain", i, x, y);
+        }
+        {   // a mixed chain as the batching kernels run it: mul steps are fma steps with c = 0
+            lfo_num g0 = lfo_random_number(&g, N, 64);
+            if (i % 7 == 3) g0 = lfo_zero(0);
+            if (i % 11 == 5) g0.status = 4;
+            const uint64_t mask = lfo_mt_next(&g);
+            auto x = da;
+            const auto dg = to_dev<N>(g0);
+            lfo_num y = a;
+            for (int s = 0; s < 6; ++s) {
+                const bool f_step = (mask >> s) & 1u;
+                if constexpr (lf::use_digits<N>()) {
+                    uint32_t bd[lf::digit_count<N>()];
+                    lf::to_digits<N>(db.w, bd);
+                    x = lf::fma_by_digits<N>(x, db, bd, f_step ? dg : lf::zero<N>(0));
+                } else
+                    x = lf::fma<N>(x, db, f_step ? dg : lf::zero<N>(0));
+                y = f_step ? lfo_fma(N, y, b, g0) : lfo_mul(N, y, b);
+            }
+            report<N>("mixed chain", i, x, y);
         }
         report<N>("div", i, lf::div<N>(da, db), lfo_div(N, a, b));
         root = a;
