@@ -182,16 +182,15 @@ __device__ __noinline__ Num<N> dot_sub_clusters(const Num<N>& c, const VecView& 
 // k = chunk + l and writes it into LDS row l as a dense W-word two's-complement window (the product shifted
 // to its offset from the smallest term, negated for a subtracted product); lane w then sums window column
 // w over the rows as an unsigned 64-bit column sum, independent loads with no carries between columns;
-// lane 0 resolves the carries (mod 2^(32W), which holds the signed total), negates a negative window and
-// rounds once.  A window of W = 2N + 24 words holds terms spread over up to ~700 bits; wider spreads take
-// the cluster path on lane 0.  Every lane of the wave calls it with the same arguments; the result is
-// valid in lane 0.
+// the lanes then resolve the carries together (mod 2^(32W), which holds the signed total), negate a
+// negative window and round once (wave_round).  A window of W = 2N + 24 words holds terms spread over up
+// to ~700 bits; wider spreads take the cluster path on lane 0.  Every lane of the wave calls it with the
+// same arguments; the result is valid in lane 0 (in every lane on the window path).
 constexpr int WAVE = 64;
 constexpr int DOT_ROWS = 32; // terms per chunk
 
 template <int N> struct WaveLds {
     u32 row[DOT_ROWS + 1][factor_window(N)]; // row DOT_ROWS holds c
-    u64 col[factor_window(N)];
 };
 
 __device__ __forceinline__ void wave_sync()
@@ -262,6 +261,103 @@ template <int N, int S> __device__ __forceinline__ void put_row(WaveLds<N>& L, i
 #pragma unroll
     for (int t = 0; t <= S; ++t)
         if (o + t < W) L.row[r][o + t] = q[t];
+}
+
+// Carries of one 64-word group of a multiword sum, all lanes at once: lane l's word has a carry-out g
+// (generate) or is all ones p (propagate); with A = G | P and B = G as bit vectors over the lanes
+// (G and P disjoint), A + B + cin ripples exactly those carries, so bit l of (A + B + cin) ^ A ^ B is the
+// carry into lane l's word.  Returns that bit; cin becomes the group's carry-out.
+__device__ __forceinline__ u32 wave_carry_in(bool g, bool p, u32 lane, u64& cin)
+{
+    const u64 G = __ballot(g), A = G | __ballot(p);
+    const u64 s1 = A + G, s2 = s1 + cin;
+    const u64 c = s2 ^ A ^ G;
+    cin = (s1 < A || s2 < s1) ? 1u : 0u;
+    return (u32)(c >> lane) & 1u;
+}
+
+// The rounded value of a W-word two's-complement window whose column w (w = lane + 64t, t < CPL) holds
+// the 64-bit sum S[t] of that word over the rows: RN-even of the window's value * 2^scale, in every
+// lane.  Lane-parallel: the column sums are split into words plus a carry word into the next column,
+// single-bit carries are resolved across lanes by wave_carry_in, a negative total is negated the same
+// way, and the top word, the N mantissa words, the round bit and the sticky bit are found with ballots
+// and lane reads (the serial form is pack() of the resolved window).
+template <int N>
+__device__ __forceinline__ Num<N> wave_round(const u64 (&S)[(factor_window(N) + WAVE - 1) / WAVE], i64 scale, u32 lane)
+{
+    constexpr int W = factor_window(N);
+    constexpr int CPL = (W + WAVE - 1) / WAVE;
+    u32 word[CPL];
+    {
+        u64 cin = 0;
+        u32 hlast = 0; // carry word of column 64t - 1 (lane 63 of the previous group)
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const bool in = (int)lane + WAVE * t < W;
+            const u32 h = (u32)(S[t] >> 32); // at most the row count: a single carry bit into the next word
+            u32 hprev = __shfl_up(h, 1u);
+            hprev = lane == 0 ? hlast : hprev;
+            hlast = __builtin_amdgcn_readlane(h, WAVE - 1);
+            const u64 ts = (u64)(u32)S[t] + hprev;
+            const u32 u = in ? (u32)ts : 0u;
+            const u32 c = wave_carry_in(in && (ts >> 32) != 0, in && u == 0xffffffffu, lane, cin);
+            word[t] = in ? u + c : 0u;
+        }
+    }
+    constexpr int TL = (W - 1) % WAVE, TG = (W - 1) / WAVE;
+    const int sign = (__builtin_amdgcn_readlane(word[TG], TL) >> 31) ? -1 : 1;
+    if (sign < 0) {
+        u64 cin = 1;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const bool in = (int)lane + WAVE * t < W;
+            const u32 v = in ? ~word[t] : 0u;
+            const u32 c = wave_carry_in(false, in && v == 0xffffffffu, lane, cin);
+            word[t] = in ? v + c : 0u;
+        }
+    }
+    // the top nonzero word
+    int top = -1;
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+        const u64 nz = __ballot(word[t] != 0);
+        if (nz) top = WAVE * t + 63 - __clzll(nz);
+    }
+    if (top < 0) return zero<N>(ST_OK);
+    u32 topword = 0;
+#pragma unroll
+    for (int t = 0; t < CPL; ++t)
+        if (top / WAVE == t) topword = __builtin_amdgcn_readlane(word[t], top % WAVE);
+    const int lz = __clz(topword);
+    // normalized words: the window shifted left by lz bits
+    u32 nw[CPL];
+    {
+        u32 last = 0;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            u32 prev = __shfl_up(word[t], 1u);
+            prev = lane == 0 ? last : prev;
+            last = __builtin_amdgcn_readlane(word[t], WAVE - 1);
+            nw[t] = lz ? (word[t] << lz) | (prev >> (32 - lz)) : word[t];
+        }
+    }
+    const int rj = top - N; // the word holding the round bit
+    bool sticky = false;
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) sticky = sticky || __ballot(nw[t] != 0 && (int)lane + WAVE * t < rj) != 0;
+    auto read = [&](int j) -> u32 {
+        u32 v = 0;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t)
+            if (j >= 0 && j / WAVE == t) v = __builtin_amdgcn_readlane(nw[t], j % WAVE);
+        return v;
+    };
+    const u32 rw = read(rj);
+    sticky = sticky || (rw << 1) != 0;
+    u32 m[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) m[k] = read(rj + 1 + k);
+    return round_words<N>(m, rw >> 31, sticky, scale + 32 * (i64)top + 31 - lz, sign, ST_OK);
 }
 
 template <int N>
@@ -348,33 +444,7 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
         }
         wave_sync();
     }
-#pragma unroll
-    for (int t = 0; t < CPL; ++t)
-        if ((int)lane + WAVE * t < W) L.col[lane + WAVE * t] = S[t];
-    wave_sync();
-    Num<N> r = zero<N>(ST_OK);
-    if (lane == 0) {
-        u32 acc[W];
-        u64 cy = 0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            const u64 v = L.col[w] + cy;
-            acc[w] = (u32)v;
-            cy = v >> 32;
-        }
-        int sign = 1;
-        if (acc[W - 1] >> 31) {
-            sign = -1;
-            u32 cc = 1;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                const u64 v = (u64)(u32)~acc[w] + cc;
-                acc[w] = (u32)v;
-                cc = (u32)(v >> 32);
-            }
-        }
-        r = pack<N, W>(acc, lo, sign, ST_OK);
-    }
+    const Num<N> r = wave_round<N>(S, lo, lane);
     wave_sync(); // the rows and columns are reused by the wave's next dot
     return r;
 }
