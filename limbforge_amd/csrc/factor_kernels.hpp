@@ -190,7 +190,8 @@ constexpr int WAVE = 64;
 constexpr int DOT_ROWS = 32; // terms per chunk
 
 template <int N> struct WaveLds {
-    u32 row[DOT_ROWS + 1][factor_window(N)]; // row DOT_ROWS holds c
+    u32 row[DOT_ROWS + 1][factor_window(N) + 1]; // row DOT_ROWS holds c; odd row stride (W is even):
+                                                 // the lanes writing one column of their rows hit distinct banks
 };
 
 __device__ __forceinline__ void wave_sync()
@@ -455,16 +456,21 @@ template <int N> __device__ __forceinline__ Num<N> element(const FactorArgs& f, 
     return load_out<N>(f.w, f.pitch, (u64)i * f.n + j);
 }
 
-// Waves of the diagonal-block workgroup: as many as the LDS rows of one wave each allow in 48 KB.
+// Waves of the diagonal-block workgroup: 12 up to 12 limbs (three waves per SIMD leave 170 VGPRs a
+// lane, enough for the exact dot there), 8 above; fewer when their LDS rows would pass 144 KB.
 template <int N> constexpr int diag_waves()
 {
-    constexpr int w = (int)(49152 / sizeof(WaveLds<N>));
-    return w < 1 ? 1 : (w > 16 ? 16 : w);
+    constexpr int by_regs = N <= 12 ? 12 : 8;
+    constexpr int by_lds = (int)(147456 / sizeof(WaveLds<N>));
+    return by_lds < 1 ? 1 : (by_lds < by_regs ? by_lds : by_regs);
 }
 
 // The diagonal block [k0, k1) of a panel, column by column, in one workgroup of diag_waves<N>()
-// wavefronts: the pivot of column j on wave 0 (exact dot, then sqrt on lane 0), then the entries below
-// it in the block, one row per wave at a time.  Stops at a failing pivot.
+// wavefronts.  Column j in two phases: (A) wave 0 forms the pivot's exact dot and its square root
+// (lane 0) while the other waves form the numerators D(a_ij; l_i, l_j; [k0, j)) of the rows below, one
+// row per wave at a time, written in place (they read only columns before j); (B) one thread per row
+// divides its numerator by l_jj.  Stops at a failing pivot, before any entry of its column is written
+// (numerators of the rows below included: phase B is never reached).
 template <int N> __global__ void __launch_bounds__(diag_waves<N>() * WAVE) chol_diag_kernel(FactorArgs f)
 {
     constexpr int DW = diag_waves<N>();
@@ -489,13 +495,19 @@ template <int N> __global__ void __launch_bounds__(diag_waves<N>() * WAVE) chol_
                 }
             }
         }
+        if (DW == 1 || wave > 0) {
+            const u32 step = DW > 1 ? DW - 1 : 1;
+            for (u32 i = j + 1 + (DW > 1 ? wave - 1 : 0); i < f.k1; i += step) {
+                const VecView ri{f.w, f.pitch, (i64)i * f.n + f.k0, 1};
+                const Num<N> v = wave_dot_sub<N>(element<N>(f, i, j), ri, rj, j - f.k0, lane, L, sc);
+                if (lane == 0) store_out<N>(f.w, f.pitch, f.n, i, j, false, v);
+            }
+        }
         __syncthreads();
         if (stop) return;
-        for (u32 i = j + 1 + wave; i < f.k1; i += DW) {
-            const VecView ri{f.w, f.pitch, (i64)i * f.n + f.k0, 1};
-            const Num<N> v = wave_dot_sub<N>(element<N>(f, i, j), ri, rj, j - f.k0, lane, L, sc);
-            if (lane == 0) store_out<N>(f.w, f.pitch, f.n, i, j, false, div<N>(v, element<N>(f, j, j)));
-        }
+        const Num<N> piv = element<N>(f, j, j);
+        for (u32 i = j + 1 + threadIdx.x; i < f.k1; i += blockDim.x)
+            store_out<N>(f.w, f.pitch, f.n, i, j, false, div<N>(element<N>(f, i, j), piv));
         __syncthreads();
     }
 }
