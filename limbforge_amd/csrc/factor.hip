@@ -65,6 +65,7 @@ typedef hipError_t (*diag_fn)(const lf::FactorArgs&, uint32_t, hipStream_t);
 typedef hipError_t (*panel_fn)(const lf::FactorArgs&, hipStream_t);
 typedef hipError_t (*invalid_fn)(uint32_t*, uint64_t, uint32_t, uint32_t, hipStream_t);
 typedef hipError_t (*solve_fn)(int, const lf::SolveArgs&, hipStream_t);
+typedef hipError_t (*recip_fn)(const uint32_t*, uint64_t, uint32_t, uint32_t*, uint64_t, hipStream_t);
 struct FactorTables;
 template <int N> void fill_factor(FactorTables& t);
 struct FactorTables {
@@ -73,6 +74,7 @@ struct FactorTables {
     invalid_fn invalid[33] = {};
     hipError_t (*zero_upper[33])(uint32_t*, uint64_t, uint32_t, hipStream_t) = {};
     solve_fn solve[33] = {};
+    recip_fn reciprocals[33] = {};
     FactorTables() { fill_factor<2>(*this); }
 };
 template <int N> void fill_factor(FactorTables& t)
@@ -82,6 +84,7 @@ template <int N> void fill_factor(FactorTables& t)
     t.invalid[N] = lf::launch_chol_invalid<N>;
     t.zero_upper[N] = lf::launch_zero_upper<N>;
     t.solve[N] = lf::launch_solve<N>;
+    t.reciprocals[N] = lf::launch_reciprocals<N>;
     if constexpr (N < 32) fill_factor<N + 1>(t);
 }
 const FactorTables& tables()
@@ -222,6 +225,10 @@ int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, 
         return fail(LF_INVALID_ARGUMENT, "cholesky: the block is too large for the exact-dot scratch (use a smaller block)");
     FACTOR_HIP(dinfo.alloc(sizeof(lf::FactorInfo)), "hipMalloc (pivot)");
     FACTOR_HIP(dscratch.alloc(2 * swords * threads * 4), "hipMalloc (scratch)");
+    // reciprocals of the pivots (N+1 planes of n entries), for the divisions of the rows below each block
+    DevBuf drecip;
+    const uint64_t rpitch = (n + 63) / 64 * 64;
+    FACTOR_HIP(drecip.alloc((N + 1) * rpitch * 4), "hipMalloc (reciprocals)");
     lf::FactorInfo hinfo{-1, 0, 0, 0};
     FACTOR_HIP(hipMemcpyAsync(dinfo.p, &hinfo, sizeof hinfo, hipMemcpyHostToDevice, s), "upload pivot");
     hipEvent_t e0, e1;
@@ -239,7 +246,8 @@ int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, 
     for (uint64_t k0 = 0; k0 < n;) {
         const uint64_t k1 = std::min(n, k0 + nb), w = k1 - k0;
         auto t = Clock::now();
-        lf::FactorArgs f{W, pitch, n32, uint32_t(k0), uint32_t(k1), (lf::FactorInfo*)dinfo.p, (uint32_t*)dscratch.p, swords};
+        lf::FactorArgs f{W, pitch, n32, uint32_t(k0), uint32_t(k1), (lf::FactorInfo*)dinfo.p, (uint32_t*)dscratch.p, swords,
+                         (uint32_t*)drecip.p, rpitch};
         FACTOR_HIP(hipEventRecord(e0, s), "hipEventRecord");
         FACTOR_HIP(tables().diag[N](f, uint32_t(std::min<uint64_t>(256, (w + 63) / 64 * 64)), s), "panel launch");
         FACTOR_HIP(tables().panel[N](f, s), "panel launch");
@@ -305,6 +313,11 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
     FACTOR_HIP(dscratch.alloc(2 * swords * nrhs * 4), "hipMalloc (scratch)");
     FACTOR_HIP(derr.alloc(4), "hipMalloc");
     FACTOR_HIP(hipMemsetAsync(derr.p, 0, 4, s), "hipMemsetAsync");
+    // reciprocals of the n diagonal entries, for every division of both passes
+    DevBuf drecip;
+    const uint64_t rpitch = (n + 63) / 64 * 64;
+    FACTOR_HIP(drecip.alloc((N + 1) * rpitch * 4), "hipMalloc (reciprocals)");
+    FACTOR_HIP(tables().reciprocals[N](F, fpitch, uint32_t(n), (uint32_t*)drecip.p, rpitch, s), "reciprocals launch");
     double gpu = 0;
     RegionFields lf_, rf_;
     hipEvent_t e0, e1;
@@ -321,7 +334,7 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
     // the in-block kernel between two events; its time is collected after the stream's next synchronization
     auto block_solve = [&](int backward, uint64_t r0, uint64_t r1) {
         lf::SolveArgs a{F, fpitch, Xp, xpitch, uint32_t(n), uint32_t(nrhs), uint32_t(r0), uint32_t(r1), (uint32_t*)dscratch.p, swords,
-                        (int32_t*)derr.p};
+                        (int32_t*)derr.p, (const uint32_t*)drecip.p, rpitch};
         hipError_t e = hipEventRecord(e0, s);
         if (e == hipSuccess) e = tables().solve[N](backward, a, s);
         if (e == hipSuccess) e = hipEventRecord(e1, s);

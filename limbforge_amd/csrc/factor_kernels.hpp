@@ -189,9 +189,12 @@ __device__ __noinline__ Num<N> dot_sub_clusters(const Num<N>& c, const VecView& 
 constexpr int WAVE = 64;
 constexpr int DOT_ROWS = 32; // terms per chunk
 
-template <int N> struct WaveLds {
-    u32 row[DOT_ROWS + 1][factor_window(N) + 1]; // row DOT_ROWS holds c; odd row stride (W is even):
-                                                 // the lanes writing one column of their rows hit distinct banks
+// Row stride of the exact-dot rows: the W-word window plus one word, odd (W is even), so that the lanes
+// storing one column of their rows hit distinct banks.
+template <int N> constexpr int dot_row_stride() { return factor_window(N) + 1; }
+template <int N> struct alignas(16) WaveLds {
+    u32 row[DOT_ROWS + 1][dot_row_stride<N>()]; // row DOT_ROWS holds c
+    u32 neg[factor_window(N) + 1];              // per word: negative terms whose sign fill starts there
 };
 
 __device__ __forceinline__ void wave_sync()
@@ -232,12 +235,17 @@ __device__ __forceinline__ int wave_or(int v)
     return v;
 }
 
-// row r = sg * src (S words) * 2^rel as a W-word two's-complement window (sg in {-1, 0, 1}; rel >= 0 and the
-// term fits: rel + 32S <= 32W - 1)
+// Row r = sg * src (S words) * 2^rel in the (cleared) rows of a wave, as the S+1 words of the shifted term
+// at word offset o = rel / 32, two's complement within those words for a negative term; the rest of a
+// negative term's W-word two's complement (ones from word o+S+1 up) is left out and counted in neg[o+S+1]
+// instead, for the column sums to take off (sg in {-1, 0, 1}; rel >= 0 and the term fits: rel + 32S <= 32W - 1).
 template <int N, int S> __device__ __forceinline__ void put_row(WaveLds<N>& L, int r, const u32 (&src)[S], i64 rel, int sg)
 {
     constexpr int W = factor_window(N);
-    const int o = (int)(rel >> 5);
+    static_assert(S + 1 <= W, "a term fits the window");
+    // a term that fits has o + S <= W - 1 (its top word, q[S], is then inside the row); the clamp only
+    // keeps the stores inside the row should that ever not hold
+    const int o = min((int)(rel >> 5), W - 1 - S);
     const u32 b = (u32)(rel & 31);
     u32 q[S + 1];
 #pragma unroll
@@ -245,8 +253,8 @@ template <int N, int S> __device__ __forceinline__ void put_row(WaveLds<N>& L, i
         const u32 hi = t < S ? src[t] : 0u, lo = t > 0 ? src[t - 1] : 0u;
         q[t] = sg ? (b ? ((hi << b) | (lo >> (32u - b))) : hi) : 0u;
     }
-    bool nonzero = false;
-    if (sg < 0) { // two's complement of q within S+1 words; words above are all ones when q != 0
+    if (sg < 0) {
+        bool nonzero = false;
         u32 c = 1;
 #pragma unroll
         for (int t = 0; t <= S; ++t) {
@@ -255,13 +263,22 @@ template <int N, int S> __device__ __forceinline__ void put_row(WaveLds<N>& L, i
             q[t] = (u32)v;
             c = (u32)(v >> 32);
         }
+        if (nonzero && o + S + 1 < W) atomicAdd(&L.neg[o + S + 1], 1u);
     }
-    const u32 fill = nonzero ? 0xffffffffu : 0u;
+    u32* row = &L.row[r][o];
 #pragma unroll
-    for (int w = 0; w < W; ++w) L.row[r][w] = w < o ? 0u : fill;
-#pragma unroll
-    for (int t = 0; t <= S; ++t)
-        if (o + t < W) L.row[r][o + t] = q[t];
+    for (int t = 0; t <= S; ++t) row[t] = q[t];
+}
+
+// Clear the rows and the sign-fill counts of a wave's exact-dot LDS (all lanes, 16-byte stores).
+template <int N> __device__ __forceinline__ void clear_rows(WaveLds<N>& L, u32 lane)
+{
+    static_assert(sizeof(WaveLds<N>) % 4 == 0, "whole words");
+    constexpr u32 words = sizeof(WaveLds<N>) / 4, quads = words / 4;
+    uint4* q = reinterpret_cast<uint4*>(&L);
+    for (u32 k = lane; k < quads; k += WAVE) q[k] = make_uint4(0u, 0u, 0u, 0u);
+    u32* w = reinterpret_cast<u32*>(&L);
+    for (u32 k = 4 * quads + lane; k < words; k += WAVE) w[k] = 0u;
 }
 
 // Carries of one 64-word group of a multiword sum, all lanes at once: lane l's word has a carry-out g
@@ -278,13 +295,15 @@ __device__ __forceinline__ u32 wave_carry_in(bool g, bool p, u32 lane, u64& cin)
 }
 
 // The rounded value of a W-word two's-complement window whose column w (w = lane + 64t, t < CPL) holds
-// the 64-bit sum S[t] of that word over the rows: RN-even of the window's value * 2^scale, in every
+// the 64-bit sum S[t] of that word over the rows, less H[t] * 2^(32w) for the sign fills left out of the
+// rows (ones from word w up, mod 2^(32W): -2^(32w) each): RN-even of the window's value * 2^scale, in every
 // lane.  Lane-parallel: the column sums are split into words plus a carry word into the next column,
 // single-bit carries are resolved across lanes by wave_carry_in, a negative total is negated the same
 // way, and the top word, the N mantissa words, the round bit and the sticky bit are found with ballots
 // and lane reads (the serial form is pack() of the resolved window).
 template <int N>
-__device__ __forceinline__ Num<N> wave_round(const u64 (&S)[(factor_window(N) + WAVE - 1) / WAVE], i64 scale, u32 lane)
+__device__ __forceinline__ Num<N> wave_round(const u64 (&S)[(factor_window(N) + WAVE - 1) / WAVE], const u32 (&H)[(factor_window(N) + WAVE - 1) / WAVE],
+                                          i64 scale, u32 lane)
 {
     constexpr int W = factor_window(N);
     constexpr int CPL = (W + WAVE - 1) / WAVE;
@@ -303,6 +322,16 @@ __device__ __forceinline__ Num<N> wave_round(const u64 (&S)[(factor_window(N) + 
             const u32 u = in ? (u32)ts : 0u;
             const u32 c = wave_carry_in(in && (ts >> 32) != 0, in && u == 0xffffffffu, lane, cin);
             word[t] = in ? u + c : 0u;
+        }
+    }
+    {   // minus the sign fills: H[t] ones-words start at column w (a borrow lookahead like the carries)
+        u64 bin = 0;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const bool in = (int)lane + WAVE * t < W;
+            const u32 d = word[t] - H[t];
+            const u32 bb = wave_carry_in(in && word[t] < H[t], in && d == 0u, lane, bin);
+            word[t] = in ? d - bb : 0u;
         }
     }
     constexpr int TL = (W - 1) % WAVE, TG = (W - 1) / WAVE;
@@ -419,10 +448,16 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
     }
     // pass 2: rows of exact products, column sums
     u64 S[CPL];
+    u32 H[CPL];
 #pragma unroll
-    for (int t = 0; t < CPL; ++t) S[t] = 0;
+    for (int t = 0; t < CPL; ++t) {
+        S[t] = 0;
+        H[t] = 0;
+    }
     for (u32 kb = 0; kb < K; kb += DOT_ROWS) {
         const u32 rows = K - kb < (u32)DOT_ROWS ? K - kb : (u32)DOT_ROWS;
+        clear_rows<N>(L, lane);
+        wave_sync();
         if (lane < rows) {
             const u32 k = kb + lane;
             const Num<N> a = one_chunk ? ta : vec_load<N>(x, k), b = one_chunk ? tb : vec_load<N>(y, k);
@@ -441,11 +476,12 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
 #pragma unroll 8
                 for (u32 r = 0; r < rows; ++r) sum += L.row[r][w];
                 S[t] += sum;
+                H[t] += L.neg[w];
             }
         }
         wave_sync();
     }
-    const Num<N> r = wave_round<N>(S, lo, lane);
+    const Num<N> r = wave_round<N>(S, H, lo, lane);
     wave_sync(); // the rows and columns are reused by the wave's next dot
     return r;
 }
@@ -454,6 +490,23 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
 template <int N> __device__ __forceinline__ Num<N> element(const FactorArgs& f, u32 i, u32 j)
 {
     return load_out<N>(f.w, f.pitch, (u64)i * f.n + j);
+}
+
+// The reciprocal (limb_core.hpp reciprocal) of the pivot or diagonal entry i, N+1 planes.
+template <int N> __device__ __forceinline__ void load_recip(const u32* r, u64 rpitch, u32 i, u32 (&v)[N + 1])
+{
+#pragma unroll
+    for (int k = 0; k <= N; ++k) v[k] = r[(u64)k * rpitch + i];
+}
+template <int N> __device__ __forceinline__ void store_recip(u32* r, u64 rpitch, u32 i, const Num<N>& d)
+{
+    u32 v[N + 1];
+    if (d.s && !d.st) reciprocal<N>(d.w, v);
+    else
+#pragma unroll
+        for (int k = 0; k <= N; ++k) v[k] = 0; // never used: division by zero or a status is decided first
+#pragma unroll
+    for (int k = 0; k <= N; ++k) r[(u64)k * rpitch + i] = v[k];
 }
 
 // Waves of the diagonal-block workgroup: 12 up to 12 limbs (three waves per SIMD leave 170 VGPRs a
@@ -506,7 +559,8 @@ template <int N> __global__ void __launch_bounds__(diag_waves<N>() * WAVE) chol_
         __syncthreads();
         if (stop) return;
         const Num<N> piv = element<N>(f, j, j);
-        for (u32 i = j + 1 + threadIdx.x; i < f.k1; i += blockDim.x)
+        if (threadIdx.x == blockDim.x - 1) store_recip<N>(f.recip, f.rpitch, j, piv); // for the rows below the block
+        for (u32 i = j + 1 + threadIdx.x; i < f.k1; i += blockDim.x - 1)
             store_out<N>(f.w, f.pitch, f.n, i, j, false, div<N>(element<N>(f, i, j), piv));
         __syncthreads();
     }
@@ -526,7 +580,11 @@ template <int N> __global__ void __launch_bounds__(64) chol_panel_kernel(FactorA
     for (u32 j = f.k0; j < pe; ++j) {
         const VecView rj{f.w, f.pitch, (i64)j * f.n + f.k0, 1};
         const Num<N> v = wave_dot_sub<N>(element<N>(f, i, j), ri, rj, j - f.k0, lane, L, sc);
-        if (lane == 0) store_out<N>(f.w, f.pitch, f.n, i, j, false, div<N>(v, element<N>(f, j, j)));
+        if (lane == 0) {
+            u32 rv[N + 1];
+            load_recip<N>(f.recip, f.rpitch, j, rv);
+            store_out<N>(f.w, f.pitch, f.n, i, j, false, div_by_reciprocal<N>(v, element<N>(f, j, j), rv));
+        }
         __syncthreads(); // entry (i, j) is a term of the row's next dot
     }
 }
@@ -561,7 +619,11 @@ template <int N> __global__ void __launch_bounds__(64) solve_forward_kernel(Solv
     for (u32 i = a.r0; i < a.r1; ++i) {
         const VecView li{a.f, a.fpitch, (i64)i * a.n + a.r0, 1};
         const Num<N> v = wave_dot_sub<N>(load_out<N>(a.x, a.xpitch, (u64)i * a.nrhs + c), li, xc, i - a.r0, lane, L, sc);
-        if (lane == 0) store_out<N>(a.x, a.xpitch, a.nrhs, i, c, false, div<N>(v, load_out<N>(a.f, a.fpitch, (u64)i * a.n + i)));
+        if (lane == 0) {
+            u32 rv[N + 1];
+            load_recip<N>(a.recip, a.rpitch, i, rv);
+            store_out<N>(a.x, a.xpitch, a.nrhs, i, c, false, div_by_reciprocal<N>(v, load_out<N>(a.f, a.fpitch, (u64)i * a.n + i), rv));
+        }
         __syncthreads(); // x_i is a term of the next row's dot
     }
 }
@@ -579,9 +641,21 @@ template <int N> __global__ void __launch_bounds__(64) solve_backward_kernel(Sol
             const VecView xc{a.x, a.xpitch, (i64)(i + 1) * a.nrhs + c, (i64)a.nrhs};
             v = wave_dot_sub<N>(v, col, xc, a.r1 - i - 1, lane, L, sc);
         }
-        if (lane == 0) store_out<N>(a.x, a.xpitch, a.nrhs, i, c, false, div<N>(v, load_out<N>(a.f, a.fpitch, (u64)i * a.n + i)));
+        if (lane == 0) {
+            u32 rv[N + 1];
+            load_recip<N>(a.recip, a.rpitch, i, rv);
+            store_out<N>(a.x, a.xpitch, a.nrhs, i, c, false, div_by_reciprocal<N>(v, load_out<N>(a.f, a.fpitch, (u64)i * a.n + i), rv));
+        }
         __syncthreads();
     }
+}
+
+// Reciprocals of the n diagonal entries of the factor, one thread each (for the solves' divisions).
+template <int N> __global__ void __launch_bounds__(64) reciprocals_kernel(const u32* w, u64 pitch, u32 n, u32* recip, u64 rpitch)
+{
+    const u32 i = blockIdx.x * 64u + threadIdx.x;
+    if (i >= n) return;
+    store_recip<N>(recip, rpitch, i, load_out<N>(w, pitch, (u64)i * n + i));
 }
 
 } // namespace lf

@@ -20,6 +20,8 @@ struct FactorArgs {
     FactorInfo* info;
     uint32_t* scratch; // per thread: 2 * swords words
     uint64_t swords;
+    uint32_t* recip;   // reciprocals of the pivots l_jj (N+1 planes, entry j), written by the diagonal kernel
+    uint64_t rpitch;
 };
 
 struct SolveArgs {
@@ -31,6 +33,8 @@ struct SolveArgs {
     uint32_t* scratch;
     uint64_t swords;
     int32_t* error;
+    const uint32_t* recip; // reciprocals of the diagonal l_ii (N+1 planes, entry i), from launch_reciprocals
+    uint64_t rpitch;
 };
 
 // Window words of the register path of the exact dot: 2N product words plus 24 words (768 bits) of
@@ -52,5 +56,7 @@ template <int N> hipError_t launch_chol_panel(const FactorArgs& f, hipStream_t s
 template <int N> hipError_t launch_chol_invalid(uint32_t* w, uint64_t pitch, uint32_t n, uint32_t p, hipStream_t s);
 template <int N> hipError_t launch_zero_upper(uint32_t* w, uint64_t pitch, uint32_t n, hipStream_t s);
 template <int N> hipError_t launch_solve(int backward, const SolveArgs& a, hipStream_t s);
+// recip entry i = reciprocal of the mantissa of the diagonal element (i, i) of the n x n planes, i < n
+template <int N> hipError_t launch_reciprocals(const uint32_t* w, uint64_t pitch, uint32_t n, uint32_t* recip, uint64_t rpitch, hipStream_t s);
 
 } // namespace lf

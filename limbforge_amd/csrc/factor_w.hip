@@ -48,10 +48,18 @@ template <int N> hipError_t launch_solve(int backward, const SolveArgs& a, hipSt
     return hipGetLastError();
 }
 
+template <int N> hipError_t launch_reciprocals(const uint32_t* w, uint64_t pitch, uint32_t n, uint32_t* recip, uint64_t rpitch, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL((reciprocals_kernel<N>), dim3((n + 63) / 64), dim3(64), 0, s, w, pitch, n, recip, rpitch);
+    return hipGetLastError();
+}
+
 template hipError_t launch_chol_diag<LF_WORDS>(const FactorArgs&, uint32_t, hipStream_t);
 template hipError_t launch_chol_panel<LF_WORDS>(const FactorArgs&, hipStream_t);
 template hipError_t launch_chol_invalid<LF_WORDS>(uint32_t*, uint64_t, uint32_t, uint32_t, hipStream_t);
 template hipError_t launch_zero_upper<LF_WORDS>(uint32_t*, uint64_t, uint32_t, hipStream_t);
 template hipError_t launch_solve<LF_WORDS>(int, const SolveArgs&, hipStream_t);
+template hipError_t launch_reciprocals<LF_WORDS>(const uint32_t*, uint64_t, uint32_t, uint32_t*, uint64_t, hipStream_t);
 
 } // namespace lf

@@ -507,6 +507,87 @@ __device__ __forceinline__ void divide_word(u32 high, u32 low, u32 d, u32 inv, u
     r = rr;
 }
 
+// One quotient word of Knuth's algorithm D (TAOCP vol. 2, 4.3.1): the partial remainder R (N+1 words,
+// R < B before the step) takes the next dividend word, the quotient word is estimated from the top two
+// remainder words and B's top word (with the reused reciprocal inv of divide_word), corrected with B's
+// second word, multiplied out and subtracted (added back once when the estimate was still one too high).
+template <int N> __device__ __forceinline__ u32 knuth_step(u32 (&R)[N + 1], const u32 (&b)[N], u32 next, u32 inv)
+{
+    const u32 d = b[N - 1], dn = b[N - 2];
+#pragma unroll
+    for (int i = N; i > 0; --i) R[i] = R[i - 1];
+    R[0] = next;
+    u64 q, rem;
+    if (R[N] >= d) {
+        q = 0xffffffffull;
+        rem = ((((u64)R[N]) << 32) | R[N - 1]) - q * d;
+    } else {
+        u32 qq, rr;
+        divide_word(R[N], R[N - 1], d, inv, qq, rr);
+        q = qq;
+        rem = rr;
+    }
+    while (rem < ((u64)1 << 32) && q * dn > ((rem << 32) | R[N - 2])) {
+        --q;
+        rem += d;
+    }
+    u64 borrow = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u64 p = q * b[i] + borrow;
+        u32 lo = (u32)p, old = R[i];
+        R[i] = old - lo;
+        borrow = (p >> 32) + (old < lo ? 1u : 0u);
+    }
+    const bool under = (u64)R[N] < borrow;
+    R[N] = (u32)((u64)R[N] - borrow);
+    if (under) {
+        --q;
+        u32 c = 0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            u64 t = (u64)R[i] + b[i] + c;
+            R[i] = (u32)t;
+            c = (u32)(t >> 32);
+        }
+        R[N] += c;
+    }
+    return (u32)q;
+}
+
+// The rounding of a quotient: Q (N words, top bit set) with remainder R < B (R[N] = 0); 2R against B
+// decides, ties to even; e is the exponent of Q's top bit.
+template <int N>
+__device__ __forceinline__ Num<N> round_quotient(const u32 (&Q)[N], const u32 (&R)[N + 1], const u32 (&b)[N], i64 e, int sign)
+{
+    int half;
+    if (R[N - 1] >> 31) half = 1;
+    else {
+        u32 tw[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) tw[i] = __builtin_amdgcn_alignbit(R[i], i ? R[i - 1] : 0u, 31u);
+        half = compare_words<N>(tw, b);
+    }
+    const bool inc = half > 0 || (half == 0 && (Q[0] & 1u));
+    Num<N> r;
+    r.s = sign;
+    r.st = ST_OK;
+    u32 c = inc ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u64 t = (u64)Q[i] + c;
+        r.w[i] = (u32)t;
+        c = (u32)(t >> 32);
+    }
+    if (c) {
+        r.w[N - 1] = 0x80000000u;
+        e += 1;
+    }
+    if (e > EMAX || e < -(i64)EMAX) return zero<N>(ST_OVERFLOW);
+    r.e = (int)e;
+    return r;
+}
+
 // RN(a / b): normalized base-2^32 long division (Knuth D), then the remainder decides the rounding
 // (contract of core.hpp:185-215).  The partial remainder R is an (N+1)-word shift register: each step
 // brings in the next dividend word, so no array is indexed at run time.
@@ -531,80 +612,144 @@ template <int N> __device__ __forceinline__ Num<N> div(const Num<N>& a, const Nu
     R[N] = 0;
     u32 next = lt ? 0u : (a.w[0] << 31);
     const u32 d = b.w[N - 1];
-    const u32 dn = b.w[N - 2];
     const u32 inv = (u32)(~(u64)0 / d - ((u64)1 << 32));
 #pragma unroll 2
     for (int j = N - 1; j >= 0; --j) {
-#pragma unroll
-        for (int i = N; i > 0; --i) R[i] = R[i - 1];
-        R[0] = next;
+        const u32 q = knuth_step<N>(R, b.w, next, inv);
         next = 0;
-        u64 q, rem;
-        if (R[N] >= d) {
-            q = 0xffffffffull;
-            rem = ((((u64)R[N]) << 32) | R[N - 1]) - q * d;
-        } else {
-            u32 qq, rr;
-            divide_word(R[N], R[N - 1], d, inv, qq, rr);
-            q = qq;
-            rem = rr;
-        }
-        while (rem < ((u64)1 << 32) && q * dn > ((rem << 32) | R[N - 2])) {
-            --q;
-            rem += d;
-        }
-        u64 borrow = 0;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            u64 p = q * b.w[i] + borrow;
-            u32 lo = (u32)p, old = R[i];
-            R[i] = old - lo;
-            borrow = (p >> 32) + (old < lo ? 1u : 0u);
-        }
-        const bool under = (u64)R[N] < borrow;
-        R[N] = (u32)((u64)R[N] - borrow);
-        if (under) {
-            --q;
-            u32 c = 0;
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-                u64 t = (u64)R[i] + b.w[i] + c;
-                R[i] = (u32)t;
-                c = (u32)(t >> 32);
-            }
-            R[N] += c;
-        }
 #pragma unroll
         for (int i = N - 1; i > 0; --i) Q[i] = Q[i - 1];
-        Q[0] = (u32)q;
+        Q[0] = q;
     }
-    // round: compare 2R (R < B, N words) with B; ties go to even
-    int half;
-    if (R[N - 1] >> 31) half = 1;
-    else {
-        u32 tw[N];
+    return round_quotient<N>(Q, R, b.w, e, a.s * b.s);
+}
+
+// ---------------------------------------------------- division by a reused divisor
+// V = floor((2^(64N) - 1) / B) for a normalized mantissa B (N+1 words, V[N] = 1): Knuth D on the
+// all-ones 2N-word dividend, whose top N words exceed B by less than B (first quotient word 1).
+template <int N> __device__ __forceinline__ void reciprocal(const u32 (&b)[N], u32 (&v)[N + 1])
+{
+    u32 R[N + 1];
 #pragma unroll
-        for (int i = 0; i < N; ++i) tw[i] = __builtin_amdgcn_alignbit(R[i], i ? R[i - 1] : 0u, 31u);
-        half = compare_words<N>(tw, b.w);
-    }
-    const bool inc = half > 0 || (half == 0 && (Q[0] & 1u));
-    Num<N> r;
-    r.s = a.s * b.s;
-    r.st = ST_OK;
-    u32 c = inc ? 1u : 0u;
+    for (int i = 0; i < N; ++i) R[i] = ~b[i]; // 2^(32N) - 1 - B
+    R[N] = 0;
+    v[N] = 1;
+    const u32 inv = (u32)(~(u64)0 / b[N - 1] - ((u64)1 << 32));
+#pragma unroll 2
+    for (int j = N - 1; j >= 0; --j) {
+        const u32 q = knuth_step<N>(R, b, 0xffffffffu, inv);
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        u64 t = (u64)Q[i] + c;
-        r.w[i] = (u32)t;
-        c = (u32)(t >> 32);
+        for (int i = 0; i < N; ++i)
+            if (i == j) v[i] = q;
     }
-    if (c) {
-        r.w[N - 1] = 0x80000000u;
-        e += 1;
+}
+
+// The low P words of the product of two digit strings (29-bit digits, DA and DB of them): columns below
+// ceil(32P / 29) only, each an in-place v_mad_u64_u32 sum (at most 36 products below 2^58), one carry pass.
+template <int DA, int DB, int P>
+__device__ __forceinline__ void digit_product_low(const u32 (&da)[DA], const u32 (&db)[DB], u32 (&p)[P])
+{
+    constexpr int K = (32 * P + DIGIT_BITS - 1) / DIGIT_BITS;
+    static_assert(DA <= 37 && DB <= 37, "column sums of 29-bit digit products must fit in 64 bits"); // 37 * 2^58 + carry < 2^64
+    u64 col[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) col[k] = 0;
+#pragma unroll
+    for (int i = 0; i < DA; ++i)
+#pragma unroll
+        for (int j = 0; j < DB; ++j)
+            if (i + j < K) col[i + j] += (u64)da[i] * db[j];
+    u32 dg[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        dg[k] = (u32)col[k] & DIGIT_MASK;
+        if (k + 1 < K) col[k + 1] += col[k] >> DIGIT_BITS;
     }
-    if (e > EMAX || e < -(i64)EMAX) return zero<N>(ST_OVERFLOW);
-    r.e = (int)e;
-    return r;
+#pragma unroll
+    for (int w = 0; w < P; ++w) {
+        const int bit = 32 * w, k = bit / DIGIT_BITS, s = bit % DIGIT_BITS;
+        u32 v = dg[k] >> s;
+        if (k + 1 < K) v |= dg[k + 1] << (DIGIT_BITS - s);
+        if (k + 2 < K && 2 * DIGIT_BITS - s < 32) v |= dg[k + 2] << (2 * DIGIT_BITS - s);
+        p[w] = v;
+    }
+}
+
+// 29-bit digits of an M-word string (M need not be a limb count).
+template <int M> constexpr int digits_of_words() { return (32 * M + DIGIT_BITS - 1) / DIGIT_BITS; }
+template <int M> __device__ __forceinline__ void words_to_digits(const u32 (&w)[M], u32 (&d)[digits_of_words<M>()])
+{
+#pragma unroll
+    for (int k = 0; k < digits_of_words<M>(); ++k) {
+        const int bit = DIGIT_BITS * k, wi = bit / 32, s = bit % 32;
+        const u32 lo = w[wi] >> s;
+        const u32 hi = (s && wi + 1 < M) ? (w[wi + 1] << (32 - s)) : 0u;
+        d[k] = (lo | hi) & DIGIT_MASK;
+    }
+}
+
+// RN(a / b), bit for bit the result of div, given v = reciprocal of b's mantissa.  With U = A *
+// 2^(32N-1+lt) and Q = floor(U / B) as in div: V is within one unit below 2^(64N) / B and U < 2^(64N),
+// so the estimate floor(U * V / 2^(64N)) = floor(A * V / 2^(32N+1-lt)) is Q or Q - 1; the remainder
+// U - estimate * B lies in [0, 2B), below 2^(32N+1), so its low N+1 words (computed modulo 2^(32(N+1)))
+// decide which, and 2R against B rounds as in div.  Two products with independent multiply-adds in
+// place of N dependent quotient steps: the short-latency form for divisors used many times.
+template <int N> __device__ __forceinline__ Num<N> div_by_reciprocal(const Num<N>& a, const Num<N>& b, const u32 (&v)[N + 1])
+{
+    const u32 st = a.st | b.st;
+    if (st) return zero<N>(st);
+    if (!b.s) return zero<N>(ST_DIV0);
+    if (!a.s) return zero<N>(ST_OK);
+    const bool lt = compare_words<N>(a.w, b.w) < 0;
+    i64 e = (i64)a.e - b.e - (lt ? 1 : 0);
+    if (e > EMAX || e < -(i64)EMAX - 1) return zero<N>(ST_OVERFLOW);
+    constexpr int DA = digits_of_words<N>(), DV = digits_of_words<N + 1>();
+    u32 da[DA], dv[DV], p[2 * N + 1];
+    words_to_digits<N>(a.w, da);
+    words_to_digits<N + 1>(v, dv);
+    digit_product_low<DA, DV, 2 * N + 1>(da, dv, p);
+    u32 Q[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) Q[k] = lt ? p[N + k] : __builtin_amdgcn_alignbit(p[N + k + 1], p[N + k], 1u);
+    // low N+1 words of Q * B and of U
+    u32 dq[DA], db[DA], qb[N + 1], R[N + 1];
+    words_to_digits<N>(Q, dq);
+    words_to_digits<N>(b.w, db);
+    digit_product_low<DA, DA, N + 1>(dq, db, qb);
+#pragma unroll
+    for (int k = 0; k <= N; ++k) {
+        u32 uk = 0;
+        if (k == N) uk = lt ? a.w[0] : __builtin_amdgcn_alignbit(a.w[1], a.w[0], 1u);
+        else if (k == N - 1) uk = lt ? 0u : (a.w[0] << 31);
+        R[k] = uk;
+    }
+    u32 bw = 0;
+#pragma unroll
+    for (int k = 0; k <= N; ++k) {
+        const u64 t = (u64)R[k] - qb[k] - bw;
+        R[k] = (u32)t;
+        bw = (u32)(t >> 63);
+    }
+    // R in [0, 2B): one more unit of the quotient when R >= B
+    bool ge = R[N] != 0;
+    if (!ge) ge = compare_words<N>(*reinterpret_cast<const u32(*)[N]>(R), b.w) >= 0;
+    if (ge) {
+        u32 c = 0;
+#pragma unroll
+        for (int k = 0; k <= N; ++k) {
+            const u64 t = (u64)R[k] - (k < N ? b.w[k] : 0u) - c;
+            R[k] = (u32)t;
+            c = (u32)(t >> 63);
+        }
+        c = 1;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const u64 t = (u64)Q[k] + c;
+            Q[k] = (u32)t;
+            c = (u32)(t >> 32);
+        }
+    }
+    return round_quotient<N>(Q, R, b.w, e, a.s * b.s);
 }
 
 // ------------------------------------------------------------------- sqrt
@@ -696,46 +841,69 @@ template <int N> __device__ __noinline__ Num<N> sqrt_restoring(const Num<N>& a)
     return r;
 }
 
-template <int N, int M> __device__ __forceinline__ Num<N> cast(const Num<M>& x);
-
-// y ~ sqrt(v) for v > 0, accurate to about N words: Newton's step y <- (y + v/y) / 2 at N words on top of
-// the result for v rounded to N/2 + 1 words (so each level doubles the precision at a quarter of the cost
-// of a full-width step), down to two words seeded from the double-precision square root of v's top 64 bits.
-template <int N> __device__ Num<N> sqrt_newton(const Num<N>& v)
+// Bits [P, P + 32K) of a W-word string, P a compile-time bit position (words past the end read as 0).
+template <int K, int P, int W> __device__ __forceinline__ void bits_at(const u32 (&a)[W], u32 (&o)[K])
 {
-    Num<N> y;
-    if constexpr (N <= 2) {
-        const int ev = v.e >= 0 ? v.e / 2 : -((1 - v.e) / 2); // floor(e / 2)
-        const int odd = v.e - 2 * ev;
-        const u64 top = ((u64)v.w[N - 1] << 32) | (N > 1 ? v.w[N - 2] : 0u);
-        const double u = (double)top * (odd ? 1.0 : 0.5);            // in [2^62, 2^64)
-        const u64 s0 = (u64)(__builtin_sqrt(u) * 2147483648.0) << 1; // sqrt(u) * 2^32, in [2^63, 2^64)
+    constexpr int q = P / 32, b = P % 32;
 #pragma unroll
-        for (int i = 0; i < N; ++i) y.w[i] = i == N - 1 ? (u32)(s0 >> 32) : (i == N - 2 ? (u32)s0 : 0u);
-        y.e = ev;
-        y.s = 1;
-        y.st = ST_OK;
-#pragma unroll
-        for (int it = 0; it < 2; ++it) {
-            y = add<N>(y, div<N>(v, y));
-            y.e -= 1;
-        }
-    } else {
-        constexpr int H = N / 2 + 1;
-        y = cast<N, H>(sqrt_newton<H>(cast<H, N>(v)));
-        y = add<N>(y, div<N>(v, y));
-        y.e -= 1;
+    for (int k = 0; k < K; ++k) {
+        const u32 lo = q + k < W ? a[q + k] : 0u, hi = q + k + 1 < W ? a[q + k + 1] : 0u;
+        o[k] = b ? __builtin_amdgcn_alignbit(hi, lo, (u32)b) : lo;
     }
-    return y;
+}
+
+// Z ~ 2^F / sqrt(x), F = 32M - 3, for x = (top M words of the W-word X) / 2^(32M) in [1/4, 1): Newton's
+// iteration for the reciprocal square root, z <- z (3 - x z^2) / 2, in M-word fixed point.  Every term
+// stays non-negative (x z^2 is near 1, so 3 - x z^2 is near 2), so each step is three integer products
+// and a subtraction from a constant, with no normalization shifts; the step at M words refines the result
+// at M/2 + 1 words (each step doubles the accurate bits), down to two words seeded from the double
+// reciprocal square root of X's top 64 bits.  Z <= 2^(F+1) * (1 + tiny) fits in M words.
+template <int M, int W> __device__ void rsqrt_fixed(const u32 (&X)[W], u32 (&Z)[M])
+{
+    constexpr int F = 32 * M - 3, DM = digits_of_words<M>();
+    if constexpr (M <= 2) {
+        static_assert(M == 2, "the seed fills two words");
+        const u64 top = ((u64)X[W - 1] << 32) | X[W - 2];
+        const double z0 = 1.0 / __builtin_sqrt((double)top * 0x1p-64); // in (1, 2]
+        const u64 z = (u64)(z0 * 0x1p61);
+        Z[0] = (u32)z;
+        Z[1] = (u32)(z >> 32);
+    } else {
+        constexpr int MP = M / 2 + 1;
+        u32 zp[MP];
+        rsqrt_fixed<MP, W>(X, zp);
+#pragma unroll
+        for (int i = 0; i < M; ++i) Z[i] = i >= M - MP ? zp[i - (M - MP)] : 0u; // 2^(F - F') = 2^(32(M - MP))
+    }
+    u32 dz[DM], dx[DM], da[DM], dd[DM], A[2 * M], Ap[M], Xm[M], B[2 * M], D[M], P[2 * M];
+    words_to_digits<M>(Z, dz);
+    digit_product_low<DM, DM, 2 * M>(dz, dz, A);
+    bits_at<M, F>(A, Ap); // z^2 * 2^F
+#pragma unroll
+    for (int i = 0; i < M; ++i) Xm[i] = X[W - M + i];
+    words_to_digits<M>(Xm, dx);
+    words_to_digits<M>(Ap, da);
+    digit_product_low<DM, DM, 2 * M>(dx, da, B); // x z^2 * 2^(F + 32M)
+    u32 bw = 0;
+#pragma unroll
+    for (int i = 0; i < M; ++i) { // D = 3 * 2^F - x z^2 * 2^F
+        const u64 t = (u64)(i == M - 1 ? 0x60000000u : 0u) - B[M + i] - bw;
+        D[i] = (u32)t;
+        bw = (u32)(t >> 63);
+    }
+    words_to_digits<M>(D, dd);
+    digit_product_low<DM, DM, 2 * M>(dz, dd, P);
+    bits_at<M, F + 1>(P, Z); // z (3 - x z^2) / 2 * 2^F
 }
 
 // RN(sqrt(a)).  The mantissa m (N words, top bit set) and the exponent parity define the integer
 // X = m * 2^(32N-1+odd) < 2^(64N), whose integer square root r has exactly 32N bits; the result is r, plus
-// one ulp exactly when X - r^2 > r (the contract of core.hpp:216-235: no ties are possible).  r comes from
-// Newton's iteration y <- (y + X/y) / 2 in N-word arithmetic started from a double-precision seed, and is
-// then made exact by integer checks r^2 <= X < (r+1)^2 (a squaring and a compare each), so the result
-// does not depend on how close the iteration gets.  The iteration runs at increasing widths (sqrt_newton).  Should the checks not settle within a few steps, the
-// bit-serial restoring method decides.
+// one ulp exactly when X - r^2 > r (the contract of core.hpp:216-235: no ties are possible).  r starts as
+// x * z from the fixed-point reciprocal square root z of x = X / 2^(64N) at N+1 words (rsqrt_fixed:
+// products only), which lands within a unit or two; one squaring gives X - r^2 and unit steps on r and the
+// remainder (X - (r+1)^2 = X - r^2 - (2r+1)) make r^2 <= X < (r+1)^2 exactly, so the result does not depend
+// on how close the estimate is.  Should that take more than a few steps, the bit-serial restoring method
+// decides.
 template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
 {
     if (a.st || !a.s) return zero<N>(a.st);
@@ -752,21 +920,26 @@ template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
         const u32 hi = i < N ? a.w[i] : 0u, lo = i ? a.w[i - 1] : 0u;
         X[N - 1 + i] = __builtin_amdgcn_alignbit(hi, lo, odd ? 0u : 1u);
     }
-    Num<N> v;
+    // r = x z 2^(32N) = Xt Z / 2^(32 + F), Xt = top N+1 words of X, F = 32(N+1) - 3
+    constexpr int DT = digits_of_words<N + 1>();
+    u32 Z[N + 1], Xt[N + 1], dz[DT], dt[DT], P[2 * N + 2], r[N], over[1];
+    rsqrt_fixed<N + 1, 2 * N>(X, Z);
 #pragma unroll
-    for (int i = 0; i < N; ++i) v.w[i] = a.w[i];
-    v.e = 64 * N - 2 + odd; // value X
-    v.s = 1;
-    v.st = ST_OK;
-    const Num<N> y = sqrt_newton<N>(v);
-    u32 r[N];
+    for (int i = 0; i <= N; ++i) Xt[i] = X[N - 1 + i];
+    words_to_digits<N + 1>(Z, dz);
+    words_to_digits<N + 1>(Xt, dt);
+    digit_product_low<DT, DT, 2 * N + 2>(dt, dz, P);
+    bits_at<N, 32 * N + 61>(P, r);
+    bits_at<1, 64 * N + 61>(P, over);
+    if (over[0]) // estimate at or above 2^(32N): start from the largest 32N-bit value
 #pragma unroll
-    for (int i = 0; i < N; ++i) r[i] = y.e == 32 * N - 1 ? y.w[i] : (y.e > 32 * N - 1 ? 0xffffffffu : (i == N - 1 ? 0x80000000u : 0u));
-    // exact correction: r^2 <= X < (r+1)^2
-    u32 rem[2 * N];
-    bool done = false;
-#pragma unroll 1
-    for (int guard = 0; guard < 6 && !done; ++guard) {
+        for (int i = 0; i < N; ++i) r[i] = 0xffffffffu;
+    else if (!(r[N - 1] >> 31)) // below 2^(32N-1): start from the smallest
+#pragma unroll
+        for (int i = 0; i < N; ++i) r[i] = i == N - 1 ? 0x80000000u : 0u;
+    // rem = X - r^2 as a (2N+1)-word two's-complement value
+    u32 rem[2 * N + 1];
+    {
         u32 sq[2 * N];
         mantissa_product<N>(r, r, sq);
         u32 bw = 0;
@@ -776,7 +949,34 @@ template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
             rem[i] = (u32)t;
             bw = (u32)(t >> 63);
         }
-        if (bw) { // r^2 > X
+        rem[2 * N] = bw ? 0xffffffffu : 0u;
+    }
+    bool done = false;
+#pragma unroll 1
+    for (int step = 0; step < 6 && !done; ++step) {
+        // t2 = 2r + 1 (N+1 words)
+        u32 t2[N + 1];
+#pragma unroll
+        for (int i = 0; i <= N; ++i) {
+            const u32 hi = i < N ? r[i] : 0u, lo = i ? r[i - 1] : 0u;
+            t2[i] = __builtin_amdgcn_alignbit(hi, lo, 31u);
+        }
+        t2[0] |= 1u;
+        if (rem[2 * N] >> 31) { // r^2 > X: r <- r - 1, rem <- rem + 2r - 1 (= rem + t2 - 2)
+            u32 c = 0;
+#pragma unroll
+            for (int i = 0; i <= 2 * N; ++i) {
+                const u64 t = (u64)rem[i] + (i <= N ? t2[i] : 0u) + c;
+                rem[i] = (u32)t;
+                c = (u32)(t >> 32);
+            }
+            u32 bw = 2;
+#pragma unroll
+            for (int i = 0; i <= 2 * N; ++i) {
+                const u64 t = (u64)rem[i] - bw;
+                rem[i] = (u32)t;
+                bw = (u32)(t >> 63);
+            }
             u32 b = 1;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
@@ -786,21 +986,21 @@ template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
             }
             continue;
         }
-        // (r+1)^2 <= X  <=>  X - r^2 >= 2r + 1
-        u32 t2[N + 1];
-#pragma unroll
-        for (int i = 0; i <= N; ++i) {
-            const u32 hi = i < N ? r[i] : 0u, lo = i ? r[i - 1] : 0u;
-            t2[i] = __builtin_amdgcn_alignbit(hi, lo, 31u);
-        }
-        t2[0] |= 1u;
+        // (r+1)^2 <= X  <=>  rem >= 2r + 1: r <- r + 1, rem <- rem - (2r + 1)
         bool high = false;
 #pragma unroll
-        for (int i = N + 1; i < 2 * N; ++i) high = high || rem[i] != 0;
+        for (int i = N + 1; i <= 2 * N; ++i) high = high || rem[i] != 0;
         u32 rl[N + 1];
 #pragma unroll
         for (int i = 0; i <= N; ++i) rl[i] = rem[i];
         if (high || compare_words<N + 1>(rl, t2) >= 0) {
+            u32 bw = 0;
+#pragma unroll
+            for (int i = 0; i <= 2 * N; ++i) {
+                const u64 t = (u64)rem[i] - (i <= N ? t2[i] : 0u) - bw;
+                rem[i] = (u32)t;
+                bw = (u32)(t >> 63);
+            }
             u32 c = 1;
 #pragma unroll
             for (int i = 0; i < N; ++i) {

@@ -96,6 +96,19 @@ template <int N> void run_width(int cases)
             report<N>("mixed chain", i, x, y);
         }
         report<N>("div", i, lf::div<N>(da, db), lfo_div(N, a, b));
+        {   // division by a reused divisor's reciprocal: random, equal, power-of-two and all-ones mantissas
+            lfo_num t = b, u = a;
+            if (i % 6 == 1) for (int w = 0; w < N; ++w) u.limb[w] = t.limb[w];
+            if (i % 6 == 2) { for (int w = 0; w < N; ++w) t.limb[w] = 0; t.limb[N - 1] = 0x80000000u; }
+            if (i % 6 == 3) for (int w = 0; w < N; ++w) t.limb[w] = 0xffffffffu;
+            if (i % 6 == 4) { for (int w = 0; w < N; ++w) u.limb[w] = 0xffffffffu; }
+            if (i % 6 == 5) { for (int w = 0; w < N; ++w) u.limb[w] = 0; u.limb[N - 1] = 0x80000000u; }
+            const auto dt = to_dev<N>(t);
+            uint32_t v[N + 1];
+            if (dt.s) lf::reciprocal<N>(dt.w, v);
+            else for (int w = 0; w <= N; ++w) v[w] = 0;
+            report<N>("div by reciprocal", i, lf::div_by_reciprocal<N>(to_dev<N>(u), dt, v), lfo_div(N, u, t));
+        }
         root = a;
         lfo_root_case(&g, N, i, &root);
         if (i % 5 == 0 && root.sign) root.sign = -root.sign;
