@@ -34,26 +34,26 @@ constexpr int PK = 32;        // K per MFMA step
 constexpr int LROW = PK + 16; // LDS row stride in bytes (padding against bank conflicts)
 constexpr u32 XCDS = 8;       // MI355X: 8 XCDs, workgroups dealt to them round-robin by linear id
 
-__device__ __forceinline__ u32 smod(int g, u32 m, u32 c, u32 p31)
-{
-    // g mod m in [0, m) for a signed 32-bit g: (g + 2^31) mod m - 2^31 mod m
-    u32 r = reduce32((u32)g ^ 0x80000000u, m, c);
-    return r >= p31 ? r - p31 : r + m - p31;
-}
-
-// One 32x32 block of residues: (65536 G11 + 256 Gmid + G00) mod m, written as uint16.
+// One 32x32 block of residues: (65536 G11 + 256 Gmid + G00) mod m, written as uint16.  The combination
+// is an integer below 2^48 in magnitude, exact in double precision (as are the two fused multiply-adds
+// forming it); q = floor(t / m) from the double reciprocal is off by at most one, t - q m is then exact
+// (an integer below 2m in magnitude, from one fused multiply-add) and one correction each way lands it
+// in [0, m).  About 13 VALU instructions per residue, FP64 at full rate on this GPU, against 45 for
+// three 32-bit Barrett reductions and their combination.
 __device__ __forceinline__ void store_block(uint16_t* out, u32 Np, u32 row, u32 col, u32 lane, const v16i& c11, const v16i& c00,
-                                            const v16i& cmid, u32 m, u32 cm, u32 p31, u32 p16)
+                                            const v16i& cmid, int m, double md, double inv)
 {
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         // 32x32 accumulator layout: lane l, element i -> row 8(i/4) + 4(l/32) + i%4, column l%32
         const u32 rr = row + 8u * (u32)(i / 4) + 4u * (lane >> 5) + (u32)(i % 4);
         const u32 cc = col + (lane & 31u);
-        const u32 r11 = smod(c11[i], m, cm, p31), r00 = smod(c00[i], m, cm, p31), rm = smod(cmid[i], m, cm, p31);
-        u32 v = reduce32(r11 * p16, m, cm) + reduce32(rm << 8, m, cm);
-        v = (v >= m ? v - m : v) + r00;
-        out[(u64)rr * Np + cc] = (uint16_t)(v >= m ? v - m : v);
+        const double t = __builtin_fma((double)c11[i], 65536.0, __builtin_fma((double)cmid[i], 256.0, (double)c00[i]));
+        const double q = __builtin_floor(t * inv);
+        int r = (int)__builtin_fma(-q, md, t);
+        r += r < 0 ? m : 0;
+        r -= r >= m ? m : 0;
+        out[(u64)rr * Np + cc] = (uint16_t)r;
     }
 }
 
@@ -127,18 +127,17 @@ __global__ void __launch_bounds__(256, 2) product_kernel(ProductArgs g)
         }
         __syncthreads();
     }
-    const u32 m = g.moduli[q], cm = g.recip[q];
-    const u32 p31 = reduce32(0x80000000u, m, cm);
-    const u32 p16 = reduce32(65536u, m, cm);
+    const int m = (int)g.moduli[q];
+    const double md = (double)m, inv = 1.0 / md;
     uint16_t* out = g.res + (u64)q * g.Mp * g.Np;
     // one block at a time (the scheduling barriers keep the accumulators in AGPRs until their turn)
-    store_block(out, g.Np, row0 + wr, col0 + wc, lane, c11[0][0], c00[0][0], cmid[0][0], m, cm, p31, p16);
+    store_block(out, g.Np, row0 + wr, col0 + wc, lane, c11[0][0], c00[0][0], cmid[0][0], m, md, inv);
     __builtin_amdgcn_sched_barrier(0);
-    store_block(out, g.Np, row0 + wr, col0 + wc + 32u, lane, c11[0][1], c00[0][1], cmid[0][1], m, cm, p31, p16);
+    store_block(out, g.Np, row0 + wr, col0 + wc + 32u, lane, c11[0][1], c00[0][1], cmid[0][1], m, md, inv);
     __builtin_amdgcn_sched_barrier(0);
-    store_block(out, g.Np, row0 + wr + 32u, col0 + wc, lane, c11[1][0], c00[1][0], cmid[1][0], m, cm, p31, p16);
+    store_block(out, g.Np, row0 + wr + 32u, col0 + wc, lane, c11[1][0], c00[1][0], cmid[1][0], m, md, inv);
     __builtin_amdgcn_sched_barrier(0);
-    store_block(out, g.Np, row0 + wr + 32u, col0 + wc + 32u, lane, c11[1][1], c00[1][1], cmid[1][1], m, cm, p31, p16);
+    store_block(out, g.Np, row0 + wr + 32u, col0 + wc + 32u, lane, c11[1][1], c00[1][1], cmid[1][1], m, md, inv);
 }
 
 hipError_t launch_product(const ProductArgs& g, hipStream_t s)
