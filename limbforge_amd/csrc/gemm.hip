@@ -31,7 +31,6 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int PT = 128;       // output tile per workgroup (rows and columns)
 constexpr int PK = 32;        // K per MFMA step
-constexpr int LROW = PK + 16; // LDS row stride in bytes (padding against bank conflicts)
 constexpr u32 XCDS = 8;       // MI355X: 8 XCDs, workgroups dealt to them round-robin by linear id
 
 // One 32x32 block of residues: (65536 G11 + 256 Gmid + G00) mod m, written as uint16.  The combination
@@ -58,12 +57,24 @@ __device__ __forceinline__ void store_block(uint16_t* out, u32 Np, u32 row, u32 
 }
 
 // One 128x128 output tile of one modulus: four waves, each a 64x64 sub-tile of 2x2 MFMA blocks with three
-// int32 accumulators each (G11, G00, Gmid).  Per 32-deep K step the workgroup stages both digit planes of
-// its 128 A rows and 128 B columns in LDS (double-buffered: one barrier per step) while the next step's
-// tiles are loaded into registers; each wave then reads 8 fragments and issues 16 v_mfma_i32_32x32x32_i8
-// (a 64x64 wave tile reads half the LDS bytes per MFMA of a 32x32 one).  The 1-D grid is dealt to the XCDs
-// so that all tiles of a modulus run on one XCD (modulus q on XCD q mod 8): a modulus's digit planes are
-// fetched into one L2 instead of eight.
+// int32 accumulators each (G11, G00, Gmid).  The digit tiles of a 32-deep K step (both planes of 128 A
+// rows and 128 B columns, 16 KB) travel from global memory straight into LDS (global_load_lds_dwordx4: lane
+// i's 16 bytes land at 16 i of the instruction's LDS base) through a ring of four stages, so three steps
+// are in flight while one is computed: each wave issues four loads per step, waits for its own loads of
+// the step it is about to use (vmcnt), and one barrier per step both publishes them and frees the oldest
+// stage for refilling.  In a stage, row r's two 16-byte halves sit at chunks 2r + (h xor bit 3 of r), so
+// the 16 lanes of an MFMA operand read hit distinct banks.  Each wave then reads 8 fragments and issues
+// 16 v_mfma_i32_32x32x32_i8.  The 1-D grid is dealt to the XCDs so that all tiles of a modulus run on
+// one XCD (modulus q on XCD q mod 8): a modulus's digit planes are fetched into one L2 instead of eight.
+constexpr u32 STAGES = 4;
+__device__ __forceinline__ u32 chunk_of(u32 r, u32 h) { return 2u * r + (h ^ ((r >> 3) & 1u)); }
+
+template <u32 V> __device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(V < 64, "vmcnt field");
+    __builtin_amdgcn_s_waitcnt((int)((V & 15u) | (7u << 4) | (15u << 8) | ((V >> 4) << 14)));
+}
+
 __global__ void __launch_bounds__(256, 2) product_kernel(ProductArgs g)
 {
     const u32 tiles_x = g.Np / PT, tiles = tiles_x * (g.Mp / PT);
@@ -72,42 +83,48 @@ __global__ void __launch_bounds__(256, 2) product_kernel(ProductArgs g)
     if (q >= g.L) return;
     const u32 row0 = (tile / tiles_x) * PT, col0 = (tile % tiles_x) * PT;
     if (g.lower && col0 > row0 + PT - 1) return; // tile strictly above the diagonal
-    __shared__ __attribute__((aligned(16))) int8_t sa[2][2][PT * LROW]; // [buffer][plane]
-    __shared__ __attribute__((aligned(16))) int8_t sb[2][2][PT * LROW];
+    __shared__ __attribute__((aligned(16))) int8_t ring[STAGES][4][PT * PK]; // [stage][a1, a0, b1, b0][chunks]
     const u64 aplane = (u64)g.Mp * g.Kp, bplane = (u64)g.Np * g.Kp;
     const u32 t = threadIdx.x;
     const u32 lane = t & 63u, wave = t >> 6;
     const u32 wr = (wave >> 1) * 64u, wc = (wave & 1u) * 64u;
-    // staging: thread t moves the 32 bytes of k of row (t & 127) of plane (t >> 7), for A and for B
-    const u32 lp = t >> 7, lr = t & 127u;
-    const int8_t* A = (lp ? g.a0 : g.a1) + q * aplane + (u64)(row0 + lr) * g.Kp;
-    const int8_t* B = (lp ? g.b0 : g.b1) + q * bplane + (u64)(col0 + lr) * g.Kp;
+    // loads: this wave fills chunks [64 wave, 64 wave + 64) of each of the four regions; lane i takes the
+    // chunk at position 64 wave + i, i.e. row p/2, half (p & 1) xor bit 3 of the row
+    const u32 p = 64u * wave + lane, pr = p >> 1, ph = (p & 1u) ^ ((pr >> 3) & 1u);
+    const int8_t* src[4] = {g.a1 + q * aplane + (u64)(row0 + pr) * g.Kp + 16u * ph, g.a0 + q * aplane + (u64)(row0 + pr) * g.Kp + 16u * ph,
+                            g.b1 + q * bplane + (u64)(col0 + pr) * g.Kp + 16u * ph, g.b0 + q * bplane + (u64)(col0 + pr) * g.Kp + 16u * ph};
+    auto issue = [&](u32 st) {
+        const u32 buf = st % STAGES, k0 = st * PK;
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg)
+            __builtin_amdgcn_global_load_lds((const void*)(src[rg] + k0), (__attribute__((address_space(3))) void*)&ring[buf][rg][1024u * wave], 16, 0,
+                                             0);
+    };
     v16i c11[2][2] = {}, c00[2][2] = {}, cmid[2][2] = {};
-    const u32 r = lane & 31u, kg = (lane >> 5) * 16u;
-    v4i pa0 = *(const v4i*)(A), pa1 = *(const v4i*)(A + 16), pb0 = *(const v4i*)(B), pb1 = *(const v4i*)(B + 16);
-    *(v4i*)(&sa[0][lp][lr * LROW]) = pa0;
-    *(v4i*)(&sa[0][lp][lr * LROW + 16]) = pa1;
-    *(v4i*)(&sb[0][lp][lr * LROW]) = pb0;
-    *(v4i*)(&sb[0][lp][lr * LROW + 16]) = pb1;
-    __syncthreads();
+    const u32 r = lane & 31u, kh = lane >> 5;
     const u32 steps = g.Kp / PK;
+#pragma unroll
+    for (u32 st = 0; st + 1 < STAGES; ++st)
+        if (st < steps) issue(st);
     for (u32 s = 0; s < steps; ++s) {
-        const u32 cur = s & 1u;
-        const bool more = s + 1 < steps;
-        if (more) { // next step's tiles, in flight while this step computes
-            const u32 k1 = (s + 1) * PK;
-            pa0 = *(const v4i*)(A + k1);
-            pa1 = *(const v4i*)(A + k1 + 16);
-            pb0 = *(const v4i*)(B + k1);
-            pb1 = *(const v4i*)(B + k1 + 16);
-        }
+        const u32 ahead = steps - 1 - s; // stages issued after s, capped at STAGES - 2
+        if (ahead >= STAGES - 2) wait_vmcnt<4 * (STAGES - 2)>();
+        else if (ahead == 1) wait_vmcnt<4>();
+        else wait_vmcnt<0>();
+        // a bare barrier: __syncthreads' fence would wait for every load in flight (vmcnt 0); the loads
+        // this step reads were waited for above, and the reads of the stage refilled next have completed
+        // (their values fed the previous step's MFMAs)
+        __builtin_amdgcn_s_barrier();
+        if (s + STAGES - 1 < steps) issue(s + STAGES - 1);
+        const u32 cur = s % STAGES;
         v4i a1[2], a0[2], b1[2], b0[2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            a1[h] = *(const v4i*)(&sa[cur][0][(wr + 32u * h + r) * LROW + kg]);
-            a0[h] = *(const v4i*)(&sa[cur][1][(wr + 32u * h + r) * LROW + kg]);
-            b1[h] = *(const v4i*)(&sb[cur][0][(wc + 32u * h + r) * LROW + kg]);
-            b0[h] = *(const v4i*)(&sb[cur][1][(wc + 32u * h + r) * LROW + kg]);
+        for (int hh = 0; hh < 2; ++hh) {
+            const u32 ca = 16u * chunk_of(wr + 32u * hh + r, kh), cb = 16u * chunk_of(wc + 32u * hh + r, kh);
+            a1[hh] = *(const v4i*)(&ring[cur][0][ca]);
+            a0[hh] = *(const v4i*)(&ring[cur][1][ca]);
+            b1[hh] = *(const v4i*)(&ring[cur][2][cb]);
+            b0[hh] = *(const v4i*)(&ring[cur][3][cb]);
         }
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi)
@@ -118,14 +135,6 @@ __global__ void __launch_bounds__(256, 2) product_kernel(ProductArgs g)
                 cmid[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1[mi], b0[ni], cmid[mi][ni], 0, 0, 0);
                 cmid[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0[mi], b1[ni], cmid[mi][ni], 0, 0, 0);
             }
-        if (more) {
-            const u32 nxt = cur ^ 1u;
-            *(v4i*)(&sa[nxt][lp][lr * LROW]) = pa0;
-            *(v4i*)(&sa[nxt][lp][lr * LROW + 16]) = pa1;
-            *(v4i*)(&sb[nxt][lp][lr * LROW]) = pb0;
-            *(v4i*)(&sb[nxt][lp][lr * LROW + 16]) = pb1;
-        }
-        __syncthreads();
     }
     const int m = (int)g.moduli[q];
     const double md = (double)m, inv = 1.0 / md;
