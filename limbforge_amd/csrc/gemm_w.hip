@@ -20,8 +20,9 @@ template <int N> hipError_t launch_rebuild(int wx_class, const RebuildArgs& g, h
     if (!g.rows || !g.n) return hipSuccess;
     if (g.L && g.pairs) { // stage 1: the CRT of every band pair
         const dim3 cgrid((unsigned)(((u64)g.rows * g.n + 255u) / 256u), g.pairs), cblock(256);
-        if (wx_class == 0) hipLaunchKernelGGL((crt_pairs_kernel<rebuild_words(N, 0)>), cgrid, cblock, 0, s, g);
-        else hipLaunchKernelGGL((crt_pairs_kernel<rebuild_words(N, 1)>), cgrid, cblock, 0, s, g);
+        if (wx_class == 0)
+            hipLaunchKernelGGL((crt_pairs_kernel<rebuild_words(N, 0)>), cgrid, cblock, crt_table_bytes(g.L, rebuild_words(N, 0)), s, g);
+        else hipLaunchKernelGGL((crt_pairs_kernel<rebuild_words(N, 1)>), cgrid, cblock, crt_table_bytes(g.L, rebuild_words(N, 1)), s, g);
         if (hipError_t e = hipGetLastError()) return e;
     }
     const dim3 grid(((g.n + REBUILD_BLOCK - 1) / REBUILD_BLOCK) * g.rows), block(REBUILD_BLOCK);
