@@ -688,12 +688,17 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     GEMM_HIP(hipEventRecord(e0, P.stream), "hipEventRecord");
     const uint32_t* C32 = (const uint32_t*)dconst.p;
     if (any_pair) {
-        const std::size_t lplane = Mp * Kp, rplane = Np * Kp;
-        GEMM_HIP(hipMemsetAsync(dig_l1.p, 0, L * lplane, P.stream), "hipMemsetAsync");
-        GEMM_HIP(hipMemsetAsync(dig_l0.p, 0, L * lplane, P.stream), "hipMemsetAsync");
+        // the digits kernel writes every byte of the lines' rows; the padding rows [Me, Mp) of each
+        // modulus plane are cleared here
+        auto clear_tail = [&](void* base, std::size_t rows, std::size_t rows_p) -> hipError_t {
+            if (rows_p == rows) return hipSuccess;
+            return hipMemset2DAsync((int8_t*)base + rows * Kp, rows_p * Kp, 0, (rows_p - rows) * Kp, L, P.stream);
+        };
+        GEMM_HIP(clear_tail(dig_l1.p, Me, Mp), "hipMemset2DAsync");
+        GEMM_HIP(clear_tail(dig_l0.p, Me, Mp), "hipMemset2DAsync");
         if (!P.syrk) {
-            GEMM_HIP(hipMemsetAsync(dig_r1.p, 0, L * rplane, P.stream), "hipMemsetAsync");
-            GEMM_HIP(hipMemsetAsync(dig_r0.p, 0, L * rplane, P.stream), "hipMemsetAsync");
+            GEMM_HIP(clear_tail(dig_r1.p, Ne, Np), "hipMemset2DAsync");
+            GEMM_HIP(clear_tail(dig_r0.p, Ne, Np), "hipMemset2DAsync");
         }
         lf::DigitsArgs d{};
         d.L = L;

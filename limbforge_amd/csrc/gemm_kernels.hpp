@@ -30,54 +30,113 @@ __device__ __forceinline__ u32 reduce32(u32 x, u32 m, u32 c)
 __device__ __forceinline__ u32 mulmod16(u32 a, u32 b, u32 m, u32 c) { return reduce32(a * b, m, c); }
 
 // ---------------------------------------------------------------- digits
+// One workgroup per (line, 256 consecutive k of [0, Kp)): thread t finds the band of entry k0 + t and its
+// 16-bit mantissa chunks once, then for 16 moduli at a time forms the residue of the entry's extended
+// value (linalg.metal:24-65: chunks times 2^(16h) mod m, times 2^shift mod m, sign, balanced, split into
+// d1 * 256 + d0) and leaves the two digit bytes in LDS; the workgroup then writes every band row of the
+// line for those 256 columns as 16-byte chunks, zero where the entry is not a member of that band (or is
+// zero, carries a status, or lies past K).  Every byte of a line's rows is written, so only the padding
+// rows past the last line need clearing on the host side.
+constexpr u32 DIG_QB = 16; // moduli per LDS batch
+
+__device__ __forceinline__ u32 byte_eq_mask(u32 x, u32 b4)
+{
+    // 0xff in each byte of x equal to the byte of b4, 0 elsewhere (the exact zero-byte test)
+    const u32 t = x ^ b4;
+    const u32 z = ~(((t & 0x7f7f7f7fu) + 0x7f7f7f7fu) | t | 0x7f7f7f7fu); // 0x80 where t's byte is zero
+    return (z >> 7) * 0xffu;
+}
+
 template <int N> __global__ void __launch_bounds__(256) digits_kernel(DigitsArgs g)
 {
-    const u32 kblocks = (g.K + 255u) / 256u;
+    __shared__ __attribute__((aligned(16))) int8_t stage[DIG_QB][2][256];
+    __shared__ __attribute__((aligned(16))) uint8_t bandof[256];
+    const u32 kblocks = (g.Kp + 255u) / 256u;
     const u32 line = blockIdx.x / kblocks;
-    const u32 k = (blockIdx.x - line * kblocks) * 256u + threadIdx.x;
-    if (k >= g.K || line >= g.nlines) return;
+    const u32 k0 = (blockIdx.x - line * kblocks) * 256u;
+    if (line >= g.nlines) return;
     const GemmLine* L = g.lines + line;
-    if (L->cls != 0) return;
-    const u64 idx = (u64)line * g.v.ls + (u64)k * g.v.ks;
-    const u32* P = g.v.planes;
-    const u64 pitch = g.v.pitch;
-    const u32 st = P[(u64)(N + 2) * pitch + idx];
-    const int s = (int)P[(u64)(N + 1) * pitch + idx];
-    if (st || !s) return;
-    const int e = (int)P[(u64)N * pitch + idx];
-    int row = -1, shift = 0;
-    for (int b = 0; b < L->nb; ++b)
-        if (e >= L->lo[b] && e <= L->hi[b]) {
-            row = L->member[b];
-            shift = e - L->lo[b];
-        }
-    if (row < 0) return;
+    if (L->cls != 0) return; // no rows (uniform over the workgroup)
+    const u32 t = threadIdx.x, k = k0 + t;
+    const int nb = L->nb;
+    u32 band = 0xffu;
+    int shift = 0, s = 0;
     u32 c[2 * N];
 #pragma unroll
-    for (int w = 0; w < N; ++w) {
-        const u32 x = P[(u64)w * pitch + idx];
-        c[2 * w] = x & 0xffffu;
-        c[2 * w + 1] = x >> 16;
-    }
-    const u64 plane = (u64)g.rows_p * g.Kp;
-    const u64 at = (u64)row * g.Kp + k;
-    for (u32 q = 0; q < g.L; ++q) {
-        const u32 m = g.moduli[q], cm = g.recip[q];
-        const u32* cp = g.chunk_pow + q * 64u;
-        u64 acc = 0;
+    for (int h = 0; h < 2 * N; ++h) c[h] = 0;
+    if (k < g.K) {
+        const u64 idx = (u64)line * g.v.ls + (u64)k * g.v.ks;
+        const u32* P = g.v.planes;
+        const u64 pitch = g.v.pitch;
+        const u32 st = P[(u64)(N + 2) * pitch + idx];
+        s = (int)P[(u64)(N + 1) * pitch + idx];
+        if (!st && s) {
+            const int e = (int)P[(u64)N * pitch + idx];
+            for (int b = 0; b < nb; ++b)
+                if (e >= L->lo[b] && e <= L->hi[b]) {
+                    band = (u32)b;
+                    shift = e - L->lo[b];
+                }
+            if (band != 0xffu)
 #pragma unroll
-        for (int h = 0; h < 2 * N; ++h) acc += (u64)c[h] * cp[h];
-        // acc < 2N * 2^32: fold the high part with 2^32 mod m = cp[2]
-        u32 r = reduce32((u32)acc, m, cm) + (u32)(acc >> 32) * cp[2];
-        r = reduce32(r, m, cm);
-        r = mulmod16(r, g.shift_pow[q * (g.shift_max + 1) + (u32)shift], m, cm);
-        int v = (int)r;
-        if (s < 0) v = v ? (int)m - v : 0;
-        if (v > (int)(m >> 1)) v -= (int)m;
-        const int d0 = ((v + 128) & 255) - 128;
-        const int d1 = (v - d0) / 256;
-        g.d1[q * plane + at] = (int8_t)d1;
-        g.d0[q * plane + at] = (int8_t)d0;
+                for (int w = 0; w < N; ++w) {
+                    const u32 x = P[(u64)w * pitch + idx];
+                    c[2 * w] = x & 0xffffu;
+                    c[2 * w + 1] = x >> 16;
+                }
+        }
+    }
+    bandof[t] = (uint8_t)band;
+    const u64 plane = (u64)g.rows_p * g.Kp;
+    for (u32 qb = 0; qb < g.L; qb += DIG_QB) {
+        const u32 nq = g.L - qb < DIG_QB ? g.L - qb : DIG_QB;
+        // every lane runs the same modulus loop (an entry outside every band has all chunks zero and
+        // yields zero digits); the next modulus's table row is loaded while this one is reduced, so the
+        // loads are not waited for one 16-byte piece at a time
+        u32 cp[2 * N], cpn[2 * N], m = g.moduli[qb], cm = g.recip[qb], sp = g.shift_pow[qb * (g.shift_max + 1) + (u32)shift];
+#pragma unroll
+        for (int h = 0; h < 2 * N; ++h) cp[h] = g.chunk_pow[qb * 64u + h];
+        for (u32 qi = 0; qi < nq; ++qi) {
+            const u32 q = qb + qi, qn = qi + 1 < nq ? q + 1 : q;
+#pragma unroll
+            for (int h = 0; h < 2 * N; ++h) cpn[h] = g.chunk_pow[qn * 64u + h];
+            const u32 mn = g.moduli[qn], cmn = g.recip[qn], spn = g.shift_pow[qn * (g.shift_max + 1) + (u32)shift];
+            u64 acc = 0;
+#pragma unroll
+            for (int h = 0; h < 2 * N; ++h) acc += (u64)c[h] * cp[h];
+            // acc < 2N * 2^32: fold the high part with 2^32 mod m = cp[2]
+            u32 r = reduce32((u32)acc, m, cm) + (u32)(acc >> 32) * cp[2];
+            r = reduce32(r, m, cm);
+            r = mulmod16(r, sp, m, cm);
+            int v = (int)r;
+            if (s < 0) v = v ? (int)m - v : 0;
+            if (v > (int)(m >> 1)) v -= (int)m;
+            const int d0 = ((v + 128) & 255) - 128;
+            const int d1 = (v - d0) / 256;
+            stage[qi][0][t] = (int8_t)d1;
+            stage[qi][1][t] = (int8_t)d0;
+#pragma unroll
+            for (int h = 0; h < 2 * N; ++h) cp[h] = cpn[h];
+            m = mn;
+            cm = cmn;
+            sp = spn;
+        }
+        __syncthreads();
+        // chunk j: 16 columns (c16) of band row b of plane pl of modulus qb + qi
+        const u32 chunks = nq * 2u * (u32)nb * 16u;
+        for (u32 j = t; j < chunks; j += 256u) {
+            const u32 c16 = j & 15u, rest = j >> 4, b = rest % (u32)nb, r2 = rest / (u32)nb, pl = r2 & 1u, qi = r2 >> 1;
+            const u32 kk = k0 + 16u * c16;
+            if (kk >= g.Kp) continue;
+            const uint4 dv = *reinterpret_cast<const uint4*>(&stage[qi][pl][16u * c16]);
+            const uint4 bv = *reinterpret_cast<const uint4*>(&bandof[16u * c16]);
+            const u32 b4 = b * 0x01010101u;
+            const uint4 out = make_uint4(dv.x & byte_eq_mask(bv.x, b4), dv.y & byte_eq_mask(bv.y, b4), dv.z & byte_eq_mask(bv.z, b4),
+                                         dv.w & byte_eq_mask(bv.w, b4));
+            int8_t* dst = (pl ? g.d0 : g.d1) + (u64)(qb + qi) * plane + (u64)L->member[b] * g.Kp + kk;
+            *reinterpret_cast<uint4*>(dst) = out;
+        }
+        __syncthreads();
     }
 }
 

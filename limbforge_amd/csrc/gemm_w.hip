@@ -11,7 +11,8 @@ namespace lf {
 template <int N> hipError_t launch_digits(const DigitsArgs& g, hipStream_t s)
 {
     if (!g.K || !g.nlines || !g.L) return hipSuccess;
-    hipLaunchKernelGGL((digits_kernel<N>), dim3(((g.K + 255u) / 256u) * g.nlines), dim3(256), 0, s, g);
+    if (g.Kp % 16u) return hipErrorInvalidValue; // rows are written in 16-byte chunks
+    hipLaunchKernelGGL((digits_kernel<N>), dim3(((g.Kp + 255u) / 256u) * g.nlines), dim3(256), 0, s, g);
     return hipGetLastError();
 }
 
