@@ -75,6 +75,37 @@ template <int W> __device__ __forceinline__ void shl_bits(u32 (&x)[W], int b)
     }
 }
 
+// x >>= d bits for a runtime d (any size); returns whether a set bit was shifted out.
+template <int W> __device__ __forceinline__ bool shr_sticky(u32 (&x)[W], u64 d)
+{
+    bool sticky = false;
+    if (d >= 32ull * W) {
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            sticky = sticky || x[i] != 0;
+            x[i] = 0;
+        }
+        return sticky;
+    }
+    const int k = (int)(d >> 5);
+    const u32 b = (u32)(d & 31);
+#pragma unroll
+    for (int s = 1; s < W; s <<= 1) {
+        const bool on = (k & s) != 0;
+#pragma unroll
+        for (int i = 0; i < s && i < W; ++i) sticky = sticky || (on && x[i] != 0);
+#pragma unroll
+        for (int i = 0; i < W; ++i) x[i] = on ? (i + s < W ? x[i + s] : 0u) : x[i];
+    }
+    sticky = sticky || (b && (x[0] << (32u - b)) != 0);
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        const u32 v = __builtin_amdgcn_alignbit(i + 1 < W ? x[i + 1] : 0u, x[i], b);
+        x[i] = b ? v : x[i];
+    }
+    return sticky;
+}
+
 // dst (W words) = src (S words) * 2^off for a runtime bit offset; bits that fall below bit 0 of dst
 // are dropped and reported in the return value (nonzero iff any was set); bits above dst are dropped
 // (callers size dst so that none are).
@@ -1080,8 +1111,9 @@ template <int W> __device__ __forceinline__ void accumulate_unit(u32 (&acc)[W], 
 //   c below half an ulp of P's last bit   only c's sign matters: P*2^32 +/- 1 (a sticky unit) is rounded;
 //   P below a quarter ulp of c           RN(c + p) = c (the nearest neighbours of c are at least half an
 //                                         ulp of c away, a quarter below a power of two);
-//   otherwise                            both terms span at most 96N bits: exact sum in a (3N+2)-word
-//                                         window whose bit 0 is the lower of the two last bits, rounded once.
+//   otherwise                            a (2N+3)-word window under the larger term's top bit holds the sum
+//                                         exactly, or up to a sticky unit when the smaller term reaches
+//                                         below it (fma_tail), rounded once.
 // The cases after P is formed: p = the 2N-word product of two nonzero status-free operands whose
 // exponents sum to eab, ps = the product's sign.
 template <int N> __device__ __forceinline__ Num<N> fma_tail(const u32 (&p)[2 * N], i64 eab, int ps, const Num<N>& c)
@@ -1106,49 +1138,72 @@ template <int N> __device__ __forceinline__ Num<N> fma_tail(const u32 (&p)[2 * N
         return pack<N, 2 * N + 1>(x, p_lsb - 32, ps, ST_OK);
     }
     if (p_msb < c_lsb - 2) return c; // |p| < 2^(c_lsb-2)
-    constexpr int W = 3 * N + 2;
-    const bool p_low = p_lsb <= c_lsb; // the term at window bit 0: P, else C
-    const i64 L = p_low ? p_lsb : c_lsb;
-    const i64 off = p_low ? c_lsb - p_lsb : p_lsb - c_lsb; // the other term's offset, below 96N
-    u32 base[W], sh[W];
+    // Near case: a window of W = 2N+3 words with the larger term's top bit (top bit of P or of c, the one
+    // with the higher exponent) at window bit 32W-3; the smaller term, aligned to it, may reach below bit 0
+    // only when it lies 95 or more bits below the larger one's top (the larger term's lowest bit is at
+    // window bit 94 or higher), and then its bits below the window only decide a sticky unit: with T =
+    // 2 (big +/- small) +/- sticky, RN(T 2^(E0-1)) is the result (no rounding boundary of the N-word result
+    // lies within one window unit, and no cancellation beyond one bit is possible there).
+    constexpr int W = 2 * N + 3;
+    const bool p_big = p_msb >= c_msb;
+    const i64 H = p_big ? p_msb : c_msb;
+    const u64 d = (u64)(p_big ? p_msb - c_msb : c_msb - p_msb);
+    // top-aligned images: P shifted left by 64 + pb bits (its top bit, 64N-1 or 64N-2, to 64N+93); c shifted
+    // left by 32(N+2) + 30 bits (its top bit 32N-1 to 64N+93)
+    const u32 pb = top ? 30u : 31u;
+    u32 big[W], sm[W];
 #pragma unroll
     for (int i = 0; i < W; ++i) {
-        const u32 pw = i < 2 * N ? p[i] : 0u, cw = i < N ? c.w[i] : 0u;
-        base[i] = p_low ? pw : cw;
-        sh[i] = p_low ? cw : pw;
+        const u32 phi = (i - 2 >= 0 && i - 2 < 2 * N) ? p[i - 2] : 0u, plo = (i - 3 >= 0 && i - 3 < 2 * N) ? p[i - 3] : 0u;
+        const u32 pa = __builtin_amdgcn_alignbit(phi, plo, 32u - pb);
+        const u32 chi = (i - (N + 2) >= 0 && i - (N + 2) < N) ? c.w[i - (N + 2)] : 0u;
+        const u32 clo = (i - (N + 3) >= 0 && i - (N + 3) < N) ? c.w[i - (N + 3)] : 0u;
+        const u32 ca = __builtin_amdgcn_alignbit(chi, clo, 2u);
+        big[i] = p_big ? pa : ca;
+        sm[i] = p_big ? ca : pa;
     }
-    shl_bits<W>(sh, (int)(off & 31));
-    shl_words<W>(sh, (int)(off >> 5));
-    const int sb = p_low ? ps : c.s, ss = p_low ? c.s : ps;
+    const bool sticky = shr_sticky<W>(sm, d);
+    const int sb = p_big ? ps : c.s, ss = p_big ? c.s : ps;
     int sign = sb;
-    if (sb == ss) {
+    u32 T[W];
+    if (sb == ss) { // T = 2 (big + sm) + sticky
         u32 cy = 0;
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-            const u64 v = (u64)base[i] + sh[i] + cy;
-            base[i] = (u32)v;
+            const u64 v = (u64)big[i] + sm[i] + cy;
+            T[i] = (u32)v;
             cy = (u32)(v >> 32);
         }
-    } else {
+        shl_bits<W>(T, 1);
+        T[0] |= sticky ? 1u : 0u;
+    } else { // T = 2 (big - sm) - sticky, negated when negative
         u32 bw = 0;
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-            const u64 v = (u64)base[i] - sh[i] - bw;
-            base[i] = (u32)v;
+            const u64 v = (u64)big[i] - sm[i] - bw;
+            T[i] = (u32)v;
             bw = (u32)(v >> 63);
         }
-        if (base[W - 1] >> 31) { // the shifted term was larger: negate
+        shl_bits<W>(T, 1);
+        bw = sticky ? 1u : 0u;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            const u64 v = (u64)T[i] - bw;
+            T[i] = (u32)v;
+            bw = (u32)(v >> 63);
+        }
+        if (T[W - 1] >> 31) { // the smaller-exponent term had the larger mantissa (d <= 1): negate
             sign = ss;
             u32 cy = 1;
 #pragma unroll
             for (int i = 0; i < W; ++i) {
-                const u64 v = (u64)(u32)~base[i] + cy;
-                base[i] = (u32)v;
+                const u64 v = (u64)(u32)~T[i] + cy;
+                T[i] = (u32)v;
                 cy = (u32)(v >> 32);
             }
         }
     }
-    return pack<N, W>(base, L, sign, ST_OK);
+    return pack<N, W>(T, H - (64 * N + 93) - 1, sign, ST_OK);
 }
 
 template <int N> __device__ __forceinline__ Num<N> fma(const Num<N>& a, const Num<N>& b, const Num<N>& c)
