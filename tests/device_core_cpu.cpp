@@ -138,6 +138,25 @@ template <int N> void run_width(int cases)
                     report<N>("fma border", i, lf::fma<N>(da, db, to_dev<N>(cc)), lfo_fma(N, a, b, cc));
                 }
         }
+        if (a.sign && b.sign) { // fma over the whole near range: c's top bit from 96N+8 below to 96N+8 above
+                                // P's, and c equal to RN(a*b) within one binade (cancellation)
+            const long pe = (long)a.exponent + b.exponent, span = 96L * N + 8;
+            for (int rep = 0; rep < 4; ++rep) {
+                lfo_num cc = lfo_random_number(&g, N, 10);
+                cc.exponent = (int)(pe + (long)(lfo_mt_next(&g) % (2 * span + 1)) - span);
+                if (rep % 2 == 0) {
+                    const lfo_num pr = lfo_mul(N, a, b);
+                    if (pr.sign) {
+                        for (int w = 0; w < N; ++w) cc.limb[w] = pr.limb[w];
+                        if (rep == 2) cc.limb[0] ^= 1u << (lfo_mt_next(&g) % 32);
+                        cc.exponent = pr.exponent + (int)(lfo_mt_next(&g) % 3) - 1;
+                    }
+                }
+                if (cc.exponent > 1000000000 || cc.exponent < -1000000000) continue;
+                cc.sign = (lfo_mt_next(&g) & 1) ? 1 : -1;
+                report<N>("fma sweep", i, lf::fma<N>(da, db, to_dev<N>(cc)), lfo_fma(N, a, b, cc));
+            }
+        }
         if (i % 4 == 0) {
             lfo_num x[2], y[2], r[2];
             for (int p = 0; p < 2; ++p) { x[p] = lfo_random_number(&g, N, 1000); y[p] = lfo_random_number(&g, N, 1000); }
