@@ -345,8 +345,10 @@ template <int N> constexpr int short_cut()
 }
 
 // p (2N words) = a * b from the digits of a and b (each < 2^(32N)) when full; otherwise the short
-// product P' = a * b - delta, 0 <= delta < 2^(32N-32), from the columns from short_cut<N>() up.
-template <int N>
+// product P' = a * b - delta, 0 <= delta < 2^(32N-32), from the columns from short_cut<N>() up.  With
+// HIGH, only what a rounding to N words reads is formed: words N-1 .. 2N-1 exactly, word 0 nonzero
+// exactly when a bit below word N-1 is set, words 1 .. N-2 zero (no repacking of the low half).
+template <int N, bool HIGH = false>
 __device__ __forceinline__ void digit_columns(const u32 (&da)[digit_count<N>()], const u32 (&db)[digit_count<N>()], u32 (&p)[2 * N],
                                               bool full)
 {
@@ -382,8 +384,18 @@ __device__ __forceinline__ void digit_columns(const u32 (&da)[digit_count<N>()],
         col[k + 1] += col[k] >> DIGIT_BITS;
     }
     dg[2 * D - 1] = (u32)col[2 * D - 1]; // the product is < 2^(64N) <= 2^(58D): the top digit fits in 29 bits
+    if constexpr (HIGH) {
+        // bits below word N-1: digits 0 .. KF-1 whole, the low LB bits of digit KF
+        constexpr int LOWBITS = 32 * (N - 1), KF = LOWBITS / DIGIT_BITS, LB = LOWBITS - DIGIT_BITS * KF;
+        u32 any = LB ? (dg[KF] & ((1u << LB) - 1u)) : 0u;
 #pragma unroll
-    for (int w = 0; w < 2 * N; ++w) {
+        for (int k = 0; k < KF; ++k) any |= dg[k];
+#pragma unroll
+        for (int w = 1; w < N - 1; ++w) p[w] = 0;
+        p[0] = any;
+    }
+#pragma unroll
+    for (int w = HIGH ? N - 1 : 0; w < 2 * N; ++w) {
         const int bit = 32 * w, k = bit / DIGIT_BITS, s = bit % DIGIT_BITS;
         u32 v = dg[k] >> s;
         if (k + 1 < 2 * D) v |= dg[k + 1] << (DIGIT_BITS - s);
@@ -416,14 +428,15 @@ template <int N> __device__ __forceinline__ bool short_decides(const u32 (&p)[2 
 
 // p = the product a * b as far as round_product needs it: the short product when it decides the
 // rounding, else (or with full set) the exact product.
-template <int N>
+// With HIGH, p holds only what round_product reads (see digit_columns).
+template <int N, bool HIGH = false>
 __device__ __forceinline__ void rounding_product(const u32 (&da)[digit_count<N>()], const u32 (&db)[digit_count<N>()], u32 (&p)[2 * N],
                                                  bool full)
 {
     // two calls rather than a retry loop: in a loop the compiler hoists the low columns out of the
     // loop as invariant and forms them on every call
-    digit_columns<N>(da, db, p, full);
-    if (!full && !short_decides<N>(p)) digit_columns<N>(da, db, p, true);
+    digit_columns<N, HIGH>(da, db, p, full);
+    if (!full && !short_decides<N>(p)) digit_columns<N, HIGH>(da, db, p, true);
 }
 
 // p (2N words) = a * b: digit columns from 8 limbs on, schoolbook rows of v_mad_u64_u32 carry chains below.
@@ -499,7 +512,7 @@ template <int N> __device__ __forceinline__ Num<N> mul(const Num<N>& a, const Nu
         u32 da[digit_count<N>()], db[digit_count<N>()];
         to_digits<N>(a.w, da);
         to_digits<N>(b.w, db);
-        rounding_product<N>(da, db, p, false);
+        rounding_product<N, true>(da, db, p, false);
     } else
         mantissa_product<N>(a.w, b.w, p);
     return round_product<N>(p, (i64)a.e + b.e, a.s * b.s, ST_OK);
@@ -512,7 +525,7 @@ template <int N> __device__ __forceinline__ Num<N> mul_by_digits(const Num<N>& a
     if (st || !a.s || !b.s) return zero<N>(st);
     u32 ad[digit_count<N>()], p[2 * N];
     to_digits<N>(a.w, ad);
-    rounding_product<N>(ad, bd, p, false);
+    rounding_product<N, true>(ad, bd, p, false);
     return round_product<N>(p, (i64)a.e + b.e, a.s * b.s, ST_OK);
 }
 
