@@ -95,8 +95,8 @@ const FactorTables& tables()
 
 bool valid_bits(int bits) { return bits >= 64 && bits <= 1024 && bits % 32 == 0; }
 
-// Pinned host copies of the exponent, sign and status planes of whole rows [first_row, first_row + rows)
-// of a matrix with `cols` columns: three contiguous copies, for the band analysis of an update product.
+// Pinned host copies of the exponent, sign and status planes of rows of a matrix (whole rows, or a range
+// of columns), for the band analysis of an update product.
 struct RegionFields {
     uint32_t* host = nullptr;
     std::size_t cap = 0, plane = 0;
@@ -117,6 +117,25 @@ struct RegionFields {
         for (int f = 0; f < 3 && plane; ++f)
             FACTOR_HIP(hipMemcpyAsync(host + f * plane, planes + (uint64_t)(N + f) * pitch + first_row * cols, plane * 4, hipMemcpyDeviceToHost,
                                       stream),
+                       "download fields");
+        return LF_OK;
+    }
+    // columns [col0, col0 + w) of the rows [first_row, first_row + rows) of a matrix with `cols` columns:
+    // three pinned 2-D copies into rows of w words
+    int download_cols(const uint32_t* planes, uint64_t pitch, int N, uint64_t first_row, uint64_t rows, uint64_t cols, uint64_t col0,
+                      uint64_t w, hipStream_t stream)
+    {
+        plane = rows * w;
+        if (3 * plane > cap) {
+            if (host) (void)hipHostFree(host);
+            host = nullptr;
+            cap = 0;
+            FACTOR_HIP(hipHostMalloc((void**)&host, std::max<std::size_t>(3 * plane, 1) * 4, hipHostMallocDefault), "hipHostMalloc (fields)");
+            cap = 3 * plane;
+        }
+        for (int f = 0; f < 3 && plane; ++f)
+            FACTOR_HIP(hipMemcpy2DAsync(host + f * plane, w * 4, planes + (uint64_t)(N + f) * pitch + first_row * cols + col0, cols * 4, w * 4, rows,
+                                        hipMemcpyDeviceToHost, stream),
                        "download fields");
         return LF_OK;
     }
@@ -253,7 +272,7 @@ int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, 
         FACTOR_HIP(tables().panel[N](f, s), "panel launch");
         FACTOR_HIP(hipEventRecord(e1, s), "hipEventRecord");
         FACTOR_HIP(hipMemcpyAsync(&hinfo, dinfo.p, sizeof hinfo, hipMemcpyDeviceToHost, s), "download pivot");
-        if (int r = panel.download(W, pitch, N, k1, n - k1, n, s)) return r;
+        if (int r = panel.download_cols(W, pitch, N, k1, n - k1, n, k0, w, s)) return r; // the panel's columns below the block
         FACTOR_HIP(hipStreamSynchronize(s), "device execution (panel)");
         float ms = 0;
         FACTOR_HIP(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
@@ -272,7 +291,7 @@ int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, 
         if (k1 < n) {
             t = Clock::now();
             const lf::MatView v{W + (k1 * n + k0), pitch, n, 1};
-            if (int r = update(bits, lo, true, v, panel.fields(k0, n, 1), v, panel.fields(k0, n, 1), n - k1, n - k1, w, W + (k1 * n + k1), pitch,
+            if (int r = update(bits, lo, true, v, panel.fields(0, w, 1), v, panel.fields(0, w, 1), n - k1, n - k1, w, W + (k1 * n + k1), pitch,
                                n, s, ms))
                 return r;
             ci.timing_gpu_seconds += ms * 1e-3;
@@ -319,7 +338,12 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
     FACTOR_HIP(drecip.alloc((N + 1) * rpitch * 4), "hipMalloc (reciprocals)");
     FACTOR_HIP(tables().reciprocals[N](F, fpitch, uint32_t(n), (uint32_t*)drecip.p, rpitch, s), "reciprocals launch");
     double gpu = 0;
+    // L does not change during a solve: the fields of all its rows come to the host once (band analysis
+    // of every block's update reads them), the solution's rows block by block
     RegionFields lf_, rf_;
+    if (n > nb) {
+        if (int r = lf_.download(F, fpitch, N, 0, n, n, s)) return r;
+    }
     hipEvent_t e0, e1;
     FACTOR_HIP(hipEventCreate(&e0), "hipEventCreate");
     FACTOR_HIP(hipEventCreate(&e1), "hipEventCreate");
@@ -351,12 +375,11 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
             FACTOR_HIP(block_solve(0, r0, r1), "solve launch");
             if (r1 == n) break;
             // rows [r1, n) of X -= L[r1:, r0:r1] X[r0:r1, :]
-            if (int r = lf_.download(F, fpitch, N, r1, n - r1, n, s)) return r;
             if (int r = rf_.download(Xp, xpitch, N, r0, K, nrhs, s)) return r;
             FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
             collect();
             const lf::MatView left{F + (r1 * n + r0), fpitch, n, 1}, right{Xp + r0 * nrhs, xpitch, 1, nrhs};
-            if (int r = update(bits, lo, false, left, lf_.fields(r0, n, 1), right, rf_.fields(0, 1, nrhs), n - r1, nrhs, K, Xp + r1 * nrhs,
+            if (int r = update(bits, lo, false, left, lf_.fields(r1 * n + r0, n, 1), right, rf_.fields(0, 1, nrhs), n - r1, nrhs, K, Xp + r1 * nrhs,
                                xpitch, nrhs, s, ms))
                 return r;
             gpu += ms * 1e-3;
@@ -371,12 +394,11 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
             FACTOR_HIP(block_solve(1, r0, r1), "solve launch");
             if (!r0) break;
             // rows [0, r0) of X -= L[r0:r1, :r0]^T X[r0:r1, :]
-            if (int r = lf_.download(F, fpitch, N, r0, K, n, s)) return r;
             if (int r = rf_.download(Xp, xpitch, N, r0, K, nrhs, s)) return r;
             FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
             collect();
             const lf::MatView left{F + r0 * n, fpitch, 1, n}, right{Xp + r0 * nrhs, xpitch, 1, nrhs};
-            if (int r = update(bits, lo, false, left, lf_.fields(0, 1, n), right, rf_.fields(0, 1, nrhs), r0, nrhs, K, Xp, xpitch, nrhs, s, ms))
+            if (int r = update(bits, lo, false, left, lf_.fields(r0 * n, 1, n), right, rf_.fields(0, 1, nrhs), r0, nrhs, K, Xp, xpitch, nrhs, s, ms))
                 return r;
             gpu += ms * 1e-3;
         }
