@@ -427,16 +427,17 @@ template <int N> __device__ __forceinline__ bool short_decides(const u32 (&p)[2 
 }
 
 // p = the product a * b as far as round_product needs it: the short product when it decides the
-// rounding, else (or with full set) the exact product.
+// rounding, else (or with full set) the exact product.  With discard (the caller throws p away: a zero
+// or status operand) the short product stands, so such lanes never take the exact path.
 // With HIGH, p holds only what round_product reads (see digit_columns).
 template <int N, bool HIGH = false>
 __device__ __forceinline__ void rounding_product(const u32 (&da)[digit_count<N>()], const u32 (&db)[digit_count<N>()], u32 (&p)[2 * N],
-                                                 bool full)
+                                                 bool full, bool discard = false)
 {
     // two calls rather than a retry loop: in a loop the compiler hoists the low columns out of the
     // loop as invariant and forms them on every call
     digit_columns<N, HIGH>(da, db, p, full);
-    if (!full && !short_decides<N>(p)) digit_columns<N, HIGH>(da, db, p, true);
+    if (!full && !discard && !short_decides<N>(p)) digit_columns<N, HIGH>(da, db, p, true);
 }
 
 // p (2N words) = a * b: digit columns from 8 limbs on, schoolbook rows of v_mad_u64_u32 carry chains below.
@@ -465,15 +466,18 @@ template <int N> __device__ __forceinline__ void mantissa_product(const u32 (&a)
 }
 
 // Round a normalized-input product: the msb of p is bit 64N-1 or 64N-2 (inputs have their top bits set).
+// With kill (a zero or status operand) the result is zero with status st whatever p holds.  No early
+// returns: the words are rounded whatever the range, and only a lane whose result is zero or out of
+// range (msb outside [-1e9, 1e9] after rounding; a msb below -1e9-1 before rounding cannot be rescued
+// by the carry, so one check after rounding covers the reference's two) clears them in place.  A return
+// of zero<N>() on an early path makes the compiler materialize a whole zero result in other registers
+// and copy the rounded one over on every call.
 template <int N>
-__device__ __forceinline__ Num<N> round_product(const u32 (&p)[2 * N], i64 ea_plus_eb, int sign, u32 st)
+__device__ __forceinline__ Num<N> round_product(const u32 (&p)[2 * N], i64 ea_plus_eb, int sign, u32 st, bool kill = false)
 {
     const bool top = (p[2 * N - 1] >> 31) != 0;
     i64 msb = ea_plus_eb + (top ? 1 : 0);
-    if (msb > EMAX || msb < -(i64)EMAX - 1) return zero<N>(st | ST_OVERFLOW);
     Num<N> r;
-    r.s = sign;
-    r.st = st;
     // mantissa = p[N..2N) shifted left by (top ? 0 : 1)
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -486,19 +490,33 @@ __device__ __forceinline__ Num<N> round_product(const u32 (&p)[2 * N], i64 ea_pl
 #pragma unroll
     for (int i = 0; i < N - 1; ++i) sticky = sticky || p[i] != 0;
     const bool inc = round && (sticky || (r.w[0] & 1u));
-    u32 c = inc ? 1u : 0u;
+    // The increment carries out of word 0 only when that word is all ones: the carry chain through the
+    // other words runs on a branch that lanes almost never take (done unconditionally, its N dependent
+    // carries cost about 2N instructions on every call).
+    const u32 w0 = r.w[0] + (inc ? 1u : 0u);
+    u32 c = 0;
+    if (inc && w0 == 0) {
+        c = 1;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        u64 t = (u64)r.w[i] + c;
-        r.w[i] = (u32)t;
-        c = (u32)(t >> 32);
+        for (int i = 1; i < N; ++i) {
+            u64 t = (u64)r.w[i] + c;
+            r.w[i] = (u32)t;
+            c = (u32)(t >> 32);
+        }
     }
-    if (c) {
-        r.w[N - 1] = 0x80000000u;
-        msb += 1;
-    }
-    if (msb > EMAX || msb < -(i64)EMAX) return zero<N>(st | ST_OVERFLOW);
+    r.w[0] = w0;
+    r.w[N - 1] |= c << 31; // a carry out leaves all words zero: the mantissa becomes 2^(32N-1)
+    msb += c;
+    const bool ovf = !kill && (msb > EMAX || msb < -(i64)EMAX);
     r.e = (int)msb;
+    r.s = sign;
+    r.st = ovf ? st | ST_OVERFLOW : st;
+    if (kill || ovf) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) r.w[i] = 0;
+        r.e = 0;
+        r.s = 0;
+    }
     return r;
 }
 
@@ -506,27 +524,27 @@ __device__ __forceinline__ Num<N> round_product(const u32 (&p)[2 * N], i64 ea_pl
 template <int N> __device__ __forceinline__ Num<N> mul(const Num<N>& a, const Num<N>& b)
 {
     const u32 st = a.st | b.st;
-    if (st || !a.s || !b.s) return zero<N>(st);
+    const bool kill = st || !a.s || !b.s;
     u32 p[2 * N];
     if constexpr (use_digits<N>()) {
         u32 da[digit_count<N>()], db[digit_count<N>()];
         to_digits<N>(a.w, da);
         to_digits<N>(b.w, db);
-        rounding_product<N, true>(da, db, p, false);
+        rounding_product<N, true>(da, db, p, false, kill);
     } else
         mantissa_product<N>(a.w, b.w, p);
-    return round_product<N>(p, (i64)a.e + b.e, a.s * b.s, ST_OK);
+    return round_product<N>(p, (i64)a.e + b.e, a.s * b.s, st, kill);
 }
 
 // RN(a * b) with the digits of b's mantissa given (a multiplier reused over a chain); needs use_digits<N>().
 template <int N> __device__ __forceinline__ Num<N> mul_by_digits(const Num<N>& a, const Num<N>& b, const u32 (&bd)[digit_count<N>()])
 {
     const u32 st = a.st | b.st;
-    if (st || !a.s || !b.s) return zero<N>(st);
+    const bool kill = st || !a.s || !b.s;
     u32 ad[digit_count<N>()], p[2 * N];
     to_digits<N>(a.w, ad);
-    rounding_product<N, true>(ad, bd, p, false);
-    return round_product<N>(p, (i64)a.e + b.e, a.s * b.s, ST_OK);
+    rounding_product<N, true>(ad, bd, p, false, kill);
+    return round_product<N>(p, (i64)a.e + b.e, a.s * b.s, st, kill);
 }
 
 // -------------------------------------------------------------------- div
