@@ -157,16 +157,57 @@ template <int W> __device__ __forceinline__ int normalize(u32 (&x)[W])
 }
 
 // ------------------------------------------------------------------ rounding
+// m += inc (0 or 1) for an N-word mantissa; returns the carry out of the top word (every word is then
+// zero).  The increment passes word 0 only when that word is all ones, so the carry chain through the
+// other words runs on a branch that lanes almost never take (done unconditionally, its N dependent
+// carries cost about 2N instructions on every rounding).
+template <int N> __device__ __forceinline__ u32 increment_mantissa(u32 (&m)[N], bool inc)
+{
+    const u32 w0 = m[0] + (inc ? 1u : 0u);
+    u32 c = 0;
+    if (inc && w0 == 0) {
+        c = 1;
+#pragma unroll
+        for (int i = 1; i < N; ++i) {
+            u64 t = (u64)m[i] + c;
+            m[i] = (u32)t;
+            c = (u32)(t >> 32);
+        }
+    }
+    m[0] = w0;
+    return c;
+}
+
+// The fields of a result whose rounded mantissa is in r.w and whose top bit has exponent msb (after
+// rounding): zero with status st when kill (a zero or status operand), zero with st | overflow when msb
+// lies outside [-1e9, 1e9] (the reference's range rule, core.hpp:81-99: a msb below -1e9-1 before
+// rounding cannot be rescued by the carry, so this one check after rounding covers its two).  No early
+// returns: a lane whose result is zero clears its words in place.  A return of zero<N>() on an early
+// path makes the compiler materialize a whole zero result in other registers and copy the rounded one
+// over it on every call.
+template <int N> __device__ __forceinline__ void finish_rounded(Num<N>& r, i64 msb, int sign, u32 st, bool kill)
+{
+    const bool ovf = !kill && (msb > EMAX || msb < -(i64)EMAX);
+    r.e = (int)msb;
+    r.s = sign;
+    r.st = ovf ? st | ST_OVERFLOW : st;
+    if (kill || ovf) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) r.w[i] = 0;
+        r.e = 0;
+        r.s = 0;
+    }
+}
+
 // x (W words) is normalized (bit 32W-1 set) and represents sign * x * 2^(msb - (32W-1)).
 // Round to nearest even to N words with the reference's range rule (core.hpp:81-99): the msb may be
-// emin-1 before rounding (a carry can rescue it) and must lie in [-1e9, 1e9] after.
+// emin-1 before rounding (a carry can rescue it) and must lie in [-1e9, 1e9] after.  With kill the
+// result is zero with status st.
 template <int N, int W>
-__device__ __forceinline__ Num<N> round_normalized(const u32 (&x)[W], i64 msb, int sign, u32 st, bool extra_sticky = false)
+__device__ __forceinline__ Num<N> round_normalized(const u32 (&x)[W], i64 msb, int sign, u32 st, bool extra_sticky = false,
+                                                   bool kill = false)
 {
-    if (msb > EMAX || msb < -(i64)EMAX - 1) return zero<N>(st | ST_OVERFLOW);
     Num<N> r;
-    r.s = sign;
-    r.st = st;
     if constexpr (W <= N) {
 #pragma unroll
         for (int i = 0; i < N; ++i) r.w[i] = i >= N - W ? x[i - (N - W)] : 0u;
@@ -178,20 +219,11 @@ __device__ __forceinline__ Num<N> round_normalized(const u32 (&x)[W], i64 msb, i
 #pragma unroll
         for (int i = 0; i < W - N - 1; ++i) sticky = sticky || x[i] != 0;
         const bool inc = (g >> 31) && (sticky || (r.w[0] & 1u));
-        u32 c = inc ? 1u : 0u;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            u64 t = (u64)r.w[i] + c;
-            r.w[i] = (u32)t;
-            c = (u32)(t >> 32);
-        }
-        if (c) {
-            r.w[N - 1] = 0x80000000u;
-            msb += 1;
-        }
+        const u32 c = increment_mantissa<N>(r.w, inc);
+        r.w[N - 1] |= c << 31;
+        msb += c;
     }
-    if (msb > EMAX || msb < -(i64)EMAX) return zero<N>(st | ST_OVERFLOW);
-    r.e = (int)msb;
+    finish_rounded<N>(r, msb, sign, st, kill);
     return r;
 }
 
@@ -200,33 +232,22 @@ __device__ __forceinline__ Num<N> round_normalized(const u32 (&x)[W], i64 msb, i
 template <int N>
 __device__ __forceinline__ Num<N> round_words(const u32 (&m)[N], u32 round, bool sticky, i64 msb, int sign, u32 st)
 {
-    if (msb > EMAX || msb < -(i64)EMAX - 1) return zero<N>(st | ST_OVERFLOW);
     Num<N> r;
-    r.s = sign;
-    r.st = st;
-    const bool inc = round && (sticky || (m[0] & 1u));
-    u32 c = inc ? 1u : 0u;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        u64 t = (u64)m[i] + c;
-        r.w[i] = (u32)t;
-        c = (u32)(t >> 32);
-    }
-    if (c) {
-        r.w[N - 1] = 0x80000000u;
-        msb += 1;
-    }
-    if (msb > EMAX || msb < -(i64)EMAX) return zero<N>(st | ST_OVERFLOW);
-    r.e = (int)msb;
+    for (int i = 0; i < N; ++i) r.w[i] = m[i];
+    const bool inc = round && (sticky || (m[0] & 1u));
+    const u32 c = increment_mantissa<N>(r.w, inc);
+    r.w[N - 1] |= c << 31;
+    msb += c;
+    finish_rounded<N>(r, msb, sign, st, false);
     return r;
 }
 
 // RN-even of sign * int(x) * 2^scale (the reference's pack for any word count, core.hpp:81-99).
 template <int N, int W> __device__ __forceinline__ Num<N> pack(u32 (&x)[W], i64 scale, int sign, u32 st)
 {
-    int lz = normalize(x);
-    if (lz < 0) return zero<N>(st);
-    return round_normalized<N, W>(x, scale + (32 * W - 1 - lz), sign, st);
+    const int lz = normalize(x);
+    return round_normalized<N, W>(x, scale + (32 * W - 1 - lz), sign, st, false, lz < 0);
 }
 
 // ------------------------------------------------------------- comparisons
@@ -466,12 +487,7 @@ template <int N> __device__ __forceinline__ void mantissa_product(const u32 (&a)
 }
 
 // Round a normalized-input product: the msb of p is bit 64N-1 or 64N-2 (inputs have their top bits set).
-// With kill (a zero or status operand) the result is zero with status st whatever p holds.  No early
-// returns: the words are rounded whatever the range, and only a lane whose result is zero or out of
-// range (msb outside [-1e9, 1e9] after rounding; a msb below -1e9-1 before rounding cannot be rescued
-// by the carry, so one check after rounding covers the reference's two) clears them in place.  A return
-// of zero<N>() on an early path makes the compiler materialize a whole zero result in other registers
-// and copy the rounded one over on every call.
+// With kill (a zero or status operand) the result is zero with status st whatever p holds.
 template <int N>
 __device__ __forceinline__ Num<N> round_product(const u32 (&p)[2 * N], i64 ea_plus_eb, int sign, u32 st, bool kill = false)
 {
@@ -490,33 +506,10 @@ __device__ __forceinline__ Num<N> round_product(const u32 (&p)[2 * N], i64 ea_pl
 #pragma unroll
     for (int i = 0; i < N - 1; ++i) sticky = sticky || p[i] != 0;
     const bool inc = round && (sticky || (r.w[0] & 1u));
-    // The increment carries out of word 0 only when that word is all ones: the carry chain through the
-    // other words runs on a branch that lanes almost never take (done unconditionally, its N dependent
-    // carries cost about 2N instructions on every call).
-    const u32 w0 = r.w[0] + (inc ? 1u : 0u);
-    u32 c = 0;
-    if (inc && w0 == 0) {
-        c = 1;
-#pragma unroll
-        for (int i = 1; i < N; ++i) {
-            u64 t = (u64)r.w[i] + c;
-            r.w[i] = (u32)t;
-            c = (u32)(t >> 32);
-        }
-    }
-    r.w[0] = w0;
+    const u32 c = increment_mantissa<N>(r.w, inc);
     r.w[N - 1] |= c << 31; // a carry out leaves all words zero: the mantissa becomes 2^(32N-1)
     msb += c;
-    const bool ovf = !kill && (msb > EMAX || msb < -(i64)EMAX);
-    r.e = (int)msb;
-    r.s = sign;
-    r.st = ovf ? st | ST_OVERFLOW : st;
-    if (kill || ovf) {
-#pragma unroll
-        for (int i = 0; i < N; ++i) r.w[i] = 0;
-        r.e = 0;
-        r.s = 0;
-    }
+    finish_rounded<N>(r, msb, sign, st, kill);
     return r;
 }
 
@@ -1255,12 +1248,14 @@ template <int N>
 __device__ __forceinline__ Num<N> fma_by_digits(const Num<N>& a, const Num<N>& b, const u32 (&bd)[digit_count<N>()], const Num<N>& c)
 {
     const u32 st = a.st | b.st | c.st;
-    if (st) return zero<N>(st);
     const int ps = a.s * b.s;
-    if (!ps) return c.s ? c : zero<N>(ST_OK);
+    const bool kill = st || !ps;
     u32 ad[digit_count<N>()], p[2 * N];
     to_digits<N>(a.w, ad);
-    rounding_product<N>(ad, bd, p, c.s != 0); // with c = 0 only the rounding of P is needed
+    rounding_product<N>(ad, bd, p, c.s != 0, kill); // with c = 0 only the rounding of P is needed
+    // c = 0 (a chain's mul step) and status operands: the product's rounding, cleared when kill
+    if (!c.s || st) return round_product<N>(p, (i64)a.e + b.e, ps, st, kill);
+    if (!ps) return c; // RN(c) = c
     return fma_tail<N>(p, (i64)a.e + b.e, ps, c);
 }
 
