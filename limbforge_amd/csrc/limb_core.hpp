@@ -75,6 +75,17 @@ template <int W> __device__ __forceinline__ void shl_bits(u32 (&x)[W], int b)
     }
 }
 
+// x - y - bw (bw 0 or 1) in 32 bits, with the borrow out in bw.  Written with 32-bit compares, a chain of
+// these becomes one v_sub_co / v_subb_co pair per word; the 64-bit form t = x - y - bw, bw = t >> 63
+// costs about six instructions per word.
+__device__ __forceinline__ u32 sub_borrow(u32 x, u32 y, u32& bw)
+{
+    const u32 d = x - y;
+    const u32 r = d - bw;
+    bw = (u32)(x < y) | (u32)(d < bw);
+    return r;
+}
+
 // x >>= d bits for a runtime d (any size); returns whether a set bit was shifted out.
 template <int W> __device__ __forceinline__ bool shr_sticky(u32 (&x)[W], u64 d)
 {
@@ -150,7 +161,9 @@ template <int W> __device__ __forceinline__ int normalize(u32 (&x)[W])
         lzw += seen ? 0 : 1;
     }
     if (lzw == W) return -1;
-    shl_words(x, lzw);
+    // the word shifter (log2(W) select stages over W words) runs only on a branch: a value whose top word
+    // is set, the usual case after an addition or an fma window, needs the bit shift alone
+    if (lzw) shl_words(x, lzw);
     int b = __clz(x[W - 1]);
     shl_bits(x, b);
     return 32 * lzw + b;
@@ -244,10 +257,39 @@ __device__ __forceinline__ Num<N> round_words(const u32 (&m)[N], u32 round, bool
 }
 
 // RN-even of sign * int(x) * 2^scale (the reference's pack for any word count, core.hpp:81-99).
+// Only the words a rounding reads are shifted into place: after the (rarely needed) word shift, the N
+// mantissa words and the guard word are funnel-shifted by the top word's leading zeros, and the sticky
+// bit is the OR of the words below, which a shift does not change (x is clobbered).
 template <int N, int W> __device__ __forceinline__ Num<N> pack(u32 (&x)[W], i64 scale, int sign, u32 st)
 {
-    const int lz = normalize(x);
-    return round_normalized<N, W>(x, scale + (32 * W - 1 - lz), sign, st, false, lz < 0);
+    if constexpr (W <= N + 1) {
+        const int lz = normalize(x);
+        return round_normalized<N, W>(x, scale + (32 * W - 1 - lz), sign, st, false, lz < 0);
+    } else {
+        int lzw = 0;
+        bool seen = false;
+#pragma unroll
+        for (int i = W - 1; i >= 0; --i) {
+            seen = seen || x[i] != 0;
+            lzw += seen ? 0 : 1;
+        }
+        const bool kill = lzw == W;
+        if (lzw && !kill) shl_words(x, lzw); // the word shifter runs on a branch: the top word is usually set
+        const int b = __clz(x[W - 1]) & 31;  // 0 when x is zero (killed)
+        const u32 r = (u32)(32 - b) & 31u;
+        u32 m[N + 1]; // m[0] = the guard word, m[1..N] the mantissa words of x << b
+#pragma unroll
+        for (int i = 0; i <= N; ++i) {
+            const int k = W - 1 - N + i;
+            m[i] = b ? __builtin_amdgcn_alignbit(x[k], x[k - 1], r) : x[k];
+        }
+        // bits of x << b below the guard word: all of x[0 .. W-N-3] and x[W-N-2] << b
+        u32 low = x[W - N - 2] << b;
+#pragma unroll
+        for (int i = 0; i < W - N - 2; ++i) low |= x[i];
+        const i64 msb = scale + (32 * W - 1 - (32 * lzw + b));
+        return round_normalized<N, N + 1>(m, msb, sign, st, low != 0, kill);
+    }
 }
 
 // ------------------------------------------------------------- comparisons
@@ -318,9 +360,7 @@ template <int N> __device__ __forceinline__ Num<N> add(Num<N> a, Num<N> b)
         u32 bw = 0;
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-            u64 t = (u64)x[i] - y[i] - bw;
-            x[i] = (u32)t;
-            bw = (u32)(t >> 63);
+            x[i] = sub_borrow(x[i], y[i], bw);
         }
     }
     return pack<N, W>(x, (i64)a.e - (32 * N - 1) - (32 * N + 2), a.s, ST_OK);
@@ -488,8 +528,12 @@ template <int N> __device__ __forceinline__ void mantissa_product(const u32 (&a)
 
 // Round a normalized-input product: the msb of p is bit 64N-1 or 64N-2 (inputs have their top bits set).
 // With kill (a zero or status operand) the result is zero with status st whatever p holds.
+// With nudge = +1 or -1 the value rounded is P + nudge * delta for some 0 < delta < half a unit of P's
+// last bit: a positive nudge sets the sticky bit (a tail of exactly one half rounds up), a negative one
+// puts a tail of exactly one half below the midpoint (it rounds down); any other tail rounds as P's.
 template <int N>
-__device__ __forceinline__ Num<N> round_product(const u32 (&p)[2 * N], i64 ea_plus_eb, int sign, u32 st, bool kill = false)
+__device__ __forceinline__ Num<N> round_product(const u32 (&p)[2 * N], i64 ea_plus_eb, int sign, u32 st, bool kill = false,
+                                                int nudge = 0)
 {
     const bool top = (p[2 * N - 1] >> 31) != 0;
     i64 msb = ea_plus_eb + (top ? 1 : 0);
@@ -505,7 +549,7 @@ __device__ __forceinline__ Num<N> round_product(const u32 (&p)[2 * N], i64 ea_pl
     bool sticky = (top ? (g << 1) : (g << 2)) != 0;
 #pragma unroll
     for (int i = 0; i < N - 1; ++i) sticky = sticky || p[i] != 0;
-    const bool inc = round && (sticky || (r.w[0] & 1u));
+    const bool inc = round && (sticky || nudge > 0 || (nudge == 0 && (r.w[0] & 1u)));
     const u32 c = increment_mantissa<N>(r.w, inc);
     r.w[N - 1] |= c << 31; // a carry out leaves all words zero: the mantissa becomes 2^(32N-1)
     msb += c;
@@ -781,9 +825,7 @@ template <int N> __device__ __forceinline__ Num<N> div_by_reciprocal(const Num<N
     u32 bw = 0;
 #pragma unroll
     for (int k = 0; k <= N; ++k) {
-        const u64 t = (u64)R[k] - qb[k] - bw;
-        R[k] = (u32)t;
-        bw = (u32)(t >> 63);
+        R[k] = sub_borrow(R[k], qb[k], bw);
     }
     // R in [0, 2B): one more unit of the quotient when R >= B
     bool ge = R[N] != 0;
@@ -792,9 +834,7 @@ template <int N> __device__ __forceinline__ Num<N> div_by_reciprocal(const Num<N
         u32 c = 0;
 #pragma unroll
         for (int k = 0; k <= N; ++k) {
-            const u64 t = (u64)R[k] - (k < N ? b.w[k] : 0u) - c;
-            R[k] = (u32)t;
-            c = (u32)(t >> 63);
+            R[k] = sub_borrow(R[k], (k < N ? b.w[k] : 0u), c);
         }
         c = 1;
 #pragma unroll
@@ -857,9 +897,7 @@ template <int N> __device__ __noinline__ Num<N> sqrt_restoring(const Num<N>& a)
             u32 diff[N + 1];
 #pragma unroll
             for (int i = 0; i <= N; ++i) {
-                u64 t = (u64)rem[i] - trial[i] - bw;
-                diff[i] = (u32)t;
-                bw = (u32)(t >> 63);
+                diff[i] = sub_borrow(rem[i], trial[i], bw);
             }
             const bool ge = bw == 0;
 #pragma unroll
@@ -942,9 +980,7 @@ template <int M, int W> __device__ void rsqrt_fixed(const u32 (&X)[W], u32 (&Z)[
     u32 bw = 0;
 #pragma unroll
     for (int i = 0; i < M; ++i) { // D = 3 * 2^F - x z^2 * 2^F
-        const u64 t = (u64)(i == M - 1 ? 0x60000000u : 0u) - B[M + i] - bw;
-        D[i] = (u32)t;
-        bw = (u32)(t >> 63);
+        D[i] = sub_borrow(i == M - 1 ? 0x60000000u : 0u, B[M + i], bw);
     }
     words_to_digits<M>(D, dd);
     digit_product_low<DM, DM, 2 * M>(dz, dd, P);
@@ -1000,9 +1036,7 @@ template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
         u32 bw = 0;
 #pragma unroll
         for (int i = 0; i < 2 * N; ++i) {
-            const u64 t = (u64)X[i] - sq[i] - bw;
-            rem[i] = (u32)t;
-            bw = (u32)(t >> 63);
+            rem[i] = sub_borrow(X[i], sq[i], bw);
         }
         rem[2 * N] = bw ? 0xffffffffu : 0u;
     }
@@ -1028,16 +1062,12 @@ template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
             u32 bw = 2;
 #pragma unroll
             for (int i = 0; i <= 2 * N; ++i) {
-                const u64 t = (u64)rem[i] - bw;
-                rem[i] = (u32)t;
-                bw = (u32)(t >> 63);
+                rem[i] = sub_borrow(rem[i], 0u, bw);
             }
             u32 b = 1;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                const u64 t = (u64)r[i] - b;
-                r[i] = (u32)t;
-                b = (u32)(t >> 63);
+                r[i] = sub_borrow(r[i], 0u, b);
             }
             continue;
         }
@@ -1052,9 +1082,7 @@ template <int N> __device__ __forceinline__ Num<N> sqrt(const Num<N>& a)
             u32 bw = 0;
 #pragma unroll
             for (int i = 0; i <= 2 * N; ++i) {
-                const u64 t = (u64)rem[i] - (i <= N ? t2[i] : 0u) - bw;
-                rem[i] = (u32)t;
-                bw = (u32)(t >> 63);
+                rem[i] = sub_borrow(rem[i], (i <= N ? t2[i] : 0u), bw);
             }
             u32 c = 1;
 #pragma unroll
@@ -1110,9 +1138,7 @@ template <int W> __device__ __forceinline__ void accumulate_signed(u32 (&acc)[W]
         u32 bw = 0;
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-            u64 t = (u64)acc[i] - x[i] - bw;
-            acc[i] = (u32)t;
-            bw = (u32)(t >> 63);
+            acc[i] = sub_borrow(acc[i], x[i], bw);
         }
     }
 }
@@ -1147,20 +1173,9 @@ template <int N> __device__ __forceinline__ Num<N> fma_tail(const u32 (&p)[2 * N
     const i64 p_msb = eab + (top ? 1 : 0);
     const i64 p_lsb = eab - 2 * (32 * N - 1);
     const i64 c_msb = c.e, c_lsb = (i64)c.e - (32 * N - 1);
-    if (c_msb < p_lsb - 1) {
-        // |c| < 2^(p_lsb-1): magnitude P*2^p_lsb + t*delta, 0 < delta < half of P's last unit
-        const int t = ps * c.s;
-        u32 x[2 * N + 1];
-        x[0] = t > 0 ? 1u : 0xffffffffu;
-        u32 bw = t > 0 ? 0u : 1u;
-#pragma unroll
-        for (int i = 0; i < 2 * N; ++i) {
-            const u64 v = (u64)p[i] - bw;
-            x[i + 1] = (u32)v;
-            bw = (u32)(v >> 63);
-        }
-        return pack<N, 2 * N + 1>(x, p_lsb - 32, ps, ST_OK);
-    }
+    // |c| < 2^(p_lsb-1), half a unit of P's last bit: the exact sum is P + t * delta, t the sign of c
+    // relative to P, 0 < delta < that half unit, which is P's rounding with the tail nudged by t
+    if (c_msb < p_lsb - 1) return round_product<N>(p, eab, ps, ST_OK, false, ps * c.s);
     if (p_msb < c_lsb - 2) return c; // |p| < 2^(c_lsb-2)
     // Near case: a window of W = 2N+3 words with the larger term's top bit (top bit of P or of c, the one
     // with the higher exponent) at window bit 32W-3; the smaller term, aligned to it, may reach below bit 0
@@ -1194,9 +1209,9 @@ template <int N> __device__ __forceinline__ Num<N> fma_tail(const u32 (&p)[2 * N
         u32 cy = 0;
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-            const u64 v = (u64)big[i] + sm[i] + cy;
-            T[i] = (u32)v;
-            cy = (u32)(v >> 32);
+            const u32 t = big[i] + sm[i];
+            T[i] = t + cy;
+            cy = (u32)(t < big[i]) | (u32)(T[i] < t);
         }
         shl_bits<W>(T, 1);
         T[0] |= sticky ? 1u : 0u;
@@ -1204,17 +1219,17 @@ template <int N> __device__ __forceinline__ Num<N> fma_tail(const u32 (&p)[2 * N
         u32 bw = 0;
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-            const u64 v = (u64)big[i] - sm[i] - bw;
-            T[i] = (u32)v;
-            bw = (u32)(v >> 63);
+            const u32 d = big[i] - sm[i];
+            T[i] = d - bw;
+            bw = (u32)(big[i] < sm[i]) | (u32)(d < bw);
         }
         shl_bits<W>(T, 1);
         bw = sticky ? 1u : 0u;
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-            const u64 v = (u64)T[i] - bw;
-            T[i] = (u32)v;
-            bw = (u32)(v >> 63);
+            const u32 t = T[i];
+            T[i] = t - bw;
+            bw = (u32)(t < bw);
         }
         if (T[W - 1] >> 31) { // the smaller-exponent term had the larger mantissa (d <= 1): negate
             sign = ss;
