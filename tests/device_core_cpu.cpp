@@ -138,6 +138,31 @@ template <int N> void run_width(int cases)
                     report<N>("fma border", i, lf::fma<N>(da, db, to_dev<N>(cc)), lfo_fma(N, a, b, cc));
                 }
         }
+        if (i % 3 == 0) { // fma with c below half a unit of P's last bit on a product whose tail is exactly one
+                          // half: a odd in [2^(32N-1), 2/3 * 2^(32N)) times b = 1.5 puts P's lowest set bit on
+                          // the round bit (P = 3a * 2^(32N-2) below 2^(64N-1)); c's sign decides the tie
+            lfo_num ta = lfo_random_number(&g, N, 50), tb = lfo_pow2(N, (int)(lfo_mt_next(&g) % 41) - 20);
+            ta.limb[N - 1] = 0x80000000u | (ta.limb[N - 1] & 0x1fffffffu); // below 2/3 of 2^(32N): top bits 100
+            ta.limb[0] |= 1u;
+            tb.limb[N - 1] = 0xc0000000u;
+            if (ta.sign && tb.sign) {
+                for (int side = 0; side < 4; ++side) {
+                    lfo_num cc = lfo_random_number(&g, N, 10);
+                    const long plsb = (long)ta.exponent + tb.exponent - 2 * (32L * N - 1);
+                    cc.exponent = (int)(plsb - 2 - (long)(lfo_mt_next(&g) % 40));
+                    cc.sign = side % 2 ? 1 : -1;
+                    if (side >= 2) ta.sign = -ta.sign;
+                    if (cc.exponent > 1000000000 || cc.exponent < -1000000000) continue;
+                    report<N>("fma far tie", i, lf::fma<N>(to_dev<N>(ta), to_dev<N>(tb), to_dev<N>(cc)), lfo_fma(N, ta, tb, cc));
+                    if constexpr (lf::use_digits<N>()) {
+                        uint32_t bd[lf::digit_count<N>()];
+                        lf::to_digits<N>(to_dev<N>(tb).w, bd);
+                        report<N>("fma far tie digits", i, lf::fma_by_digits<N>(to_dev<N>(ta), to_dev<N>(tb), bd, to_dev<N>(cc)),
+                                  lfo_fma(N, ta, tb, cc));
+                    }
+                }
+            }
+        }
         if (a.sign && b.sign) { // fma over the whole near range: c's top bit from 96N+8 below to 96N+8 above
                                 // P's, and c equal to RN(a*b) within one binade (cancellation)
             const long pe = (long)a.exponent + b.exponent, span = 96L * N + 8;
