@@ -100,11 +100,26 @@ template <int W> __device__ __forceinline__ bool shr_sticky(u32 (&x)[W], u64 d)
     }
     const int k = (int)(d >> 5);
     const u32 b = (u32)(d & 31);
+    // the whole words shifted out are x[0 .. k-1]: their OR is the prefix OR pre[k], picked from the prefix
+    // ORs by a select tree over the bits of k (about 2W instructions; OR-ing the words each shifter stage
+    // drops costs about four per word per stage)
+    {
+        constexpr int P = W + 1;
+        u32 pre[P];
+        pre[0] = 0;
+#pragma unroll
+        for (int j = 0; j < W; ++j) pre[j + 1] = pre[j] | x[j];
+#pragma unroll
+        for (int s = 1; s < P; s <<= 1) { // after this level, pre[j] holds the prefix OR at j + (k mod 2s) rounded
+            const bool on = (k & s) != 0;  // to the level: entries j stepped by s while the bit is set
+#pragma unroll
+            for (int j = 0; j < P; ++j) pre[j] = on ? (j + s < P ? pre[j + s] : pre[P - 1]) : pre[j];
+        }
+        sticky = pre[0] != 0;
+    }
 #pragma unroll
     for (int s = 1; s < W; s <<= 1) {
         const bool on = (k & s) != 0;
-#pragma unroll
-        for (int i = 0; i < s && i < W; ++i) sticky = sticky || (on && x[i] != 0);
 #pragma unroll
         for (int i = 0; i < W; ++i) x[i] = on ? (i + s < W ? x[i + s] : 0u) : x[i];
     }
