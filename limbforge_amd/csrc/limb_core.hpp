@@ -281,15 +281,19 @@ template <int N, int W> __device__ __forceinline__ Num<N> pack(u32 (&x)[W], i64 
         const int lz = normalize(x);
         return round_normalized<N, W>(x, scale + (32 * W - 1 - lz), sign, st, false, lz < 0);
     } else {
+        // the count of leading zero words and the word shifter run on a branch: the top word is usually set
         int lzw = 0;
-        bool seen = false;
+        bool kill = false;
+        if (x[W - 1] == 0) {
+            bool seen = false;
 #pragma unroll
-        for (int i = W - 1; i >= 0; --i) {
-            seen = seen || x[i] != 0;
-            lzw += seen ? 0 : 1;
+            for (int i = W - 1; i >= 0; --i) {
+                seen = seen || x[i] != 0;
+                lzw += seen ? 0 : 1;
+            }
+            kill = lzw == W;
+            if (!kill) shl_words(x, lzw);
         }
-        const bool kill = lzw == W;
-        if (lzw && !kill) shl_words(x, lzw); // the word shifter runs on a branch: the top word is usually set
         const int b = __clz(x[W - 1]) & 31;  // 0 when x is zero (killed)
         const u32 r = (u32)(32 - b) & 31u;
         u32 m[N + 1]; // m[0] = the guard word, m[1..N] the mantissa words of x << b
