@@ -35,7 +35,7 @@ BITS, COUNT, CHAIN = 1024, 65536, 16
 # (benchmarks/results/final.csv:145) -> 65,536 / 3.192 ms.
 REFERENCE_VALUES_PER_S = COUNT / 3.192e-3
 # Measured HBM traffic of one mul_chain_kernel<32> launch (counter passes described at the roofline below).
-TRAFFIC_BYTES = round((9127.7 * 262144 / 131084 + 8960.0) * 1024)
+TRAFFIC_BYTES = round((9081.8 * 262144 / 131084.6 + 8960.0) * 1024)
 
 
 def log(*a):
@@ -529,7 +529,7 @@ def main(argv=None) -> int:
         "frac": round(achieved / peak, 4),
         # HBM bytes per launch from the FETCH_SIZE and WRITE_SIZE counter passes of this command
         # (rocprofv3 --pmc, one counter per run), each scaled by the same counter's reading for the 256 MiB
-        # reference copy lf_calibrate_copy: FETCH 9127.7 KiB x (262144 / 131084) + WRITE 8960 KiB x 1.0
+        # reference copy lf_calibrate_copy: FETCH 9081.8 KiB x (262144 / 131084.6) + WRITE 8960 KiB x 1.0
         "traffic": TRAFFIC_BYTES,
         "kernel": "mul_chain_kernel<32>",
         "kernel_ms": round(kernel_ms, 5),
