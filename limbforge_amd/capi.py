@@ -221,22 +221,47 @@ def mul_chain(bits: int, x: Buffer, f: Buffer, out: Buffer, steps: int, f_index=
     check(lib().lf_mul_chain(bits, C.byref(ox), C.byref(of), C.byref(oo), x.count, _bcast(f_index), int(steps), stream))
 
 
+def broadcast_elements(count: int, index) -> int:
+    """Elements an operand read through Broadcast{stride, period} must hold (engine.hpp:22-25)."""
+    if index is None:
+        return count
+    stride, period = index
+    if stride <= 0:
+        raise ValueError("broadcast stride must be positive")
+    n = count // stride + (count % stride != 0)
+    return period if period and period < n else n
+
+
 def run_host(bits: int, op, a: np.ndarray, b: np.ndarray | None = None, c: np.ndarray | None = None,
              b_index=None, c_index=None):
     """Engine::run / run_unary / run_ternary (engine.hpp:202-204): host arrays in, host array out.
-    Returns (out, gpu_seconds, wall_seconds)."""
+    Returns (out, gpu_seconds, wall_seconds).  b and c must hold broadcast_elements(count, index) records of
+    the operation's width (ValueError otherwise), as the reference checks its buffer sizes."""
     code = OPS[op] if isinstance(op, str) else int(op)
+    rw = record_words(bits, code in COMPLEX_OPS)
     a = np.ascontiguousarray(a, dtype=np.uint32)
+    if a.ndim != 2 or a.shape[1] != rw:
+        raise ValueError(f"a must be an array of {rw}-word records")
     count = a.shape[0]
     out = np.zeros_like(a)
-    pb = np.ascontiguousarray(b, dtype=np.uint32).ctypes.data if b is not None else None
-    bb = np.ascontiguousarray(b, dtype=np.uint32) if b is not None else None
-    cc = np.ascontiguousarray(c, dtype=np.uint32) if c is not None else None
+
+    def operand(x, index, name, used):
+        if not used:
+            return None
+        if x is None:
+            raise ValueError(f"operation {op} needs operand {name}")
+        x = np.ascontiguousarray(x, dtype=np.uint32)
+        need = broadcast_elements(count, index)
+        if x.ndim != 2 or x.shape[1] != rw or x.shape[0] < need:
+            raise ValueError(f"operand {name} must hold {need} records of {rw} words")
+        return x
+
+    bb = operand(b, b_index, "b", code not in UNARY_OPS)
+    cc = operand(c, c_index, "c", code in TERNARY_OPS)
     g, w = C.c_double(0), C.c_double(0)
     check(lib().lf_run_host(bits, code, a.ctypes.data, bb.ctypes.data if bb is not None else None,
                             cc.ctypes.data if cc is not None else None, out.ctypes.data, count,
                             _bcast(b_index), _bcast(c_index), C.byref(g), C.byref(w)))
-    del pb
     return out, g.value, w.value
 
 
@@ -310,7 +335,9 @@ class LinalgReport(C.Structure):
                 ("trivial_outputs", C.c_uint64), ("npairs", C.c_uint32), ("pairs", LinalgPair * 64),
                 ("gemm_rows", C.c_uint64), ("gemm_cols", C.c_uint64), ("int8_gemms", C.c_uint64), ("moduli", C.c_uint64),
                 ("crt_words", C.c_uint32), ("analysis_seconds", C.c_double), ("upload_seconds", C.c_double),
-                ("gpu_seconds", C.c_double), ("assembly_seconds", C.c_double), ("fallback_seconds", C.c_double)]
+                ("gpu_seconds", C.c_double), ("assembly_seconds", C.c_double), ("fallback_seconds", C.c_double),
+                ("digits_seconds", C.c_double), ("product_seconds", C.c_double), ("rebuild_seconds", C.c_double),
+                ("dot_seconds", C.c_double)]
 
     def bands(self, side="left"):
         arr = self.left_bands if side == "left" else self.right_bands

@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(256) cast_kernel(const u32* in, u64 pin, int M
 
 template <int N> hipError_t launch_cast(const u32* in, u64 pin, int M, u32* out, u64 pout, u32 count, int parts, hipStream_t s)
 {
-    hipLaunchKernelGGL((cast_kernel<N>), dim3((count + 255u) / 256u), dim3(256), 0, s, in, pin, M, out, pout, count, parts);
+    hipLaunchKernelGGL((cast_kernel<N>), dim3(lf::grid_blocks(count, 256)), dim3(256), 0, s, in, pin, M, out, pout, count, parts);
     return hipGetLastError();
 }
 

@@ -103,7 +103,7 @@ template <int N> __global__ void __launch_bounds__(256) mul_chain_kernel(ChainAr
 
 template <int N, int OP> inline hipError_t launch_op(const ElemArgs& g, hipStream_t s)
 {
-    const dim3 grid((g.count + 255u) / 256u), block(256);
+    const dim3 grid(grid_blocks(g.count, 256)), block(256);
     hipLaunchKernelGGL((elementwise_kernel<N, OP>), grid, block, 0, s, g);
     return hipGetLastError();
 }
@@ -130,7 +130,7 @@ template <int N> hipError_t launch_elementwise(int op, const ElemArgs& g, hipStr
 template <int N> hipError_t launch_mul_chain(const ChainArgs& g, hipStream_t s)
 {
     if (!g.count) return hipSuccess;
-    const dim3 grid((g.count + 255u) / 256u), block(256);
+    const dim3 grid(grid_blocks(g.count, 256)), block(256);
     hipLaunchKernelGGL((mul_chain_kernel<N>), grid, block, 0, s, g);
     return hipGetLastError();
 }

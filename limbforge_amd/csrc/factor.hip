@@ -158,6 +158,26 @@ int copy_planes(int N, const lf_operand& from, const lf_operand& to, uint64_t co
     return LF_OK;
 }
 
+// A matrix operand of `elements` values: planes present and a pitch that holds them.
+int check_matrix(const lf_operand* o, uint64_t elements, const char* name)
+{
+    if (!o || !o->planes) return fail(LF_INVALID_ARGUMENT, std::string("null operand ") + name);
+    if (o->pitch < elements) return fail(LF_INVALID_ARGUMENT, std::string("operand ") + name + " pitch is smaller than its element count");
+    return LF_OK;
+}
+
+// Two operands of N-word values: the same planes at the same pitch (when `same_ok`), or disjoint memory.
+int check_overlap(int N, const lf_operand& a, const lf_operand& b, bool same_ok, const char* what)
+{
+    if (same_ok && a.planes == b.planes && a.pitch == b.pitch) return LF_OK;
+    const uintptr_t a0 = (uintptr_t)a.planes, a1 = a0 + (N + 3) * a.pitch * 4;
+    const uintptr_t b0 = (uintptr_t)b.planes, b1 = b0 + (N + 3) * b.pitch * 4;
+    if (a0 < b1 && b0 < a1)
+        return fail(LF_INVALID_ARGUMENT, std::string(what) + (same_ok ? ": operands overlap without being the same planes at the same pitch"
+                                                                       : ": the output overlaps an input it does not replace"));
+    return LF_OK;
+}
+
 int check_factor(int bits, uint64_t n, uint64_t nrhs, const lf_factor_options* fo, lf_factor_options& o, const lf_linalg_options* lo,
                  lf_linalg_options& l)
 {
@@ -220,7 +240,11 @@ int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, 
     lf_factor_options fo;
     lf_linalg_options lo;
     if (int r = check_factor(bits, n, 0, options, fo, linalg, lo)) return r;
-    if (n && (!A || !L || !A->planes || !L->planes)) return fail(LF_INVALID_ARGUMENT, "null operand");
+    if (n) {
+        if (int r = check_matrix(A, n * n, "A")) return r;
+        if (int r = check_matrix(L, n * n, "L")) return r;
+        if (int r = check_overlap(bits / 32, *A, *L, true, "cholesky")) return r;
+    }
     lf_cholesky_info ci;
     std::memset(&ci, 0, sizeof ci);
     ci.pivot = n;
@@ -316,7 +340,11 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
     if (int r = check_factor(bits, n, nrhs, options, fo, linalg, lo)) return r;
     if (passes < 1 || passes > 3) return fail(LF_INVALID_ARGUMENT, "solve: passes 1 (L X = B), 2 (L^T X = B) or 3 (both)");
     if (!n || !nrhs) return LF_OK;
-    if (!L || !B || !X || !L->planes || !B->planes || !X->planes) return fail(LF_INVALID_ARGUMENT, "null operand");
+    if (int r = check_matrix(L, n * n, "L")) return r;
+    if (int r = check_matrix(B, n * nrhs, "B")) return r;
+    if (int r = check_matrix(X, n * nrhs, "X")) return r;
+    if (int r = check_overlap(bits / 32, *B, *X, true, "solve")) return r;
+    if (int r = check_overlap(bits / 32, *L, *X, false, "solve")) return r;
     const int N = bits / 32;
     const hipStream_t s = (hipStream_t)stream;
     const uint64_t nb = fo.block ? std::min<uint64_t>(fo.block, n) : n;

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "lf_internal.hpp"
+
 namespace lf {
 
 constexpr int TILE = 64;
@@ -54,14 +56,14 @@ __global__ void __launch_bounds__(256) soa_to_aos_kernel(const uint32_t* __restr
 hipError_t launch_aos_to_soa(const uint32_t* aos, uint32_t* soa, uint64_t pitch, uint32_t count, int record, hipStream_t s)
 {
     if (!count) return hipSuccess;
-    hipLaunchKernelGGL(aos_to_soa_kernel, dim3((count + TILE - 1) / TILE), dim3(256), 0, s, aos, soa, pitch, count, record);
+    hipLaunchKernelGGL(aos_to_soa_kernel, dim3(lf::grid_blocks(count, TILE)), dim3(256), 0, s, aos, soa, pitch, count, record);
     return hipGetLastError();
 }
 
 hipError_t launch_soa_to_aos(const uint32_t* soa, uint32_t* aos, uint64_t pitch, uint32_t count, int record, hipStream_t s)
 {
     if (!count) return hipSuccess;
-    hipLaunchKernelGGL(soa_to_aos_kernel, dim3((count + TILE - 1) / TILE), dim3(256), 0, s, soa, aos, pitch, count, record);
+    hipLaunchKernelGGL(soa_to_aos_kernel, dim3(lf::grid_blocks(count, TILE)), dim3(256), 0, s, soa, aos, pitch, count, record);
     return hipGetLastError();
 }
 

@@ -11,6 +11,9 @@
 
 namespace lf {
 
+// Workgroups for `count` items at `per` items each, summed in 64 bits (counts reach 2^32 - 1).
+inline unsigned grid_blocks(uint64_t count, uint64_t per) { return unsigned((count + per - 1) / per); }
+
 struct ElemArgs {
     const uint32_t* a;
     const uint32_t* b;

@@ -169,7 +169,7 @@ int lf_mixed_graph_create(const lf_chain_bucket* buckets, int nbuckets, const un
         Group& G = grp[lf::mixed_group_of(b.bits / 32)];
         G.args.push_back(args_of(b));
         G.words.push_back(b.bits / 32);
-        G.first.push_back(G.first.back() + uint32_t((b.count + G.block - 1) / G.block));
+        G.first.push_back(G.first.back() + lf::grid_blocks(b.count, G.block));
     }
     std::size_t bytes = 0, off[4][3] = {};
     for (int k = 0; k < 4; ++k) {

@@ -36,7 +36,7 @@ template <int N> hipError_t launch_mixed_chain(const MixedArgs& g, unsigned bloc
 {
     if (!g.count) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((mixed_chain_kernel<N>), dim3((g.count + block - 1) / block), dim3(block), 0, s, g);
+    hipLaunchKernelGGL((mixed_chain_kernel<N>), dim3(lf::grid_blocks(g.count, block)), dim3(block), 0, s, g);
     return hipGetLastError();
 }
 
