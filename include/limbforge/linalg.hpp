@@ -125,19 +125,21 @@ private:
 };
 
 namespace detail {
-// One exact dot product on the GPU: RN(c -/+ sum a[k*sa] b[k*sb]) (c may be null).
-void exact_dot_device(int bits, const void* c, bool subtract, const void* a, std::ptrdiff_t sa, const void* b, std::ptrdiff_t sb,
-                      std::size_t K, void* out);
+// One exact dot product on the host: RN(c -/+ sum a[k*sa] b[k*sb]) (c may be null), records of bits/32 limbs.
+void exact_dot_host(int bits, const void* c, bool subtract, const void* a, std::ptrdiff_t sa, const void* b, std::ptrdiff_t sb,
+                    std::size_t K, void* out);
 } // namespace detail
 
 // RN(c - sum_k a[k*sa]*b[k*sb]) (subtract) or RN(c + sum ...) with one rounding; c == nullptr is a zero
-// addend (the reference's linalg.hpp:268-318 contract, computed on the GPU by the exact-dot kernels).
+// addend (the reference's linalg.hpp:268-318 contract), computed on the host.  Any status among c, a[k], b[k]
+// gives zero with the OR of those statuses; without a nonzero product the result is c itself; exact
+// cancellation gives canonical zero.
 template <int N>
 Number<N> exact_dot_add(const Number<N>* c, bool subtract, const Number<N>* a, std::ptrdiff_t sa, const Number<N>* b, std::ptrdiff_t sb,
                         std::size_t K)
 {
     Number<N> r;
-    detail::exact_dot_device(32 * N, c, subtract, a, sa, b, sb, K, &r);
+    detail::exact_dot_host(32 * N, c, subtract, a, sa, b, sb, K, &r);
     return r;
 }
 // RN(sum_k a[k*sa]*b[k*sb]) with one rounding; empty K and exact cancellation give canonical zero.

@@ -50,7 +50,8 @@ def _hip_jobs() -> list[tuple[Path, list[str]]]:
     jobs = []
     # sources compiled once per limb count (LF_WORDS), and once
     per_width = ("elementwise_w", "gemm_w", "factor_w", "mixed_w")
-    once = ("layout", "capi", "calibrate", "cast", "gemm", "factor", "mixed")
+    # every other source is one translation unit
+    once = sorted(p.stem for p in CSRC.glob("*.hip") if p.stem not in per_width + ("mixed_group",))
     for n in WIDTHS:
         for name in per_width:
             out = BUILD / f"{name}{n}.o"
@@ -118,7 +119,7 @@ def build_hip(jobs: int, incremental: bool = False) -> Path:
 
 def build_cxx(jobs: int) -> Path | None:
     """The C++17 facade (include/limbforge/*.hpp) over the C ABI: liblimbforge.so."""
-    srcs = sorted(HOST.glob("*.cpp"))
+    srcs = sorted(p for p in HOST.glob("*.cpp") if p.stem != "core_host_w")
     if not srcs:
         return None
     BUILD.mkdir(exist_ok=True)
@@ -126,6 +127,12 @@ def build_cxx(jobs: int) -> Path | None:
     cxx = os.environ.get("CXX", "g++")
     flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", "-pthread"]
     work = []
+    # the scalar arithmetic of core.hpp: the GPU kernels' arithmetic core built for the host, per width
+    for n in WIDTHS:
+        out = BUILD / f"host_core{n}.o"
+        work.append((out, [cxx, *flags, "-Wno-unknown-pragmas", "-DLF_HOST_BUILD", f"-DLF_WORDS={n}", f"-I{CSRC}", "-c", str(HOST / "core_host_w.cpp"),
+                           "-o", str(out)]))
+    work.sort(key=lambda j: -int(j[0].stem[len("host_core"):]))
     for s in srcs:
         out = BUILD / f"host_{s.stem}.o"
         work.append((out, [cxx, *flags, "-c", str(s), "-o", str(out)]))

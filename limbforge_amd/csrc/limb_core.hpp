@@ -12,7 +12,7 @@
 // stay rolled and move their windows as shift registers, which keeps code size within the instruction
 // cache at 1024 bits.
 #pragma once
-#include <hip/hip_runtime.h>
+#include "lf_target.hpp"
 #include <stdint.h>
 
 namespace lf {

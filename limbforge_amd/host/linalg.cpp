@@ -170,38 +170,4 @@ Timing Linalg::cholesky_solve(int bits, const void* L, std::size_t n, const void
     return solve(bits, 3, L, n, B, nrhs, X, options, impl->c_options(), "Linalg::cholesky_solve");
 }
 
-namespace detail {
-void exact_dot_device(int bits, const void* c, bool subtract, const void* a, std::ptrdiff_t sa, const void* b, std::ptrdiff_t sb,
-                      std::size_t K, void* out)
-{
-    const std::size_t E = std::size_t(bits / 8 + 12);
-    // gather the strided operands into a 1 x K row and a K x 1 column
-    std::vector<unsigned char> row(std::max<std::size_t>(K, 1) * E), col(std::max<std::size_t>(K, 1) * E), cc(E, 0);
-    for (std::size_t k = 0; k < K; ++k) {
-        std::memcpy(&row[k * E], static_cast<const unsigned char*>(a) + std::ptrdiff_t(k) * sa * std::ptrdiff_t(E), E);
-        std::memcpy(&col[k * E], static_cast<const unsigned char*>(b) + std::ptrdiff_t(k) * sb * std::ptrdiff_t(E), E);
-    }
-    if (c) std::memcpy(cc.data(), c, E);
-    lf_linalg_options o;
-    lf_linalg_default_options(&o);
-    o.force_fallback = 1; // a single output: the exact dot kernels directly
-    if (!c) {
-        check(lf_gemm_host(bits, 0, row.data(), col.data(), 1, 1, K, cc.data(), 0, &o, nullptr, nullptr, nullptr), "exact_dot");
-    } else if (subtract) {
-        check(lf_gemm_host(bits, 0, row.data(), col.data(), 1, 1, K, cc.data(), 1, &o, nullptr, nullptr, nullptr), "exact_dot_add");
-    } else {
-        // c + sum = -( (-c) - sum )
-        int sign;
-        std::memcpy(&sign, &cc[E - 8], 4);
-        sign = -sign;
-        std::memcpy(&cc[E - 8], &sign, 4);
-        check(lf_gemm_host(bits, 0, row.data(), col.data(), 1, 1, K, cc.data(), 1, &o, nullptr, nullptr, nullptr), "exact_dot_add");
-        std::memcpy(&sign, &cc[E - 8], 4);
-        sign = -sign;
-        std::memcpy(&cc[E - 8], &sign, 4);
-    }
-    std::memcpy(out, cc.data(), E);
-}
-} // namespace detail
-
 } // namespace limbforge

@@ -1,6 +1,6 @@
 // Host build of the device arithmetic core (limbforge_amd/csrc/limb_core.hpp) checked against the
 // oracle on the reference tests' case classes, for every width.  Test infrastructure: compiled by
-// tests/test_device_core_cpu.py with tests/cpu_emul on the include path.
+// tests/test_device_core_cpu.py with LF_HOST_BUILD defined.
 #include "limb_core.hpp"
 #include "lf_oracle.h"
 #include "lfo_cases.h"

@@ -1,6 +1,6 @@
-"""The device arithmetic core (limbforge_amd/csrc/limb_core.hpp) built for the host with a test-only
-stand-in for the two AMDGPU builtins it uses (tests/cpu_emul), checked bit for bit against the oracle
-at every width on the reference tests' case classes.  Catches logic errors before any GPU run."""
+"""The device arithmetic core (limbforge_amd/csrc/limb_core.hpp) built for the host (LF_HOST_BUILD, the same
+target the product's host arithmetic uses), checked bit for bit against the oracle at every width on the
+reference tests' case classes.  Catches logic errors before any GPU run."""
 import subprocess
 from pathlib import Path
 
@@ -9,7 +9,7 @@ ROOT = Path(__file__).resolve().parents[1]
 
 def test_device_core_matches_oracle_on_host(tmp_path, oracle):
     exe = tmp_path / "devcore"
-    cmd = ["g++", "-O1", "-std=c++17", f"-I{ROOT/'tests'/'cpu_emul'}", f"-I{ROOT/'limbforge_amd'/'csrc'}",
+    cmd = ["g++", "-O1", "-std=c++17", "-DLF_HOST_BUILD", f"-I{ROOT/'limbforge_amd'/'csrc'}",
            f"-I{ROOT/'oracle'}", "-I/opt/conda/include", str(ROOT / "tests" / "device_core_cpu.cpp"),
            "-x", "c", str(ROOT / "oracle" / "lf_oracle.c"), "-x", "none", "-L/opt/conda/lib",
            "-Wl,-rpath,/opt/conda/lib", "-lgmp", "-o", str(exe)]
