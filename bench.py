@@ -145,29 +145,86 @@ def gemm_inputs(bits: int, n: int, seed: int):
     return mat(), mat()
 
 
-def gemm_section(reps: int, oracle_samples: int):
+def isa_mad_peak() -> float:
+    """The clock-derived issue bound of v_mad_u64_u32 on MI355X: 256 CUs x 64 lanes per clock x 2.4 GHz peak
+    engine clock (one 32x32->64 multiply-add per lane per clock)."""
+    return 256 * 64 * 2.4e9
+
+
+MFMA_INT8_PEAK = 5.0e15  # MI355X dense int8 MFMA, ops/s (AMD's published figure without sparsity)
+
+
+def stage_report(rep) -> dict:
+    return {"analysis_ms": round(rep.analysis_seconds * 1e3, 3), "upload_ms": round(rep.upload_seconds * 1e3, 3),
+            "device_ms": round(rep.gpu_seconds * 1e3, 3), "assembly_ms": round(rep.assembly_seconds * 1e3, 3),
+            "digits_ms": round(rep.digits_seconds * 1e3, 3), "product_ms": round(rep.product_seconds * 1e3, 3),
+            "rebuild_ms": round(rep.rebuild_seconds * 1e3, 3), "dot_ms": round(rep.dot_seconds * 1e3, 3)}
+
+
+def mpfr_gemm_baseline(bits, A, B, m, n, k, budget_s, got=None):
+    """oracle/mpfr_gemm_bench on the same operands: exact MPFR dots (2*bits products + one mpfr_sum) and the
+    reference benchmark's rounded consumer loop, serial and on all allowed cores, on bounded row samples;
+    the exact rows are compared with the GPU's output `got`."""
+    import tempfile
+
+    exe = ROOT / "oracle" / "mpfr_gemm_bench"
+    if not exe.exists():
+        return {"error": "oracle/mpfr_gemm_bench is not built"}
+    threads = host_threads()
+    with tempfile.TemporaryDirectory() as d:
+        src, dst = Path(d) / "in.bin", Path(d) / "out.bin"
+        src.write_bytes(np.concatenate([np.array([bits, m, n, k], np.uint32), A.ravel(), B.ravel()]).astype(np.uint32).tobytes())
+        p = subprocess.run([str(exe), str(src), str(threads), str(budget_s), str(dst)], capture_output=True, text=True, timeout=600)
+        r = json.loads(p.stdout)
+        out = {"threads": threads, "mpfr": r["mpfr"],
+               "exact_serial_ms": round(r["exact_serial_s_per_output"] * m * n * 1e3, 1),
+               "exact_parallel_ms": round(r["exact_parallel_s_per_output"] * m * n * 1e3, 1),
+               "consumer_serial_ms": round(r["consumer_serial_s_per_output"] * m * n * 1e3, 1),
+               "consumer_parallel_ms": round(r["consumer_parallel_s_per_output"] * m * n * 1e3, 1),
+               "sample": (f"exact: {r['exact_serial_rows']} rows serial, {r['exact_parallel_rows']} rows on {threads} threads; "
+                          f"consumer: {r['consumer_serial_rows']} / {r['consumer_parallel_rows']} rows; of {m} rows of {n} outputs, "
+                          "scaled to the whole product")}
+        if got is not None:
+            buf = np.frombuffer(dst.read_bytes(), dtype=np.uint32)
+            rows, rw = int(buf[0]), bits // 32 + 3
+            bad, at = 0, 1
+            for _ in range(rows):
+                i = int(buf[at])
+                want = buf[at + 1:at + 1 + n * rw].reshape(n, rw)
+                bad += int((got[i * n:(i + 1) * n] != want).any(axis=1).sum())
+                at += 1 + n * rw
+            out["mpfr_vs_gpu"] = {"outputs": rows * n, "mismatches": bad}
+    return out
+
+
+def gemm_section(reps: int, oracle_samples: int, cpu_budget: float, peak_tps: float):
     """512-bit exact GEMM C = A B at M = N = K = 512 (three exponent bands per line): the int8-MFMA residue
     GEMM (default options and the one-band lever band_bits = 512) against the plain on-GPU limb GEMM; all
-    three bit-identical, a sample of outputs checked against the oracle."""
+    three bit-identical, a sample of outputs checked against the oracle, the product kernel against the
+    int8 MFMA peak, and MPFR exact dots on the host as the CPU baseline."""
     from limbforge_amd import capi
 
     bits, n = 512, 512
     A, B = gemm_inputs(bits, n, 20261007 + bits)
     res = {}
     outs = {}
+    reports = {}
     for name, kw in (("residue", {}), ("residue_band512", {"band_bits": 512}), ("limb", {"method": 1})):
         o = capi.LinalgOptions.default(**kw)
-        times, walls = [], []
+        times, walls, stages = [], [], []
         r = rep = None
         for i in range(reps if name != "limb" else max(2, reps // 3)):
             r, rep, g, w = capi.gemm(bits, False, A, B, n, n, n, options=o)
             if i:  # first call warms
                 times.append(g)
                 walls.append(w)
+                stages.append(stage_report(rep))
         outs[name] = r
+        reports[name] = rep
+        med = stages[len(stages) // 2]
         res[name] = {"device_ms": round(1e3 * float(np.median(times)), 4), "wall_ms": round(1e3 * float(np.median(walls)), 3),
                      "moduli": int(rep.moduli), "int8_gemms": int(rep.int8_gemms), "gemm_rows": int(rep.gemm_rows),
-                     "band_pairs": int(rep.npairs), "fallback_outputs": int(rep.fallback_outputs)}
+                     "band_pairs": int(rep.npairs), "fallback_outputs": int(rep.fallback_outputs), "stages": med}
         log(f"[bench] gemm {name}: {res[name]}")
     same = all((outs[k] == outs["residue"]).all() for k in outs)
     check = None
@@ -176,19 +233,40 @@ def gemm_section(reps: int, oracle_samples: int):
 
         rng = np.random.default_rng(1)
         rows = rng.choice(n, size=max(1, oracle_samples // n + 1), replace=False)
-        bad = 0
-        for i in rows:
-            want = lf_oracle.gemm(bits, False, A[i * n:(i + 1) * n], B, 1, n, n)
-            bad += int((outs["residue"][i * n:(i + 1) * n] != want).any(axis=1).sum())
-        check = {"rows": len(rows), "outputs": len(rows) * n, "mismatches": bad}
+        want = lf_oracle.gemm_rows(bits, A, B, n, n, n, rows=rows)
+        got = np.concatenate([outs["residue"][i * n:(i + 1) * n] for i in rows])
+        check = {"rows": len(rows), "outputs": len(rows) * n, "mismatches": int((got != want).any(axis=1).sum())}
+    # roofline of the dominant kernel, the int8-MFMA residue product: 4 int8 products (d1 d1, d1 d0, d0 d1,
+    # d0 d0) of 2 ops each per modulus, extended row, extended column and K step, as launched (padded K)
+    rr = reports["residue"]
+    kp = max(64, (n + 63) // 64 * 64)
+    ops = 8 * rr.moduli * rr.gemm_rows * rr.gemm_cols * kp
+    prod_s = res["residue"]["stages"]["product_ms"] * 1e-3
+    roof = {"bound": "mfma", "achieved": round(ops / prod_s / 1e12, 2), "peak": MFMA_INT8_PEAK / 1e12, "unit": "Tops/s (int8)",
+            "frac": round(ops / prod_s / MFMA_INT8_PEAK, 4), "traffic": None, "kernel": "product_kernel (v_mfma_i32_32x32x32_i8)",
+            "kernel_ms": res["residue"]["stages"]["product_ms"], "int8_ops": int(ops)}
+    limb_wp = n * n * n * (bits // 32) ** 2
+    limb_s = res["limb"]["device_ms"] * 1e-3
+    limb_roof = {"bound": "valu", "achieved": round(limb_wp / limb_s / 1e12, 3), "unit": "Tword-products/s",
+                 "frac_of_measured_peak": round(limb_wp / limb_s / peak_tps, 4), "frac_of_isa_peak": round(limb_wp / limb_s / isa_mad_peak(), 4),
+                 "word_products": limb_wp}
+    base = None
+    if cpu_budget > 0:
+        log("[bench] gemm: MPFR CPU baseline")
+        base = mpfr_gemm_baseline(bits, A, B, n, n, n, cpu_budget, outs["residue"])
+        base.update({"value": base.get("exact_parallel_ms"), "unit": "ms", "cores": base.get("threads"), "kind": "port"})
     return {
         "workload": "512-bit exact GEMM C = A B, M = N = K = 512, exponents in bands -200/0/+200 (sigma 8) (config 3)",
         "residue_ms": res["residue"]["device_ms"],
         "limb_ms": res["limb"]["device_ms"],
         "speedup_residue_vs_limb": round(res["limb"]["device_ms"] / res["residue"]["device_ms"], 2),
         "speedup_band512_vs_limb": round(res["limb"]["device_ms"] / res["residue_band512"]["device_ms"], 2),
+        "wall_speedup_residue_vs_limb": round(res["limb"]["wall_ms"] / res["residue"]["wall_ms"], 2),
         "bit_identical_all_paths": bool(same),
         "oracle_check": check,
+        "roofline": roof,
+        "limb_gemm_roofline": limb_roof,
+        "cpu_baseline": base,
         "paths": res,
         "timing": "device time of the product kernels (HIP events), median; wall includes host transfers and band analysis",
     }
@@ -352,26 +430,57 @@ def mixed_section(peak_tps: float, oracle_check: bool, cpu: bool, reps: int = 3)
         "valu_frac_graph": round(work / (graph_ms * 1e-3) / peak_tps, 4),
         "autotuned_workgroup": {str(k): v for k, v in sorted(tuned.items())},
     }
+    res["roofline"] = {"bound": "valu", "achieved": round(work / (graph_ms * 1e-3) / 1e12, 3), "peak": round(peak_tps / 1e12, 3),
+                       "unit": "Tword-products/s", "frac": round(work / (graph_ms * 1e-3) / peak_tps, 4),
+                       "frac_of_isa_peak": round(work / (graph_ms * 1e-3) / isa_mad_peak(), 4), "traffic": None,
+                       "kernel": "mixed_group_kernel<0..3> (one graph, four concurrent group nodes)"}
     if oracle_check or cpu:
+        import tempfile
+
         from oracle import lf_oracle
 
+        # a sample spread over every bucket: the oracle's step sequence checks the GPU, and MPFR (mpfr_fma /
+        # mpfr_mul at each value's precision) is the CPU baseline on the same sample, serial and on all cores
         bad = checked = 0
-        t_cpu = 0.0
         sample_work = 0
+        parts = [np.array([len(buckets)], np.uint32)]
+        order = []
         for b in buckets:
             x, f, g, ln, mk = host[b.bits]
-            sel = np.linspace(0, b.count - 1, num=min(b.count, 400)).astype(np.int64)
-            t0 = time.perf_counter()
+            sel = np.linspace(0, b.count - 1, num=min(b.count, 300)).astype(np.int64)
             want = lf_oracle.mixed_chain(b.bits, x[sel], f[sel], g[sel], ln[sel], mk[sel])
-            t_cpu += time.perf_counter() - t0
-            sample_work += int(ln[sel].astype(np.int64).sum()) * (b.bits // 32) ** 2
             got = b.O.download()[sel]
             bad += int((got != want).any(axis=1).sum())
             checked += sel.size
+            sample_work += int(ln[sel].astype(np.int64).sum()) * (b.bits // 32) ** 2
+            parts += [np.array([b.bits, sel.size], np.uint32), x[sel].ravel(), f[sel].ravel(), g[sel].ravel(),
+                      ln[sel].astype(np.uint32), mk[sel].astype(np.uint64).view(np.uint32)]
+            order.append((b.bits, got))
         res["oracle_check"] = {"values": checked, "mismatches": bad}
-        res["cpu_port"] = {"kind": "port", "cores": 1, "word_products_per_s": round(sample_work / t_cpu, 1),
-                           "values_per_s_at_batch_mix": round(count / (work / (sample_work / t_cpu)), 1),
-                           "sample": f"{checked} values spread over every bucket, the oracle's step sequence on one thread"}
+        exe = ROOT / "oracle" / "mpfr_mixed_bench"
+        if cpu and exe.exists():
+            threads = host_threads()
+            with tempfile.TemporaryDirectory() as d:
+                src, dst = Path(d) / "in.bin", Path(d) / "out.bin"
+                src.write_bytes(np.concatenate([q.astype(np.uint32).ravel() for q in parts]).tobytes())
+                p = subprocess.run([str(exe), str(src), str(threads), "3", str(dst)], capture_output=True, text=True, timeout=600)
+                m = json.loads(p.stdout)
+                outw = np.frombuffer(dst.read_bytes(), dtype=np.uint32)
+                at, mism = 0, 0
+                for bits, got in order:
+                    rw = bits // 32 + 3
+                    want = outw[at:at + got.shape[0] * rw].reshape(got.shape[0], rw)
+                    at += got.shape[0] * rw
+                    mism += int((got != want).any(axis=1).sum())
+            ser = sample_work / m["serial_s"]
+            par = sample_work / m["parallel_s"]
+            res["cpu_baseline"] = {"value": round(count / (work / par), 1), "unit": "values/s at the batch's precision mix",
+                                   "cores": threads, "kind": "port", "mpfr": m["mpfr"],
+                                   "serial_values_per_s": round(count / (work / ser), 1),
+                                   "word_products_per_s": {"serial": round(ser, 1), "parallel": round(par, 1)},
+                                   "sample": f"{checked} values spread over every bucket (mpfr_fma / mpfr_mul per step at each value's "
+                                             "precision, median of 3), scaled to the batch by its word products",
+                                   "mpfr_vs_gpu_mismatches": mism}
     log(f"[bench] mixed: {res}")
     return res
 
@@ -410,13 +519,14 @@ def cholesky_section(oracle_check: bool):
     A = records(rows * n, np.round(8 * rng.standard_normal(rows * n)))
     B = records(n * nrhs, rng.integers(-10, 11, size=n * nrhs))
     S, srep, syrk_g, syrk_w = capi.syrk(bits, A, rows, n, lower_only=False)
+    S0 = S.copy()
     one = np.zeros((n, w + 3), np.uint32)
     one[:, w - 1] = 0x80000000
     one[:, w + 1] = 1
     diag = np.arange(n) * (n + 1)
     S[diag], _, _ = capi.run_host(bits, "add", S[diag], one)
     res = {"syrk": {"device_ms": round(syrk_g * 1e3, 3), "wall_ms": round(syrk_w * 1e3, 3), "moduli": int(srep.moduli),
-                    "band_pairs": int(srep.npairs)}}
+                    "band_pairs": int(srep.npairs), "stages": stage_report(srep)}}
     chol = []
     for _ in range(3):
         t = time.perf_counter()
@@ -446,20 +556,48 @@ def cholesky_section(oracle_check: bool):
     ratio, _, _ = capi.run_host(bits, "div", absval(r), scale)
     res["scaled_residual"] = max(abs(to_float(q, bits)) for q in ratio)
     log(f"[bench] cholesky: {res}")
+    # roofline of the SYRK's int8-MFMA residue product (lower-triangle tiles only)
+    kp = (rows + 63) // 64 * 64
+    tiles = (srep.gemm_rows + 127) // 128
+    ops = 8 * srep.moduli * 128 * 128 * (tiles * (tiles + 1) // 2) * kp
+    res["roofline"] = {"bound": "mfma", "achieved": round(ops / srep.product_seconds / 1e12, 2), "peak": MFMA_INT8_PEAK / 1e12,
+                       "unit": "Tops/s (int8)", "frac": round(ops / srep.product_seconds / MFMA_INT8_PEAK, 4), "traffic": None,
+                       "kernel": "product_kernel (SYRK, lower-triangle 128 x 128 tiles)",
+                       "kernel_ms": round(srep.product_seconds * 1e3, 3), "int8_ops": int(ops),
+                       "note": "the factorization itself is latency-bound (single-wave exact dots, square roots and divisions per column)"}
     if oracle_check:
         from oracle import lf_oracle
 
+        # the oracle's own S = A^T A (every entry, on all host threads) + I, then the oracle's Cholesky and
+        # solves of it in the reference's fixed host order, serially and on all host threads
+        threads = host_threads()
+        At = np.ascontiguousarray(A.reshape(rows, n, w + 3).transpose(1, 0, 2)).reshape(n * rows, w + 3)
         t = time.perf_counter()
-        WL, piv, _, _ = lf_oracle.cholesky(bits, S, n, block)
-        t_chol = time.perf_counter() - t
-        t = time.perf_counter()
-        WX = lf_oracle.solve(bits, WL, n, B, nrhs, block, 3)
-        t_solve = time.perf_counter() - t
-        res["oracle_check"] = {"factor_mismatches": int((WL != L).any(axis=1).sum()), "solve_mismatches": int((WX != X).any(axis=1).sum()),
-                               "pivot": int(piv)}
-        res["cpu_host_path"] = {"cholesky_s": round(t_chol, 3), "solve_s": round(t_solve, 3), "cores": 1,
-                                "kind": "port", "note": "the oracle's restatement of the reference's fixed-order host path, one thread"}
-        log(f"[bench] cholesky oracle check: {res['oracle_check']}, {res['cpu_host_path']}")
+        WS = lf_oracle.gemm_rows(bits, At, A, n, n, rows)
+        t_syrk = time.perf_counter() - t
+        syrk_bad = int((WS != S0).any(axis=1).sum())
+        WS[diag] = lf_oracle.elementwise(bits, 0, WS[diag], one)
+        times = {}
+        for th in (1, threads):
+            lf_oracle.set_threads(th)
+            t = time.perf_counter()
+            WL, piv, _, _ = lf_oracle.cholesky(bits, WS, n, block)
+            t_chol = time.perf_counter() - t
+            t = time.perf_counter()
+            WX = lf_oracle.solve(bits, WL, n, B, nrhs, block, 3)
+            times[th] = (t_chol, time.perf_counter() - t)
+        lf_oracle.set_threads(1)
+        res["oracle_check"] = {"syrk_mismatches": syrk_bad, "factor_mismatches": int((WL != L).any(axis=1).sum()),
+                               "solve_mismatches": int((WX != X).any(axis=1).sum()), "pivot": int(piv),
+                               "from": "the oracle's own S (SYRK of every entry, then + I)"}
+        res["cpu_baseline"] = {"value": round((times[threads][0] + times[threads][1]) * 1e3, 1), "unit": "ms (Cholesky + solve)",
+                               "cores": threads, "kind": "port",
+                               "cholesky_s": {"serial": round(times[1][0], 3), "parallel": round(times[threads][0], 3)},
+                               "solve_s": {"serial": round(times[1][1], 3), "parallel": round(times[threads][1], 3)},
+                               "syrk_s_parallel": round(t_syrk, 3),
+                               "sample": "the whole workload: the oracle's restatement of the reference's fixed-order host path "
+                                         "(panel rows, trailing-update rows and right-hand sides on the host threads)"}
+        log(f"[bench] cholesky oracle check: {res['oracle_check']}, {res['cpu_baseline']}")
     res["workload"] = ("256-bit SYRK (A 2048 x 512, single band) + blocked Cholesky n = 512 (block 32) + cholesky_solve "
                        "with 64 right-hand sides (config 4)")
     return res
@@ -534,6 +672,9 @@ def main(argv=None) -> int:
         "kernel": "mul_chain_kernel<32>",
         "kernel_ms": round(kernel_ms, 5),
         "peak_source": "measured: lf_calibrate_mad64 (8 independent v_mad_u64_u32 chains per lane, 16384 workgroups)",
+        # the clock-derived issue bound beside the measured one: 256 CUs x 64 lanes x 2.4 GHz peak engine clock
+        "isa_peak": round(isa_mad_peak() / 1e12, 4),
+        "frac_of_isa_peak": round(achieved * 1e12 / isa_mad_peak(), 4),
         # achieved counts the schoolbook's N^2 32x32 word products per multiplication (the reference's
         # algorithm); the kernel issues fewer v_mad_u64_u32: a short product of 29-bit digits
         "executed_mad64_per_multiplication": 735,
@@ -555,7 +696,7 @@ def main(argv=None) -> int:
     gemm = None
     if not a.no_gemm:
         log("[bench] 512-bit exact GEMM 512^3")
-        gemm = gemm_section(a.gemm_reps, 0 if a.no_parity else 256)
+        gemm = gemm_section(a.gemm_reps, 0 if a.no_parity else 1024, 0 if a.no_cpu_baseline else a.cpu_budget / 3, rate.value)
 
     ew = None
     if not a.no_elementwise:

@@ -5,7 +5,7 @@
 //   for every nonzero value, exponent in [-1e9, 1e9], sign in {-1, 0, 1}, canonical zero (all fields 0 except
 //   the status bitmask).  Every operation rounds to nearest, ties to even; arithmetic failures are status bits
 //   in the result (division by zero 1, exponent overflow 2, invalid 4), never exceptions.
-// The scalar functions below (add ... cdiv) run on the host, in liblimbforge.so: they are the arithmetic core
+// The scalar functions below (add ... cdiv, dot2_add, cfma, cfms) run on the host, in liblimbforge.so: they are the arithmetic core
 // of the GPU kernels built for the CPU, so a host result is the same bits as the GPU's for the same operands.
 // N is the limb count, 2..32 (64 to 1024 bits).
 #include <cstdint>
@@ -80,8 +80,11 @@ inline WordDivision divide_word(word high, word low, word d, word inv)
 
 namespace detail {
 // The host arithmetic of liblimbforge.so: out = op(a, b, c) on records of `words` limbs, op numbered as
-// limbforge::Operation (add 0, sub 1, mul 2, div 3, complex add/mul/div 4-6, square 7, sqrt 8, fma 9, fms 10).
+// limbforge::Operation (add 0, sub 1, mul 2, div 3, complex add/mul/div 4-6, square 7, sqrt 8, fma 9, fms 10,
+// complex fma / fms 11-12).
 void host_arithmetic(int op, int words, const void* a, const void* b, const void* c, void* out);
+// out = RN(a b + c d + e), one rounding, on records of `words` limbs
+void host_dot2_add(int words, const void* a, const void* b, const void* c, const void* d, const void* e, void* out);
 template <int N> inline Number<N> host_real(int op, const Number<N>& a, const Number<N>& b, const Number<N>& c)
 {
     static_assert(N >= 2 && N <= 32, "limb count must be in [2, 32]");
@@ -89,11 +92,11 @@ template <int N> inline Number<N> host_real(int op, const Number<N>& a, const Nu
     host_arithmetic(op, N, &a, &b, &c, &r);
     return r;
 }
-template <int N> inline Complex<N> host_complex(int op, const Complex<N>& a, const Complex<N>& b)
+template <int N> inline Complex<N> host_complex(int op, const Complex<N>& a, const Complex<N>& b, const Complex<N>* c = nullptr)
 {
     static_assert(N >= 2 && N <= 32, "limb count must be in [2, 32]");
     Complex<N> r;
-    host_arithmetic(op, N, &a, &b, nullptr, &r);
+    host_arithmetic(op, N, &a, &b, c, &r);
     return r;
 }
 } // namespace detail
@@ -127,4 +130,16 @@ template <int N> inline Number<N> fms(const Number<N>& a, const Number<N>& b, co
 template <int N> inline Complex<N> cadd(Complex<N> a, Complex<N> b) { return detail::host_complex<N>(4, a, b); }
 template <int N> inline Complex<N> cmul(Complex<N> a, Complex<N> b) { return detail::host_complex<N>(5, a, b); }
 template <int N> inline Complex<N> cdiv(Complex<N> a, Complex<N> b) { return detail::host_complex<N>(6, a, b); }
+// RN(a*b + c*d + e) with one rounding.
+template <int N>
+inline Number<N> dot2_add(const Number<N>& a, const Number<N>& b, const Number<N>& c, const Number<N>& d, const Number<N>& e)
+{
+    static_assert(N >= 2 && N <= 32, "limb count must be in [2, 32]");
+    Number<N> r;
+    detail::host_dot2_add(N, &a, &b, &c, &d, &e, &r);
+    return r;
+}
+// Complex a*b + c and a*b - c, each part one rounding of its exact value.
+template <int N> inline Complex<N> cfma(Complex<N> a, Complex<N> b, Complex<N> c) { return detail::host_complex<N>(11, a, b, &c); }
+template <int N> inline Complex<N> cfms(Complex<N> a, Complex<N> b, Complex<N> c) { return detail::host_complex<N>(12, a, b, &c); }
 } // namespace limbforge

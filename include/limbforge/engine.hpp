@@ -7,8 +7,8 @@
 // arrays, make_buffer, batch, prewarm, pipeline_info, device_name), with the same element layout
 // (bits/8 + 12 bytes per real value), argument meaning and exceptions.  Behind it is the C ABI of
 // liblimbforge_hip.so (include/limbforge_hip.h): resident buffers live in HBM as limb planes and
-// batches are ordered launches on one HIP stream.  Recurrences, tree sums, segmented dots and batched
-// 4x4 systems of the reference engine are not part of this build.
+// batches are ordered launches on one HIP stream.  Tree sums and segmented dot products are provided;
+// the reference engine's vector recurrences and batched 4x4 systems are not part of this build.
 #include "core.hpp"
 #include <cstddef>
 #include <cstdint>
@@ -32,6 +32,13 @@ inline bool operation_is_valid(Operation op) { return int(op) >= 0 && int(op) <=
 
 struct Timing {
     double gpu_seconds, wall_seconds;
+};
+
+// Segmented dot products: out[i] = the sequential chain s <- fma(a[i*length + k], b[i*length + k], s) from
+// s = 0 (b[k] for every segment with shared_right; cfma per component for complex values).
+struct SegmentedDot {
+    std::size_t segments = 0, length = 0;
+    bool shared_right = false;
 };
 
 // Broadcast read of an operand: element i uses index (i / stride) % period (period 0: no wrap).
@@ -154,6 +161,10 @@ class CommandBatch {
                 const std::shared_ptr<detail::BufferStorage>& out, std::size_t count, Broadcast b_index = {}, Broadcast c_index = {});
     void encode_cast(int from_bits, int to_bits, bool complex, const std::shared_ptr<detail::BufferStorage>& in,
                      const std::shared_ptr<detail::BufferStorage>& out, std::size_t count);
+    void encode_tree_sum(int bits, bool complex, const std::shared_ptr<detail::BufferStorage>& input,
+                         const std::shared_ptr<detail::BufferStorage>& out, std::size_t count);
+    void encode_segmented_dot(int bits, bool complex, const SegmentedDot& shape, const std::shared_ptr<detail::BufferStorage>& a,
+                              const std::shared_ptr<detail::BufferStorage>& b, const std::shared_ptr<detail::BufferStorage>& out);
     friend class Engine;
     friend struct detail::BatchAccess;
 
@@ -205,6 +216,23 @@ public:
         if (in.size() != out.size()) throw std::invalid_argument("buffer counts must match");
         encode_cast(detail::Format<From>::bits, detail::Format<To>::bits, detail::Format<To>::complex, in.storage_, out.storage_, in.size());
     }
+    // out (one element) = the fixed adjacent-pair tree of rounded additions of input (docs: pairs (0,1), (2,3),
+    // ... level by level, an unpaired last value carried unchanged; empty input gives canonical zero).
+    template <class T> void tree_sum(const Buffer<T>& input, Buffer<T>& out)
+    {
+        static_assert(detail::Format<T>::bits, "tree sums reduce arithmetic buffers");
+        if (out.size() != 1) throw std::invalid_argument("tree_sum output must contain one element");
+        encode_tree_sum(detail::Format<T>::bits, detail::Format<T>::complex, input.storage_, out.storage_, input.size());
+    }
+    template <class T> void segmented_dot(const SegmentedDot& shape, const Buffer<T>& a, const Buffer<T>& b, Buffer<T>& out)
+    {
+        static_assert(detail::Format<T>::bits, "segmented dots take arithmetic buffers");
+        if (shape.length && shape.segments > std::size_t(-1) / shape.length) throw std::invalid_argument("segmented dot shape overflows");
+        if (a.size() < shape.segments * shape.length || b.size() < (shape.shared_right ? shape.length : shape.segments * shape.length) ||
+            out.size() < shape.segments)
+            throw std::invalid_argument("buffer counts do not match the segmented dot shape");
+        encode_segmented_dot(detail::Format<T>::bits, detail::Format<T>::complex, shape, a.storage_, b.storage_, out.storage_);
+    }
     // Single use: the encoded operations run in order on the engine's stream.  Consecutive
     // run(mul, x, y, x) steps on the same buffers are fused into one resident chain launch (identical
     // results: every step is still rounded).
@@ -239,6 +267,8 @@ public:
                        Broadcast b_index = {}, Broadcast c_index = {});
     // Host-array precision cast between any two supported widths (complex: componentwise).
     Timing cast(int from_bits, int to_bits, bool complex, const void* in, void* out, std::size_t count);
+    // Host-array segmented dot products (SegmentedDot); out holds shape.segments values.
+    Timing segmented_dot(int bits, bool complex, const SegmentedDot& shape, const void* a, const void* b, void* out);
 
 private:
     friend class CommandBatch;

@@ -33,11 +33,12 @@ extern "C" {
 #define LF_LOGIC_ERROR 2
 #define LF_RUNTIME_ERROR 3
 
-/* Operation numbering of limbforge::Operation (engine.hpp:13). complex_fma/complex_fms (11, 12)
- * are not provided by this build. */
+/* Operation numbering of limbforge::Operation (engine.hpp:13).  complex_fma / complex_fms compute each part
+ * of a*b + c (a*b - c) with one rounding of its exact value (core.hpp:330-358). */
 enum {
     LF_OP_ADD = 0, LF_OP_SUB = 1, LF_OP_MUL = 2, LF_OP_DIV = 3, LF_OP_COMPLEX_ADD = 4, LF_OP_COMPLEX_MUL = 5,
-    LF_OP_COMPLEX_DIV = 6, LF_OP_SQUARE = 7, LF_OP_SQRT = 8, LF_OP_FMA = 9, LF_OP_FMS = 10
+    LF_OP_COMPLEX_DIV = 6, LF_OP_SQUARE = 7, LF_OP_SQRT = 8, LF_OP_FMA = 9, LF_OP_FMS = 10, LF_OP_COMPLEX_FMA = 11,
+    LF_OP_COMPLEX_FMS = 12
 };
 
 /* A device operand: limb planes and their pitch (uint32 elements per plane). */
@@ -129,6 +130,22 @@ int lf_cast(int from_bits, int to_bits, int complex_values, const lf_operand* in
 int lf_cast_host(int from_bits, int to_bits, int complex_values, const void* in, void* out, uint64_t count,
                  double* gpu_seconds, double* wall_seconds);
 
+/* ---- reductions ---- */
+/* Segmented dot products: Engine::segmented_dot / CommandBatch::segmented_dot (engine.hpp:177-180, 205; kernels
+ * segmented_dot_real / segmented_dot_complex, kernels.metal:197-208).  out[i] for i < segments is the fixed
+ * sequential chain s <- fma(a[i*length + k], b[i*length + k], s) from s = 0 (b[k] with shared_right; cfma per
+ * component for complex values), one rounding per step; length 0 gives zero.  Device form on limb planes
+ * (a holds segments*length values, b length or segments*length) and host-array form. */
+int lf_segmented_dot(int bits, int complex_values, uint64_t segments, uint64_t length, int shared_right, const lf_operand* a,
+                     const lf_operand* b, const lf_operand* out, void* stream);
+int lf_segmented_dot_host(int bits, int complex_values, uint64_t segments, uint64_t length, int shared_right, const void* a,
+                          const void* b, void* out, double* gpu_seconds, double* wall_seconds);
+/* Adjacent-pair tree sum: CommandBatch::tree_sum (engine.hpp:150-153; kernels tree_sum_* and global_tree_sum_*,
+ * kernels.metal:61-112).  out (one value) = the fixed tree of rounded additions of pairs (0,1), (2,3), ... with
+ * an unpaired last value carried unchanged, level by level; complex values componentwise; count 0 gives
+ * canonical zero.  Partial sums use stream-ordered temporary device memory. */
+int lf_tree_sum(int bits, int complex_values, const lf_operand* in, uint64_t count, const lf_operand* out, void* stream);
+
 /* ---- exact dense products (include/limbforge/linalg.hpp) ---- */
 
 /* LinalgOptions (linalg.hpp:18-25) plus `method`: 0 = exponent-band residue GEMM on int8 MFMA with CRT
@@ -150,7 +167,8 @@ typedef struct lf_linalg_pair {
 
 /* LinalgReport (linalg.hpp:33-45): band histograms per side ([b] = lines with exactly b bands; index 0:
  * zero or status lines; band_slots = max_bands + 1 entries, at most 65), fallback lines and outputs,
- * band pairs (first 64), extended GEMM sizes, moduli and int8 GEMMs (three per modulus), stage times. */
+ * band pairs (first 64), extended GEMM sizes, moduli and int8 GEMMs (three per modulus), stage times
+ * (host analysis, uploads, device time, assembly, and the device time per kernel stage). */
 typedef struct lf_linalg_report {
     uint64_t m, n, k;
     uint32_t band_slots;
@@ -162,6 +180,9 @@ typedef struct lf_linalg_report {
     uint64_t gemm_rows, gemm_cols, int8_gemms, moduli;
     uint32_t crt_words;
     double analysis_seconds, upload_seconds, gpu_seconds, assembly_seconds, fallback_seconds;
+    /* device time of each stage inside gpu_seconds (no reference counterpart): residue digits of both
+     * operands, the int8-MFMA residue products, the CRT rebuild and rounding, the exact-dot kernels */
+    double digits_seconds, product_seconds, rebuild_seconds, dot_seconds;
 } lf_linalg_report;
 
 void lf_linalg_default_options(lf_linalg_options* options);

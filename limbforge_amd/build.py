@@ -49,7 +49,7 @@ def _run(cmd: list[str]) -> tuple[list[str], int, str]:
 def _hip_jobs() -> list[tuple[Path, list[str]]]:
     jobs = []
     # sources compiled once per limb count (LF_WORDS), and once
-    per_width = ("elementwise_w", "gemm_w", "factor_w", "mixed_w")
+    per_width = ("elementwise_w", "gemm_w", "factor_w", "mixed_w", "reduce_w")
     # every other source is one translation unit
     once = sorted(p.stem for p in CSRC.glob("*.hip") if p.stem not in per_width + ("mixed_group",))
     for n in WIDTHS:

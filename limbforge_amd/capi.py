@@ -21,11 +21,11 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "liblimbforge_hip.so"
 
 OPS = {
     "add": 0, "sub": 1, "mul": 2, "div": 3, "complex_add": 4, "complex_mul": 5, "complex_div": 6,
-    "square": 7, "sqrt": 8, "fma": 9, "fms": 10,
+    "square": 7, "sqrt": 8, "fma": 9, "fms": 10, "complex_fma": 11, "complex_fms": 12,
 }
-COMPLEX_OPS = {4, 5, 6}
+COMPLEX_OPS = {4, 5, 6, 11, 12}
 UNARY_OPS = {7, 8}
-TERNARY_OPS = {9, 10}
+TERNARY_OPS = {9, 10, 11, 12}
 
 
 class Operand(C.Structure):
@@ -263,6 +263,54 @@ def run_host(bits: int, op, a: np.ndarray, b: np.ndarray | None = None, c: np.nd
                             cc.ctypes.data if cc is not None else None, out.ctypes.data, count,
                             _bcast(b_index), _bcast(c_index), C.byref(g), C.byref(w)))
     return out, g.value, w.value
+
+
+def _reduce_sigs(L):
+    if getattr(L, "_reduce", False):
+        return
+    P = C.POINTER
+    L.lf_segmented_dot.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_uint64, C.c_int, P(Operand), P(Operand), P(Operand), C.c_void_p]
+    L.lf_segmented_dot.restype = C.c_int
+    L.lf_segmented_dot_host.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        P(C.c_double), P(C.c_double)]
+    L.lf_segmented_dot_host.restype = C.c_int
+    L.lf_tree_sum.argtypes = [C.c_int, C.c_int, P(Operand), C.c_uint64, P(Operand), C.c_void_p]
+    L.lf_tree_sum.restype = C.c_int
+    L._reduce = True
+
+
+def segmented_dot(bits: int, segments: int, length: int, a: Buffer, b: Buffer, out: Buffer, shared_right=False, stream=None) -> None:
+    """CommandBatch::segmented_dot (engine.hpp:177-180) on resident buffers."""
+    L = lib()
+    _reduce_sigs(L)
+    oa, ob, oo = a.operand, b.operand, out.operand
+    check(L.lf_segmented_dot(bits, int(out.complex), segments, length, int(shared_right), C.byref(oa), C.byref(ob), C.byref(oo),
+                             stream))
+
+
+def segmented_dot_host(bits: int, complex_values: bool, segments: int, length: int, a: np.ndarray, b: np.ndarray,
+                       shared_right=False):
+    """Engine::segmented_dot (engine.hpp:205) on host arrays: returns (out, gpu_seconds, wall_seconds)."""
+    L = lib()
+    _reduce_sigs(L)
+    rw = record_words(bits, complex_values)
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    if a.shape != (segments * length, rw) or b.shape != ((length if shared_right else segments * length), rw):
+        raise ValueError("operand shapes do not match the segmented dot shape")
+    out = np.zeros((segments, rw), np.uint32)
+    g, w = C.c_double(), C.c_double()
+    check(L.lf_segmented_dot_host(bits, int(complex_values), segments, length, int(shared_right), a.ctypes.data, b.ctypes.data,
+                                  out.ctypes.data, C.byref(g), C.byref(w)))
+    return out, g.value, w.value
+
+
+def tree_sum(bits: int, x: Buffer, count: int, out: Buffer, stream=None) -> None:
+    """CommandBatch::tree_sum (engine.hpp:150-153) on resident buffers."""
+    L = lib()
+    _reduce_sigs(L)
+    ox, oo = x.operand, out.operand
+    check(L.lf_tree_sum(bits, int(out.complex), C.byref(ox), count, C.byref(oo), stream))
 
 
 class Stream:

@@ -3,11 +3,13 @@
 // host-array entry that stages AoS arrays through device limb planes.
 #include "limbforge_hip.h"
 #include "lf_internal.hpp"
+#include "host_run.hpp"
 
 #include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 
 namespace lf {
@@ -45,9 +47,13 @@ int device_error(hipError_t e, const char* where)
 
 bool valid_bits(int bits) { return bits >= 64 && bits <= 1024 && bits % 32 == 0; }
 
-bool is_complex(int op) { return op == LF_OP_COMPLEX_ADD || op == LF_OP_COMPLEX_MUL || op == LF_OP_COMPLEX_DIV; }
+bool is_complex(int op)
+{
+    return op == LF_OP_COMPLEX_ADD || op == LF_OP_COMPLEX_MUL || op == LF_OP_COMPLEX_DIV || op == LF_OP_COMPLEX_FMA ||
+           op == LF_OP_COMPLEX_FMS;
+}
 bool is_unary(int op) { return op == LF_OP_SQUARE || op == LF_OP_SQRT; }
-bool is_ternary(int op) { return op == LF_OP_FMA || op == LF_OP_FMS; }
+bool is_ternary(int op) { return op == LF_OP_FMA || op == LF_OP_FMS || op == LF_OP_COMPLEX_FMA || op == LF_OP_COMPLEX_FMS; }
 
 // broadcast_elements (engine.hpp:22-25)
 uint64_t broadcast_elements(uint64_t count, const lf_broadcast* b)
@@ -122,6 +128,7 @@ struct Staging {
     }
 };
 thread_local Staging staging;
+
 } // namespace
 
 extern "C" {
@@ -341,7 +348,6 @@ int lf_elementwise(int bits, int op, const lf_operand* a, const lf_operand* b, c
     if (!valid_bits(bits)) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
     if (count > std::numeric_limits<uint32_t>::max()) return fail(LF_INVALID_ARGUMENT, "batch exceeds 32-bit indexing");
     if (op < 0 || op > 12) return fail(LF_INVALID_ARGUMENT, "invalid operation");
-    if (op > LF_OP_FMS) return fail(LF_INVALID_ARGUMENT, "complex_fma / complex_fms are not provided by this build");
     if (int r = check_broadcast(b_index)) return r;
     if (int r = check_broadcast(c_index)) return r;
     if (!count) return LF_OK;
@@ -453,49 +459,15 @@ int lf_run_host(int bits, int op, const void* a, const void* b, const void* c, v
     const auto start = std::chrono::steady_clock::now();
     if (gpu_seconds) *gpu_seconds = 0;
     if (wall_seconds) *wall_seconds = 0;
-    if (!valid_bits(bits)) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
-    if (count > std::numeric_limits<uint32_t>::max()) return fail(LF_INVALID_ARGUMENT, "batch exceeds 32-bit indexing");
-    if (op < 0 || op > 12) return fail(LF_INVALID_ARGUMENT, "invalid operation");
-    if (op > LF_OP_FMS) return fail(LF_INVALID_ARGUMENT, "complex_fma / complex_fms are not provided by this build");
-    if (int r = check_broadcast(b_index)) return r;
-    if (int r = check_broadcast(c_index)) return r;
+    lf::HostRun R;
+    if (int r = lf::plan_host_run(bits, op, a, b, c, out, count, b_index, c_index, R)) return r;
     if (!count) return LF_OK;
-    if (!a || !out || (!is_unary(op) && !b) || (is_ternary(op) && !c)) return fail(LF_INVALID_ARGUMENT, "null host array");
-    const int parts = is_complex(op) ? 2 : 1;
-    const uint64_t E = (uint64_t)(bits / 8 + 12) * parts;
-    const uint64_t nb = is_unary(op) ? 0 : broadcast_elements(count, b_index);
-    const uint64_t nc = is_ternary(op) ? broadcast_elements(count, c_index) : 0;
     Staging& st = staging;
     if (!st.stream) LF_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : st.ev)
         if (!e) LF_HIP(hipEventCreate(&e), "hipEventCreate");
-    const uint64_t pa = lf_plane_pitch(count), pb = lf_plane_pitch(nb), pc = lf_plane_pitch(nc);
-    const uint64_t aos_bytes = count * E;
-    LF_HIP(st.reserve(0, aos_bytes), "hipMalloc (staging)");
-    LF_HIP(st.reserve(1, lf_planes_bytes(bits, parts == 2, count)), "hipMalloc (staging)");
-    LF_HIP(st.reserve(2, nb ? lf_planes_bytes(bits, parts == 2, nb) : 4), "hipMalloc (staging)");
-    LF_HIP(st.reserve(3, nc ? lf_planes_bytes(bits, parts == 2, nc) : 4), "hipMalloc (staging)");
-    LF_HIP(st.reserve(4, lf_planes_bytes(bits, parts == 2, count)), "hipMalloc (staging)");
-    hipStream_t s = st.stream;
-    lf_operand A{st.dev[1], pa}, B{st.dev[2], pb}, C{st.dev[3], pc}, O{st.dev[4], pa};
-    const int cplx = parts == 2;
-    // stage in: AoS bytes to the device, then transposed into planes
-    LF_HIP(hipMemcpyAsync(st.dev[0], a, aos_bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-    if (int r = lf_aos_to_planes(bits, cplx, st.dev[0], &A, count, s)) return r;
-    if (nb) {
-        LF_HIP(hipMemcpyAsync(st.dev[0], b, nb * E, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-        if (int r = lf_aos_to_planes(bits, cplx, st.dev[0], &B, nb, s)) return r;
-    }
-    if (nc) {
-        LF_HIP(hipMemcpyAsync(st.dev[0], c, nc * E, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-        if (int r = lf_aos_to_planes(bits, cplx, st.dev[0], &C, nc, s)) return r;
-    }
-    LF_HIP(hipEventRecord(st.ev[0], s), "hipEventRecord");
-    if (int r = lf_elementwise(bits, op, &A, nb ? &B : &A, nc ? &C : &A, &O, count, b_index, c_index, s)) return r;
-    LF_HIP(hipEventRecord(st.ev[1], s), "hipEventRecord");
-    if (int r = lf_planes_to_aos(bits, cplx, &O, st.dev[0], count, s)) return r;
-    LF_HIP(hipMemcpyAsync(out, st.dev[0], aos_bytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-    LF_HIP(hipStreamSynchronize(s), "device execution");
+    for (int k = 0; k < 5; ++k) LF_HIP(st.reserve(k, R.bytes[k]), "hipMalloc (staging)");
+    if (int r = lf::stage_and_run(R, a, b, c, out, st.dev, st.stream, st.ev)) return r;
     float ms = 0;
     LF_HIP(hipEventElapsedTime(&ms, st.ev[0], st.ev[1]), "hipEventElapsedTime");
     if (gpu_seconds) *gpu_seconds = ms * 1e-3;
@@ -504,3 +476,61 @@ int lf_run_host(int bits, int op, const void* a, const void* b, const void* c, v
 }
 
 } // extern "C"
+
+namespace lf {
+int set_error(int code, const char* message) { return fail(code, message); }
+
+int plan_host_run(int bits, int op, const void* a, const void* b, const void* c, void* out, uint64_t count, const lf_broadcast* b_index,
+                  const lf_broadcast* c_index, HostRun& R)
+{
+    if (!valid_bits(bits)) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    if (count > std::numeric_limits<uint32_t>::max()) return fail(LF_INVALID_ARGUMENT, "batch exceeds 32-bit indexing");
+    if (op < 0 || op > 12) return fail(LF_INVALID_ARGUMENT, "invalid operation");
+    if (int r = check_broadcast(b_index)) return r;
+    if (int r = check_broadcast(c_index)) return r;
+    if (!count) return LF_OK;
+    if (!a || !out || (!is_unary(op) && !b) || (is_ternary(op) && !c)) return fail(LF_INVALID_ARGUMENT, "null host array");
+    R.bits = bits;
+    R.op = op;
+    R.cplx = is_complex(op) ? 1 : 0;
+    R.count = count;
+    R.E = (uint64_t)(bits / 8 + 12) * (R.cplx ? 2 : 1);
+    R.nb = is_unary(op) ? 0 : broadcast_elements(count, b_index);
+    R.nc = is_ternary(op) ? broadcast_elements(count, c_index) : 0;
+    R.b_index = b_index;
+    R.c_index = c_index;
+    R.bytes[0] = count * R.E;
+    R.bytes[1] = lf_planes_bytes(bits, R.cplx, count);
+    R.bytes[2] = R.nb ? lf_planes_bytes(bits, R.cplx, R.nb) : 4;
+    R.bytes[3] = R.nc ? lf_planes_bytes(bits, R.cplx, R.nc) : 4;
+    R.bytes[4] = lf_planes_bytes(bits, R.cplx, count);
+    return LF_OK;
+}
+
+int stage_and_run(const HostRun& R, const void* a, const void* b, const void* c, void* out, void* const* dev, hipStream_t s,
+                  const hipEvent_t* ev)
+{
+    const int bits = R.bits, cplx = R.cplx;
+    const uint64_t count = R.count, nb = R.nb, nc = R.nc, E = R.E;
+    lf_operand A{dev[1], lf_plane_pitch(count)}, B{dev[2], lf_plane_pitch(nb)}, C{dev[3], lf_plane_pitch(nc)}, O{dev[4], lf_plane_pitch(count)};
+    // stage in: AoS bytes to the device, then transposed into planes
+    LF_HIP(hipMemcpyAsync(dev[0], a, count * E, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    if (int r = lf_aos_to_planes(bits, cplx, dev[0], &A, count, s)) return r;
+    if (nb) {
+        LF_HIP(hipMemcpyAsync(dev[0], b, nb * E, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+        if (int r = lf_aos_to_planes(bits, cplx, dev[0], &B, nb, s)) return r;
+    }
+    if (nc) {
+        LF_HIP(hipMemcpyAsync(dev[0], c, nc * E, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+        if (int r = lf_aos_to_planes(bits, cplx, dev[0], &C, nc, s)) return r;
+    }
+    if (ev) LF_HIP(hipEventRecord(ev[0], s), "hipEventRecord");
+    if (int r = lf_elementwise(bits, R.op, &A, nb ? &B : &A, nc ? &C : &A, &O, count, R.b_index, R.c_index, s)) return r;
+    if (ev) LF_HIP(hipEventRecord(ev[1], s), "hipEventRecord");
+    if (int r = lf_planes_to_aos(bits, cplx, &O, dev[0], count, s)) return r;
+    LF_HIP(hipMemcpyAsync(out, dev[0], count * E, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    LF_HIP(hipStreamSynchronize(s), "device execution");
+    return LF_OK;
+}
+
+} // namespace lf

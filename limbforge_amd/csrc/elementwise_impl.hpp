@@ -4,12 +4,13 @@
 // operands are loaded plane by plane (coalesced), computed in VGPRs and stored plane by plane.
 #pragma once
 #include "lf_internal.hpp"
+#include "dot2_core.hpp"
 #include "limb_core.hpp"
 
 namespace lf {
 
 constexpr int OP_ADD = 0, OP_SUB = 1, OP_MUL = 2, OP_DIV = 3, OP_CADD = 4, OP_CMUL = 5, OP_CDIV = 6, OP_SQUARE = 7,
-              OP_SQRT = 8, OP_FMA = 9, OP_FMS = 10;
+              OP_SQRT = 8, OP_FMA = 9, OP_FMS = 10, OP_CFMA = 11, OP_CFMS = 12;
 
 template <int N> __device__ __forceinline__ Num<N> load_num(const u32* base, u64 pitch, u64 i)
 {
@@ -53,14 +54,19 @@ template <int N, int OP> __global__ void __launch_bounds__(256) elementwise_kern
 {
     const u32 i = blockIdx.x * 256u + threadIdx.x;
     if (i >= g.count) return;
-    if constexpr (OP == OP_CADD || OP == OP_CMUL || OP == OP_CDIV) {
+    if constexpr (OP == OP_CADD || OP == OP_CMUL || OP == OP_CDIV || OP == OP_CFMA || OP == OP_CFMS) {
         const u32 jb = operand_index(i, g.b_stride, g.b_period);
         Cpx<N> x = load_cpx<N>(g.a, g.pa, i);
         Cpx<N> y = load_cpx<N>(g.b, g.pb, jb);
         Cpx<N> z;
         if constexpr (OP == OP_CADD) z = cadd<N>(x, y);
         else if constexpr (OP == OP_CMUL) z = cmul<N>(x, y);
-        else z = cdiv<N>(x, y);
+        else if constexpr (OP == OP_CDIV) z = cdiv<N>(x, y);
+        else {
+            const Cpx<N> w = load_cpx<N>(g.c, g.pc, operand_index(i, g.c_stride, g.c_period));
+            if constexpr (OP == OP_CFMA) z = lf::cfma<N>(x, y, w);
+            else z = lf::cfms<N>(x, y, w);
+        }
         store_cpx<N>(g.out, g.po, i, z);
     } else {
         Num<N> x = load_num<N>(g.a, g.pa, i), z;
@@ -123,6 +129,8 @@ template <int N> hipError_t launch_elementwise(int op, const ElemArgs& g, hipStr
     case OP_SQRT: return launch_op<N, OP_SQRT>(g, s);
     case OP_FMA: return launch_op<N, OP_FMA>(g, s);
     case OP_FMS: return launch_op<N, OP_FMS>(g, s);
+    case OP_CFMA: return launch_op<N, OP_CFMA>(g, s);
+    case OP_CFMS: return launch_op<N, OP_CFMS>(g, s);
     default: return hipErrorInvalidValue;
     }
 }

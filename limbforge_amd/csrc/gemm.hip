@@ -395,6 +395,28 @@ void route_capacity(Side& s, int wx_class)
     }
 }
 
+// Five timing events per host thread and device, created once and reused by every product.
+hipError_t stage_events(hipEvent_t** out)
+{
+    struct Set {
+        hipEvent_t e[5] = {};
+        ~Set()
+        {
+            for (auto x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    };
+    thread_local Set sets[16];
+    int device = 0;
+    if (hipError_t err = hipGetDevice(&device)) return err;
+    Set& s = sets[device & 15];
+    for (auto& x : s.e)
+        if (!x)
+            if (hipError_t err = hipEventCreate(&x)) return err;
+    *out = s.e;
+    return hipSuccess;
+}
+
 int ceil_log2(std::size_t x)
 {
     int l = 0;
@@ -523,10 +545,11 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     double have = 0;
     unsigned L = 0;
     while (have <= need) have += std::log2(double(primes()[L++]));
-    // exact check of the bound M > 2^need (the double sum above only mirrors the reference's count)
+    // the same bound checked exactly: M > 2^need, i.e. M has more than `need` bits (M is odd and `need` an
+    // integer); the double sum above decides L as the reference does, this only guards against its rounding
     Big M = big_from(1);
     for (unsigned q = 0; q < L; ++q) big_mul_small(M, primes()[q]);
-    while (M.bits() <= int(need) + 1) {
+    while (M.bits() <= int(need)) {
         big_mul_small(M, primes()[L]);
         ++L;
     }
@@ -609,17 +632,10 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     const std::size_t Kp = std::max<std::size_t>((std::max<std::size_t>(P.K, 1) + 63) / 64 * 64, 64);
     const std::size_t Mp = (Me + PT - 1) / PT * PT, Np = (Ne + PT - 1) / PT * PT; // product tiles
     PoolBuf dconst{2}, dig_l1{3}, dig_l0{4}, dig_r1{5}, dig_r0{6}, dres{7}, dlist[3] = {{8}, {9}, {10}}, derr{11};
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    GEMM_HIP(hipEventCreate(&e0), "hipEventCreate");
-    GEMM_HIP(hipEventCreate(&e1), "hipEventCreate");
-    struct EventGuard {
-        hipEvent_t a, b;
-        ~EventGuard()
-        {
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-        }
-    } guard{e0, e1};
+    // events bounding the stages: start, digits done, products done, rebuild done, exact dots done
+    hipEvent_t* ev = nullptr;
+    GEMM_HIP(stage_events(&ev), "hipEventCreate");
+    hipEvent_t e0 = ev[0], e1 = ev[4];
     const int WX = lf::rebuild_words(N, wx_class);
     const int G = P.opt.band_bits;
     // constants: moduli | recip | chunk_pow[L][64] | shift_pow[L][G+1] | cinv | crt[L][WX] | bound[2][WX]
@@ -725,6 +741,7 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
             d.d0 = (int8_t*)dig_r0.p;
             GEMM_HIP(gemm_tables().digits[N](d, P.stream), "digits launch");
         }
+        GEMM_HIP(hipEventRecord(ev[1], P.stream), "hipEventRecord");
         lf::ProductArgs pa{};
         pa.a1 = (const int8_t*)dig_l1.p;
         pa.a0 = (const int8_t*)dig_l0.p;
@@ -739,7 +756,9 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
         pa.recip = C32 + o_rec;
         pa.lower = P.syrk;
         GEMM_HIP(lf::launch_product(pa, P.stream), "product launch");
-    }
+    } else
+        GEMM_HIP(hipEventRecord(ev[1], P.stream), "hipEventRecord");
+    GEMM_HIP(hipEventRecord(ev[2], P.stream), "hipEventRecord");
     {
         lf::RebuildArgs r{};
         r.res = (const uint16_t*)dres.p;
@@ -779,6 +798,7 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
             GEMM_HIP(gemm_tables().rebuild[N](wx_class, r, P.stream), "rebuild launch");
         }
     }
+    GEMM_HIP(hipEventRecord(ev[3], P.stream), "hipEventRecord");
     for (int c = 0; c < 3; ++c) {
         if (lists[c].empty()) continue;
         lf::DotArgs x{};
@@ -841,12 +861,17 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     int32_t err = 0;
     GEMM_HIP(hipMemcpyAsync(&err, derr.p, 4, hipMemcpyDeviceToHost, P.stream), "download status");
     GEMM_HIP(hipStreamSynchronize(P.stream), "device execution (linalg)");
-    float ms = 0;
+    float ms = 0, stage_ms[4] = {};
     GEMM_HIP(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    for (int q = 0; q < 4; ++q) GEMM_HIP(hipEventElapsedTime(&stage_ms[q], ev[q], ev[q + 1]), "hipEventElapsedTime");
     if (device_ms) *device_ms = ms;
     if (err) return fail(LF_RUNTIME_ERROR, "exact dot: internal window overflow");
     if (rep) {
         rep->gpu_seconds = ms * 1e-3;
+        rep->digits_seconds = stage_ms[0] * 1e-3;
+        rep->product_seconds = stage_ms[1] * 1e-3;
+        rep->rebuild_seconds = stage_ms[2] * 1e-3;
+        rep->dot_seconds = stage_ms[3] * 1e-3;
         rep->gemm_rows = Me;
         rep->gemm_cols = Ne;
         rep->moduli = any_pair ? L : 0;

@@ -2,6 +2,7 @@
 // (limb_core.hpp) built as plain C++ (LF_HOST_BUILD), compiled once per LF_WORDS in [2, 32] so the 31
 // widths build in parallel.  Records are the public element layout (N limbs, exponent, sign, status;
 // a complex value is two of them), which is lf::Num<N>'s layout.
+#include "dot2_core.hpp"
 #include "limb_core.hpp"
 #include "core_host.hpp"
 
@@ -53,9 +54,17 @@ template <int N> int host_arithmetic(int op, const void* a, const void* b, const
     case HOST_SQRT: put<N>(out, lf::sqrt<N>(get<N>(a))); return 0;
     case HOST_FMA: put<N>(out, lf::fma<N>(get<N>(a), get<N>(b), get<N>(c))); return 0;
     case HOST_FMS: put<N>(out, fms<N>(get<N>(a), get<N>(b), get<N>(c))); return 0;
+    case HOST_CFMA: put_cpx<N>(out, lf::cfma<N>(get_cpx<N>(a), get_cpx<N>(b), get_cpx<N>(c))); return 0;
+    case HOST_CFMS: put_cpx<N>(out, lf::cfms<N>(get_cpx<N>(a), get_cpx<N>(b), get_cpx<N>(c))); return 0;
     default: return -1;
     }
 }
 
+template <int N> void host_dot2(const void* const* x, void* out)
+{
+    put<N>(out, dot2_add<N>(get<N>(x[0]), get<N>(x[1]), get<N>(x[2]), get<N>(x[3]), get<N>(x[4])));
+}
+
 template int host_arithmetic<LF_WORDS>(int op, const void* a, const void* b, const void* c, void* out);
+template void host_dot2<LF_WORDS>(const void* const* x, void* out);
 } // namespace lf

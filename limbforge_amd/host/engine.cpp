@@ -269,8 +269,6 @@ void CommandBatch::encode(int bits, bool complex, Operation op, const std::share
     if (!impl_ || impl_->submitted) throw std::logic_error("batch already submitted or moved");
     validate(bits, count);
     if (!operation_is_valid(op) || operation_is_complex(op) != complex) throw std::invalid_argument("operation and buffer format mismatch");
-    if (op == Operation::complex_fma || op == Operation::complex_fms)
-        throw std::invalid_argument("complex_fma / complex_fms are not provided by this build");
     if (!b_index.stride || !c_index.stride) throw std::invalid_argument("broadcast stride must be positive");
     for (auto* r : {&a, &b, &c})
         if (*r && (*r)->bits != bits) throw std::invalid_argument("buffer precision does not match the operation");
@@ -312,6 +310,41 @@ void CommandBatch::encode_cast(int from_bits, int to_bits, bool complex, const s
     s.a = in;
     s.out = out;
     s.count = count;
+    impl_->steps.push_back(std::move(s));
+}
+
+void CommandBatch::encode_tree_sum(int bits, bool complex, const std::shared_ptr<detail::BufferStorage>& input,
+                                   const std::shared_ptr<detail::BufferStorage>& out, std::size_t count)
+{
+    if (!impl_ || impl_->submitted) throw std::logic_error("batch already submitted or moved");
+    validate(bits, count);
+    impl_->retain(input, false);
+    impl_->retain(out, true);
+    Step s;
+    s.kind = Step::custom;
+    s.launch = [bits, complex, input, out, count](void* stream) {
+        const lf_operand in = input->operand(), o = out->operand();
+        detail::check(lf_tree_sum(bits, complex, &in, count, &o, stream), "CommandBatch::tree_sum");
+    };
+    impl_->steps.push_back(std::move(s));
+}
+
+void CommandBatch::encode_segmented_dot(int bits, bool complex, const SegmentedDot& shape, const std::shared_ptr<detail::BufferStorage>& a,
+                                        const std::shared_ptr<detail::BufferStorage>& b, const std::shared_ptr<detail::BufferStorage>& out)
+{
+    if (!impl_ || impl_->submitted) throw std::logic_error("batch already submitted or moved");
+    validate(bits, shape.segments);
+    impl_->retain(a, false);
+    impl_->retain(b, false);
+    impl_->retain(out, true);
+    if (!shape.segments) return;
+    Step s;
+    s.kind = Step::custom;
+    s.launch = [bits, complex, shape, a, b, out](void* stream) {
+        const lf_operand oa = a->operand(), ob = b->operand(), o = out->operand();
+        detail::check(lf_segmented_dot(bits, complex, shape.segments, shape.length, shape.shared_right, &oa, &ob, &o, stream),
+                      "CommandBatch::segmented_dot");
+    };
     impl_->steps.push_back(std::move(s));
 }
 
@@ -489,6 +522,16 @@ Timing Engine::run_ternary(int bits, Operation op, const void* a, const void* b,
     lf_broadcast bb = to_c(b_index), cb = to_c(c_index);
     Timing t{0, 0};
     detail::check(lf_run_host(bits, int(op), a, b, c, out, count, &bb, &cb, &t.gpu_seconds, &t.wall_seconds), "Engine::run_ternary");
+    return t;
+}
+
+Timing Engine::segmented_dot(int bits, bool complex, const SegmentedDot& shape, const void* a, const void* b, void* out)
+{
+    validate(bits, shape.segments);
+    Timing t{0, 0};
+    detail::check(lf_segmented_dot_host(bits, complex, shape.segments, shape.length, shape.shared_right, a, b, out, &t.gpu_seconds,
+                                        &t.wall_seconds),
+                  "Engine::segmented_dot");
     return t;
 }
 

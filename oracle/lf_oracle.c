@@ -493,6 +493,41 @@ void lfo_cdiv(int n, const lfo_num a[2], const lfo_num b[2], lfo_num r[2])
     r[1] = im;
 }
 
+/* RN(a*b + c*d + e) with one rounding (core.hpp:330-352, dot2_add): the exact dot of the two products with
+ * e as its addend, the same exact value under the same conventions (a status anywhere gives zero with the OR
+ * of the statuses, exact cancellation gives canonical zero, no nonzero product gives e itself). */
+static lfo_num exact_dot_add_n(int n, const lfo_num* c, int subtract, const unsigned char* a, ptrdiff_t sa,
+                               const unsigned char* b, ptrdiff_t sb, size_t K);
+lfo_num lfo_dot2_add(int n, lfo_num a, lfo_num b, lfo_num c, lfo_num d, lfo_num e)
+{
+    unsigned char x[2 * (4 * LFO_MAX_WORDS + 12)], y[2 * (4 * LFO_MAX_WORDS + 12)];
+    const size_t E = 4 * (size_t)n + 12;
+    lfo_store(n, &a, x);
+    lfo_store(n, &c, x + E);
+    lfo_store(n, &b, y);
+    lfo_store(n, &d, y + E);
+    return exact_dot_add_n(n, &e, 0, x, 1, y, 1, 2);
+}
+
+/* complex a*b + c and a*b - c, each part one rounding of its exact value (core.hpp:353-358) */
+void lfo_cfma(int n, const lfo_num a[2], const lfo_num b[2], const lfo_num c[2], lfo_num r[2])
+{
+    lfo_num m = a[1];
+    m.sign = -m.sign;
+    lfo_num re = lfo_dot2_add(n, a[0], b[0], m, b[1], c[0]);
+    lfo_num im = lfo_dot2_add(n, a[0], b[1], a[1], b[0], c[1]);
+    r[0] = re;
+    r[1] = im;
+}
+
+void lfo_cfms(int n, const lfo_num a[2], const lfo_num b[2], const lfo_num c[2], lfo_num r[2])
+{
+    lfo_num d[2] = {c[0], c[1]};
+    d[0].sign = -d[0].sign;
+    d[1].sign = -d[1].sign;
+    lfo_cfma(n, a, b, d, r);
+}
+
 /* ---------------------------------------------------------------- cast */
 /* exact widening, RN narrowing (kernels.metal:251-258) */
 lfo_num lfo_cast_words(int to_n, int from_n, const lfo_num* x)
@@ -520,8 +555,8 @@ int lfo_elementwise(int bits, int op, const void* a, const void* b, const void* 
     const unsigned char* pb = (const unsigned char*)b;
     const unsigned char* pc = (const unsigned char*)c;
     unsigned char* po = (unsigned char*)out;
-    int cplx = op == LFO_CADD || op == LFO_CMUL || op == LFO_CDIV;
-    if (op < 0 || op > LFO_FMS || op == LFO_CFMA || op == LFO_CFMS) return -1;
+    int cplx = op == LFO_CADD || op == LFO_CMUL || op == LFO_CDIV || op == LFO_CFMA || op == LFO_CFMS;
+    if (op < 0 || op > LFO_CFMS) return -1;
     for (size_t i = 0; i < count; ++i) {
         size_t jb = bidx(i, b_stride, b_period), jc = bidx(i, c_stride, c_period);
         if (cplx) {
@@ -532,7 +567,14 @@ int lfo_elementwise(int bits, int op, const void* a, const void* b, const void* 
             lfo_load(n, pb + 2 * E * jb + E, &y[1]);
             if (op == LFO_CADD) lfo_cadd(n, x, y, z);
             else if (op == LFO_CMUL) lfo_cmul(n, x, y, z);
-            else lfo_cdiv(n, x, y, z);
+            else if (op == LFO_CDIV) lfo_cdiv(n, x, y, z);
+            else {
+                lfo_num w[2];
+                lfo_load(n, pc + 2 * E * jc, &w[0]);
+                lfo_load(n, pc + 2 * E * jc + E, &w[1]);
+                if (op == LFO_CFMA) lfo_cfma(n, x, y, w, z);
+                else lfo_cfms(n, x, y, w, z);
+            }
             lfo_store(n, &z[0], po + 2 * E * i);
             lfo_store(n, &z[1], po + 2 * E * i + E);
             continue;
@@ -814,7 +856,74 @@ int lfo_syrk(int bits, const void* A, size_t rows, size_t cols, void* C, int low
     return 0;
 }
 
+/* --------------------------------------------------------- reductions */
+/* Segmented dot products (kernels.metal:197-208, docs/numerics.md "Segmented dot products"): out[i] is the
+ * sequential chain s <- fma(a[i K + k], b[i K + k] or b[k], s) from s = 0 (cfma for complex values). */
+int lfo_segmented_dot(int bits, int complex_values, size_t segments, size_t length, int shared_right, const void* a,
+                      const void* b, void* out)
+{
+    if (!valid_bits(bits)) return -1;
+    const int n = bits / 32;
+    const size_t E = 4 * (size_t)n + 12, P = complex_values ? 2 : 1;
+    const unsigned char* pa = (const unsigned char*)a;
+    const unsigned char* pb = (const unsigned char*)b;
+    unsigned char* po = (unsigned char*)out;
+    for (size_t i = 0; i < segments; ++i) {
+        lfo_num s[2] = {lfo_zero(LFO_OK), lfo_zero(LFO_OK)};
+        for (size_t k = 0; k < length; ++k) {
+            const size_t ia = i * length + k, ib = shared_right ? k : i * length + k;
+            lfo_num x[2], y[2], r[2];
+            for (size_t q = 0; q < P; ++q) {
+                lfo_load(n, pa + (ia * P + q) * E, &x[q]);
+                lfo_load(n, pb + (ib * P + q) * E, &y[q]);
+            }
+            if (complex_values) {
+                lfo_cfma(n, x, y, s, r);
+                s[0] = r[0];
+                s[1] = r[1];
+            } else
+                s[0] = lfo_fma(n, x[0], y[0], s[0]);
+        }
+        for (size_t q = 0; q < P; ++q) lfo_store(n, &s[q], po + (i * P + q) * E);
+    }
+    return 0;
+}
+
+/* Adjacent-pair tree sum (kernels.metal:61-112, docs/numerics.md "Pairwise reductions"): pairs (0,1), (2,3),
+ * ... added with one rounding each, an unpaired last value carried unchanged, level by level; complex
+ * values componentwise; an empty input gives canonical zero. */
+int lfo_tree_sum(int bits, int complex_values, const void* in, size_t count, void* out)
+{
+    if (!valid_bits(bits)) return -1;
+    const int n = bits / 32;
+    const size_t E = 4 * (size_t)n + 12, P = complex_values ? 2 : 1;
+    if (!count) {
+        lfo_num z = lfo_zero(LFO_OK);
+        for (size_t q = 0; q < P; ++q) lfo_store(n, &z, (unsigned char*)out + q * E);
+        return 0;
+    }
+    lfo_num* v = (lfo_num*)calloc(count * P, sizeof(lfo_num));
+    for (size_t i = 0; i < count * P; ++i) lfo_load(n, (const unsigned char*)in + i * E, &v[i]);
+    while (count > 1) {
+        const size_t next = (count + 1) / 2;
+        for (size_t j = 0; j < next; ++j)
+            for (size_t q = 0; q < P; ++q)
+                v[j * P + q] = 2 * j + 1 < count ? lfo_add(n, v[2 * j * P + q], v[(2 * j + 1) * P + q]) : v[2 * j * P + q];
+        count = next;
+    }
+    for (size_t q = 0; q < P; ++q) lfo_store(n, &v[q], (unsigned char*)out + q * E);
+    free(v);
+    return 0;
+}
+
 /* ----------------------------------------------------- Cholesky / solves */
+/* Host threads of the factorization and solves: the panel rows below a diagonal block, the rows of a
+ * trailing update and the right-hand sides are independent given what precedes them (the reference's host
+ * path runs them on its worker pool, linalg.mm:418-441), so they run in parallel; every entry is the same
+ * rounding of the same exact value whatever the thread count.  1 (the default) runs serially. */
+static int lfo_threads = 1;
+void lfo_set_threads(int threads) { lfo_threads = threads > 0 ? threads : 1; }
+
 /* D(c; x, y; K) = RN(c - sum x_k y_k), one rounding (linalg.mm:409-411) */
 static lfo_num D(int n, const lfo_num* c, const unsigned char* x, ptrdiff_t sx, const unsigned char* y, ptrdiff_t sy, size_t K)
 {
@@ -864,10 +973,12 @@ size_t lfo_cholesky(int bits, const void* A, size_t n, void* L, size_t block, in
             } else { s = lfo_sqrt(nw, s); lfo_store(nw, &s, EL(i, i)); }                       \
         }                                                                                      \
         size_t first = pp < (kk1) ? pp + 1 : (kk1);                                            \
+        _Pragma("omp parallel for schedule(dynamic, 2) private(v, dgn) num_threads(lfo_threads)")  \
         for (size_t r = first; r < n; ++r) { PANEL(r, kk0, pp); }                              \
         (out) = pp;                                                                            \
     } while (0)
 #define UPDATE(kk0, kk1, r0, c1)                                                               \
+    _Pragma("omp parallel for schedule(dynamic, 2) private(v) num_threads(lfo_threads)")     \
     for (size_t i = (r0); i < n; ++i)                                                          \
         for (size_t j = (r0); j <= i && j < (c1); ++j) {                                       \
             lfo_load(nw, EL(i, j), &v);                                                        \
@@ -912,6 +1023,7 @@ int lfo_solve(int bits, const void* L, size_t n, const void* B, size_t nrhs, voi
 #define XW(i, c) (W + ((i) * nrhs + (c)) * E)
     /* out rows (M x nrhs) = D(out; line i of the factor view, X rows [r0, r0+K)) */
 #define SOLVE_UPDATE(base, line_stride, k_stride, M, r0, K, out)                               \
+    _Pragma("omp parallel for schedule(dynamic, 2) private(v) num_threads(lfo_threads)")     \
     for (size_t i = 0; i < (M); ++i)                                                           \
         for (size_t c = 0; c < nrhs; ++c) {                                                    \
             unsigned char* o = (out) + (i * nrhs + c) * E;                                     \
@@ -922,6 +1034,7 @@ int lfo_solve(int bits, const void* L, size_t n, const void* B, size_t nrhs, voi
     if (passes & 1)
         for (size_t r0 = 0; r0 < n; r0 += nb) {
             size_t r1 = r0 + nb < n ? r0 + nb : n;
+#pragma omp parallel for schedule(dynamic, 1) private(v, dgn) num_threads(lfo_threads)
             for (size_t c = 0; c < nrhs; ++c)
                 for (size_t i = r0; i < r1; ++i) {
                     lfo_load(nw, XW(i, c), &v);
@@ -935,6 +1048,7 @@ int lfo_solve(int bits, const void* L, size_t n, const void* B, size_t nrhs, voi
     if (passes & 2)
         for (size_t r0 = (n - 1) / nb * nb;; r0 -= nb) {
             size_t r1 = r0 + nb < n ? r0 + nb : n;
+#pragma omp parallel for schedule(dynamic, 1) private(v, dgn) num_threads(lfo_threads)
             for (size_t c = 0; c < nrhs; ++c)
                 for (size_t i = r1; i-- > r0;) {
                     lfo_load(nw, XW(i, c), &v);

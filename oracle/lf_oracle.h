@@ -68,6 +68,11 @@ lfo_num lfo_fms(int n, lfo_num a, lfo_num b, lfo_num c);
 void lfo_cadd(int n, const lfo_num a[2], const lfo_num b[2], lfo_num r[2]);
 void lfo_cmul(int n, const lfo_num a[2], const lfo_num b[2], lfo_num r[2]);
 void lfo_cdiv(int n, const lfo_num a[2], const lfo_num b[2], lfo_num r[2]);
+/* RN(a*b + c*d + e), one rounding (core.hpp:330-352) */
+lfo_num lfo_dot2_add(int n, lfo_num a, lfo_num b, lfo_num c, lfo_num d, lfo_num e);
+/* complex fused a*b + c / a*b - c, each part one rounding (core.hpp:353-358) */
+void lfo_cfma(int n, const lfo_num a[2], const lfo_num b[2], const lfo_num c[2], lfo_num r[2]);
+void lfo_cfms(int n, const lfo_num a[2], const lfo_num b[2], const lfo_num c[2], lfo_num r[2]);
 lfo_num lfo_cast_words(int to_n, int from_n, const lfo_num* x);
 
 /* divide_word (core.hpp:176-184): returns quotient | remainder << 32 */
@@ -96,6 +101,13 @@ int lfo_syrk(int bits, const void* A, size_t rows, size_t cols, void* C, int low
  * mask[i] is set and x <- RN(x*f) otherwise (Operation::fma / mul, one rounding each). */
 int lfo_mixed_chain(int bits, const void* x, const void* f, const void* g, const uint32_t* len, const uint64_t* mask,
                     void* out, size_t count);
+/* out[i] = sequential fma (cfma) chain over a[i*length + k] * b[i*length + k] (b[k] with shared_right) */
+int lfo_segmented_dot(int bits, int complex_values, size_t segments, size_t length, int shared_right, const void* a,
+                      const void* b, void* out);
+/* out (one value) = the adjacent-pair tree of rounded additions of in[0..count) */
+int lfo_tree_sum(int bits, int complex_values, const void* in, size_t count, void* out);
+/* Host threads of lfo_cholesky / lfo_solve (default 1); the results do not depend on it. */
+void lfo_set_threads(int threads);
 size_t lfo_cholesky(int bits, const void* A, size_t n, void* L, size_t block, int* pivot_sign, uint32_t* pivot_status);
 int lfo_solve(int bits, const void* L, size_t n, const void* B, size_t nrhs, void* X, size_t block, int passes);
 

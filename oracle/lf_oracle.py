@@ -45,6 +45,12 @@ def lib() -> C.CDLL:
         L.lfo_mixed_chain.argtypes = [i32, vp, vp, vp, vp, vp, vp, sz]
         L.lfo_solve.restype = i32
         L.lfo_divide_word.argtypes = [C.c_uint32] * 4
+        L.lfo_set_threads.argtypes = [i32]
+        L.lfo_segmented_dot.argtypes = [i32, i32, sz, sz, i32, vp, vp, vp]
+        L.lfo_segmented_dot.restype = i32
+        L.lfo_tree_sum.argtypes = [i32, i32, vp, sz, vp]
+        L.lfo_tree_sum.restype = i32
+        L.lfo_set_threads.restype = None
         L.lfo_divide_word.restype = C.c_uint64
         _lib = L
     return _lib
@@ -104,12 +110,66 @@ def gemm(bits, transpose_a, A, B, m, n, k, C_in=None, subtract=False):
     return Cm
 
 
+def gemm_rows(bits, A, B, m, n, k, rows=None, threads=None):
+    """Rows `rows` (default: all m) of the untransposed product RN(A B), the row blocks spread over host
+    threads.  The oracle keeps its working storage in thread-local variables and ctypes releases the
+    interpreter lock during each call, so the blocks run in parallel."""
+    import concurrent.futures as cf
+    import os
+
+    rw = bits // 32 + 3
+    A = np.ascontiguousarray(A, dtype=np.uint32).reshape(m, k, rw)
+    B = np.ascontiguousarray(B, dtype=np.uint32)
+    rows = np.arange(m) if rows is None else np.asarray(rows)
+    out = np.zeros((rows.size, n, rw), dtype=np.uint32)
+    if threads is None:
+        try:
+            threads = len(os.sched_getaffinity(0))
+        except AttributeError:
+            threads = os.cpu_count() or 1
+        threads = max(1, min(16, threads))
+
+    def one(t):
+        i = int(rows[t])
+        a = np.ascontiguousarray(A[i])
+        o = np.zeros((n, rw), dtype=np.uint32)
+        assert lib().lfo_gemm(bits, 0, a.ctypes.data, B.ctypes.data, 1, n, k, o.ctypes.data, 0) == 0
+        out[t] = o
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(one, range(rows.size)))
+    return out.reshape(rows.size * n, rw)
+
+
 def syrk(bits, A, rows, cols, C_in=None, lower_only=True, subtract=False):
     rw = bits // 32 + 3
     Cm = np.zeros((cols * cols, rw), dtype=np.uint32) if C_in is None else np.array(C_in, dtype=np.uint32).reshape(cols * cols, rw)
     A = np.ascontiguousarray(A, dtype=np.uint32)
     assert lib().lfo_syrk(bits, A.ctypes.data, rows, cols, Cm.ctypes.data, int(lower_only), int(subtract)) == 0
     return Cm
+
+
+def segmented_dot(bits, complex_values, segments, length, shared_right, a, b):
+    rw = (bits // 32 + 3) * (2 if complex_values else 1)
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    out = np.zeros((segments, rw), dtype=np.uint32)
+    assert lib().lfo_segmented_dot(bits, int(complex_values), segments, length, int(shared_right), a.ctypes.data, b.ctypes.data,
+                                   out.ctypes.data) == 0
+    return out
+
+
+def tree_sum(bits, complex_values, x):
+    rw = (bits // 32 + 3) * (2 if complex_values else 1)
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    out = np.zeros((1, rw), dtype=np.uint32)
+    assert lib().lfo_tree_sum(bits, int(complex_values), x.ctypes.data, x.shape[0], out.ctypes.data) == 0
+    return out
+
+
+def set_threads(threads):
+    """Host threads of cholesky() and solve() (1 = serial); results do not depend on it."""
+    lib().lfo_set_threads(int(threads))
 
 
 def cholesky(bits, A, n, block):

@@ -132,6 +132,100 @@ static void pin_width(int bits, int cases, uint64_t seed)
             check_ref(bits, op, a, b, b, r, 1);
         }
     }
+    /* complex fma / fms: each part RN(x y + z w + v), exact products at 2*bits and one mpfr_sum */
+    for (int i = 0; i < cases / 2; ++i) {
+        lfo_num a[2], b[2], c[2], r[2], m[2];
+        for (int p = 0; p < 2; ++p) {
+            a[p] = lfo_random_number(&g, n, 1000); b[p] = lfo_random_number(&g, n, 1000); c[p] = lfo_random_number(&g, n, 2000);
+        }
+        switch (i % 11) {
+        case 0: c[0] = lfo_zero(0); break;
+        case 1: a[1] = lfo_zero(0); b[1] = lfo_zero(0); break;
+        case 2: /* the two products cancel in the real part; the addend decides */
+            a[1] = a[0]; b[1] = b[0]; c[0].exponent = a[0].exponent + b[0].exponent - (int)(lfo_mt_next(&g) % (uint64_t)(3 * bits)); break;
+        case 3: /* products far apart: the smaller one is a sticky unit */
+            b[1].exponent = b[0].exponent - 5 * bits; break;
+        case 4: /* addend far above the products */
+            c[0].exponent = a[0].exponent + b[0].exponent + 2 * bits; c[1].exponent = c[0].exponent; break;
+        case 5: /* addend far below, and tied to the last bit of the real part */
+            c[0].exponent = a[0].exponent + b[0].exponent - 4 * bits; break;
+        case 6: a[0].status = LFO_INVALID; break;
+        case 7: c[1].status = LFO_DIVISION_BY_ZERO; break;
+        case 8: /* exponents near the limit */
+            a[0].exponent = 500000000; b[0].exponent = 499999999; break;
+        case 9: /* huge spread between the three terms */
+            a[0].exponent = 300000000; c[0].exponent = -300000000; b[1].exponent = -200000000; break;
+        default: break;
+        }
+        for (int sub = 0; sub < 2; ++sub) {
+            lfo_num cc[2] = {c[0], c[1]}, mi = a[1];
+            if (sub) { cc[0].sign = -cc[0].sign; cc[1].sign = -cc[1].sign; }
+            mi.sign = -mi.sign;
+            if (sub) lfo_cfms(n, a, b, c, r); else lfo_cfma(n, a, b, c, r);
+            lfo_num x2[2] = {a[0], mi}, y2[2] = {b[0], b[1]};
+            m[0] = lfo_mpfr_dot(n, &cc[0], 0, x2, 1, y2, 1, 2);
+            lfo_num x3[2] = {a[0], a[1]}, y3[2] = {b[1], b[0]};
+            m[1] = lfo_mpfr_dot(n, &cc[1], 0, x3, 1, y3, 1, 2);
+            check(sub ? "cfms.re" : "cfma.re", n, i, &r[0], &m[0]);
+            check(sub ? "cfms.im" : "cfma.im", n, i, &r[1], &m[1]);
+            check_ref(bits, sub ? LFO_CFMS : LFO_CFMA, a, b, c, r, 1);
+        }
+    }
+    /* segmented dot products and tree sums against MPFR replays of their fixed step sequences */
+    for (int i = 0; i < cases / 16 + 4; ++i) {
+        const size_t segs = 1 + (size_t)(lfo_mt_next(&g) % 5), len = (size_t)(i % 6 == 0 ? 0 : lfo_mt_next(&g) % 40);
+        const int shared = i % 3 == 1, cplx = i % 2;
+        const size_t P = cplx ? 2 : 1, E = 4 * (size_t)n + 12, na = segs * len * P, nb = (shared ? len : segs * len) * P;
+        lfo_num* a = (lfo_num*)calloc(na + 1, sizeof(lfo_num));
+        lfo_num* b = (lfo_num*)calloc(nb + 1, sizeof(lfo_num));
+        for (size_t k = 0; k < na; ++k) a[k] = lfo_random_number(&g, n, 60);
+        for (size_t k = 0; k < nb; ++k) b[k] = lfo_random_number(&g, n, 60);
+        if (na > 3 && i % 5 == 2) a[3] = lfo_zero(0);
+        if (na > 5 && i % 7 == 3) a[5].status = LFO_INVALID;
+        unsigned char* ra = (unsigned char*)malloc((na + 1) * E);
+        unsigned char* rb = (unsigned char*)malloc((nb + 1) * E);
+        unsigned char* ro = (unsigned char*)malloc(segs * P * E);
+        for (size_t k = 0; k < na; ++k) lfo_store(n, &a[k], ra + k * E);
+        for (size_t k = 0; k < nb; ++k) lfo_store(n, &b[k], rb + k * E);
+        lfo_segmented_dot(bits, cplx, segs, len, shared, ra, rb, ro);
+        for (size_t q = 0; q < segs; ++q) {
+            lfo_num acc[2] = {lfo_zero(0), lfo_zero(0)};
+            for (size_t k = 0; k < len; ++k) {
+                const lfo_num* x = &a[(q * len + k) * P];
+                const lfo_num* y = &b[(shared ? k : q * len + k) * P];
+                if (cplx) { /* each part RN(x.re y.re - x.im y.im + acc.re), RN(x.re y.im + x.im y.re + acc.im) */
+                    lfo_num mi = x[1], u[2] = {x[0], mi}, v[2] = {y[0], y[1]}, u2[2] = {x[0], x[1]}, v2[2] = {y[1], y[0]};
+                    u[1].sign = -u[1].sign;
+                    lfo_num re = lfo_mpfr_dot(n, &acc[0], 0, u, 1, v, 1, 2), im = lfo_mpfr_dot(n, &acc[1], 0, u2, 1, v2, 1, 2);
+                    acc[0] = re;
+                    acc[1] = im;
+                } else
+                    acc[0] = lfo_mpfr_fused(n, x, y, &acc[0], 0);
+            }
+            for (size_t p2 = 0; p2 < P; ++p2) {
+                lfo_num r;
+                lfo_load(n, ro + (q * P + p2) * E, &r);
+                check(cplx ? "segmented dot (complex)" : "segmented dot", n, i, &r, &acc[p2]);
+            }
+        }
+        /* tree sum of a (all of it, as one array) */
+        const size_t cnt = na / P;
+        lfo_tree_sum(bits, cplx, ra, cnt, ro);
+        for (size_t p2 = 0; p2 < P; ++p2) {
+            lfo_num* v = (lfo_num*)calloc(cnt + 1, sizeof(lfo_num));
+            for (size_t k = 0; k < cnt; ++k) v[k] = a[k * P + p2];
+            size_t c = cnt;
+            while (c > 1) {
+                for (size_t j = 0; j < (c + 1) / 2; ++j) v[j] = 2 * j + 1 < c ? lfo_mpfr_real(n, LFO_ADD, &v[2 * j], &v[2 * j + 1]) : v[2 * j];
+                c = (c + 1) / 2;
+            }
+            lfo_num want = cnt ? v[0] : lfo_zero(0), r;
+            lfo_load(n, ro + p2 * E, &r);
+            check("tree sum", n, i, &r, &want);
+            free(v);
+        }
+        free(a); free(b); free(ra); free(rb); free(ro);
+    }
     /* exact dot products and updates */
     static const size_t Ks[] = {0, 1, 2, 3, 4, 7, 17, 64, 200};
     for (int i = 0; i < cases / 8 + 9; ++i) {

@@ -31,14 +31,19 @@ void elementwise(int op, const unsigned char* a, const unsigned char* b, const u
 {
     using T = Number<N>;
     using C = Complex<N>;
-    const bool cplx = op == 4 || op == 5 || op == 6;
+    const bool cplx = op == 4 || op == 5 || op == 6 || op == 11 || op == 12;
     for (std::size_t i = 0; i < count; ++i) {
         std::size_t jb = index_of(i, bs, bp), jc = index_of(i, cs, cp);
         if (cplx) {
             C x, y, z;
             std::memcpy(&x, a + i * sizeof(C), sizeof(C));
             std::memcpy(&y, b + jb * sizeof(C), sizeof(C));
-            z = op == 4 ? cadd(x, y) : op == 5 ? cmul(x, y) : cdiv(x, y);
+            if (op == 11 || op == 12) {
+                C w;
+                std::memcpy(&w, c + jc * sizeof(C), sizeof(C));
+                z = op == 11 ? cfma(x, y, w) : cfms(x, y, w);
+            } else
+                z = op == 4 ? cadd(x, y) : op == 5 ? cmul(x, y) : cdiv(x, y);
             std::memcpy(out + i * sizeof(C), &z, sizeof(C));
             continue;
         }
@@ -64,7 +69,7 @@ void elementwise(int op, const unsigned char* a, const unsigned char* b, const u
 extern "C" int ref_elementwise(int bits, int op, const void* a, const void* b, const void* c, void* out, std::size_t count,
                                std::size_t bs, std::size_t bp, std::size_t cs, std::size_t cp)
 {
-    if (bits < 64 || bits > 1024 || bits % 32 || op < 0 || op > 10) return -1;
+    if (bits < 64 || bits > 1024 || bits % 32 || op < 0 || op > 12) return -1;
     with_words<2>(bits / 32, [&](auto w) {
         elementwise<decltype(w)::value>(op, static_cast<const unsigned char*>(a), static_cast<const unsigned char*>(b),
                                         static_cast<const unsigned char*>(c), static_cast<unsigned char*>(out), count, bs, bp, cs, cp);

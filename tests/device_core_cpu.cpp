@@ -1,6 +1,7 @@
 // Host build of the device arithmetic core (limbforge_amd/csrc/limb_core.hpp) checked against the
 // oracle on the reference tests' case classes, for every width.  Test infrastructure: compiled by
 // tests/test_device_core_cpu.py with LF_HOST_BUILD defined.
+#include "dot2_core.hpp"
 #include "limb_core.hpp"
 #include "lf_oracle.h"
 #include "lfo_cases.h"
@@ -195,6 +196,36 @@ template <int N> void run_width(int cases)
             auto m = lf::cmul<N>(cx, cy);
             report<N>("cmul.re", i, m.re, r[0]);
             report<N>("cmul.im", i, m.im, r[1]);
+        }
+        {   // complex fma / fms: cancellation between the products, addends far above and below, huge spreads
+            lfo_num x[2], y[2], z[2], r[2];
+            for (int p = 0; p < 2; ++p) {
+                x[p] = lfo_random_number(&g, N, 1000);
+                y[p] = lfo_random_number(&g, N, 1000);
+                z[p] = lfo_random_number(&g, N, 3000);
+            }
+            const long pe = (long)x[0].exponent + y[0].exponent;
+            switch (i % 10) {
+            case 0: x[1] = x[0]; y[1] = y[0]; z[0].exponent = (int)(pe - (long)(lfo_mt_next(&g) % (uint64_t)(4 * 32 * N))); break;
+            case 1: y[1].exponent = y[0].exponent - 5 * 32 * N; break;
+            case 2: z[0].exponent = (int)(pe + 3 * 32 * N); break;
+            case 3: z[0].exponent = (int)(pe - 4 * 32 * N); z[1] = lfo_zero(0); break;
+            case 4: x[0].exponent = 300000000; z[0].exponent = -300000000; y[1].exponent = -200000000; break;
+            case 5: x[1] = lfo_zero(0); break;
+            case 6: x[0] = lfo_zero(0); x[1] = lfo_zero(0); break;
+            case 7: z[1].status = 2; break;
+            case 8: x[0].exponent = 500000000; y[0].exponent = 499999999; break;
+            default: break;
+            }
+            lf::Cpx<N> cx{to_dev<N>(x[0]), to_dev<N>(x[1])}, cy{to_dev<N>(y[0]), to_dev<N>(y[1])}, cz{to_dev<N>(z[0]), to_dev<N>(z[1])};
+            lfo_cfma(N, x, y, z, r);
+            auto f = lf::cfma<N>(cx, cy, cz);
+            report<N>("cfma.re", i, f.re, r[0]);
+            report<N>("cfma.im", i, f.im, r[1]);
+            lfo_cfms(N, x, y, z, r);
+            f = lf::cfms<N>(cx, cy, cz);
+            report<N>("cfms.re", i, f.re, r[0]);
+            report<N>("cfms.im", i, f.im, r[1]);
         }
     }
 }

@@ -3,7 +3,8 @@
 // and writes the results to argv[2]; tests/test_package_consumer.py compares them with the oracle.
 // Input (uint32 words): C; then per width N in {2, 3, 8, 16, 32}: C records each of a, b, c; then D and per
 // width D dot problems {K, subtract, has_c, c, a[K], b[K]}.  Output per width: for each case
-// add, sub, mul, div, sqrt(a), fma, fms, square(a), then cadd, cmul, cdiv of x = (a, b) and y = (c, b); then
+// add, sub, mul, div, sqrt(a), fma, fms, square(a), then cadd, cmul, cdiv of x = (a, b) and y = (c, b), cfma and
+// cfms of x, y and z = (c, a); then
 // the D dot results.
 #include <limbforge/core.hpp>
 #include <limbforge/linalg.hpp>
@@ -55,6 +56,9 @@ template <int N> void width(std::uint32_t C)
         give(cadd(x, y));
         give(cmul(x, y));
         give(cdiv(x, y));
+        const Complex<N> z{c[i], a[i]};
+        give(cfma(x, y, z));
+        give(cfms(x, y, z));
     }
 }
 

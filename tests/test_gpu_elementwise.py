@@ -76,6 +76,28 @@ def test_complex_ops_vs_oracle(capi, oracle, bits):
         assert_same(got, want, f"{op} bits={bits}")
 
 
+@pytest.mark.parametrize("bits", WIDTHS)
+def test_complex_fused_vs_oracle(capi, oracle, bits):
+    """complex_fma / complex_fms (core.hpp:330-358): each part one rounding of a.re b.re - a.im b.im + c.re and
+    a.re b.im + a.im b.re + c.im, with the addend broadcast, cancelling products, and addends far above and
+    below the products."""
+    print(f"[gpu] complex fma/fms bits={bits}", flush=True)
+    rng = np.random.default_rng(1177 + bits)
+    a, b = lfcases.complex_cases(rng, bits, 96)
+    rw = bits // 32 + 3
+    c = np.concatenate([lfcases.random_records(rng, bits, 96, 3000), lfcases.random_records(rng, bits, 96, 3000)], axis=1)
+    e = lfcases.get_exp(a[:, :rw], bits) + lfcases.get_exp(b[:, :rw], bits)
+    far = np.arange(96) % 4
+    c[far == 1, bits // 32] = lfcases.i32(e[far == 1] + 3 * bits)
+    c[far == 2, bits // 32] = lfcases.i32(e[far == 2] - 4 * bits)
+    c[np.arange(96) % 11 == 3] = 0
+    for op in ("complex_fma", "complex_fms"):
+        got = run_gpu(capi, bits, op, a, b, c)
+        assert_same(got, oracle.elementwise(bits, capi.OPS[op], a, b, c), f"{op} bits={bits}")
+        got = run_gpu(capi, bits, op, a, b, c[:7], c_index=(1, 7))
+        assert_same(got, oracle.elementwise(bits, capi.OPS[op], a, b, c[:7], c_index=(1, 7)), f"{op} bits={bits} broadcast c")
+
+
 @pytest.mark.parametrize("bits", [64, 96, 256, 384, 512, 1024])
 def test_golden_fixtures(capi, oracle, bits):
     print(f"[gpu] golden bits={bits}", flush=True)

@@ -1,6 +1,6 @@
 """The CMake package: configure and install LimbForge (CMakeLists.txt at the repository root) into a scratch
 prefix, build tests/package_consumer against it with find_package(LimbForge), and check what it computes on
-the host -- the scalar arithmetic of limbforge/core.hpp (add ... cdiv) and exact_dot_add -- bit for bit against
+the host -- the scalar arithmetic of limbforge/core.hpp (add ... cdiv, cfma, cfms) and exact_dot_add -- bit for bit against
 the oracle at 64, 96, 256, 512 and 1024 bits.  No GPU: both run on the CPU."""
 import os
 import shutil
@@ -85,7 +85,7 @@ def test_host_arithmetic_and_exact_dot_vs_oracle(consumer, oracle):
     for n in WIDTHS:
         bits, rw = 32 * n, n + 3
         a, b, c = cases[n]
-        per = len(OPS) * rw + 3 * 2 * rw
+        per = len(OPS) * rw + 5 * 2 * rw
         block = got[at:at + C * per].reshape(C, per)
         at += C * per
         for k, op in enumerate(OPS):
@@ -96,8 +96,9 @@ def test_host_arithmetic_and_exact_dot_vs_oracle(consumer, oracle):
         x = np.concatenate([a, b], axis=1)
         y = np.concatenate([c, b], axis=1)
         base = len(OPS) * rw
-        for k, code in enumerate((4, 5, 6)):
-            want = oracle.elementwise(bits, code, x, y)
+        z = np.concatenate([c, a], axis=1)
+        for k, code in enumerate((4, 5, 6, 11, 12)):
+            want = oracle.elementwise(bits, code, x, y, z)
             mine = block[:, base + k * 2 * rw:base + (k + 1) * 2 * rw]
             bad = np.nonzero((mine != want).any(axis=1))[0]
             assert bad.size == 0, f"{bits}-bit host complex op {code}: {bad.size} cases differ from the oracle"
