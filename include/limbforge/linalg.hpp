@@ -69,6 +69,30 @@ struct CholeskyInfo {
     double panel_seconds = 0, update_seconds = 0;
 };
 
+namespace detail {
+struct LinalgPass;
+template <class T> constexpr int real_bits()
+{
+    static_assert(Format<T>::bits && !Format<T>::complex, "dense linear algebra uses real Float<bits> buffers");
+    return Format<T>::bits;
+}
+} // namespace detail
+
+// Handle of a resident product (Linalg::syrk / gemm on a CommandBatch).  C is final once the submission is
+// waited; report() then gives the analysis and placement exactly as the host-array call reports them (no
+// timings).  Copies share state.
+class LinalgTicket {
+    std::shared_ptr<detail::LinalgPass> pass_;
+    explicit LinalgTicket(std::shared_ptr<detail::LinalgPass> p) : pass_(std::move(p)) {}
+    friend class Linalg;
+
+public:
+    LinalgTicket() = default;
+    bool resolved() const;              // the submission was waited: C is final
+    const LinalgReport& report() const; // throws std::logic_error before resolved()
+    bool provisional_reads() const;     // a later operation of the batch read C; throws before resolved()
+};
+
 // One Linalg per host thread.  C must not overlap A or B.  Calls return after the GPU has finished.
 class Linalg {
 public:
@@ -109,6 +133,57 @@ public:
     }
     const LinalgReport& report() const;
     const LinalgOptions& options() const;
+
+    // ---- Resident products: syrk / gemm encoded into an Engine's CommandBatch on Buffer<Float<bits>> operands
+    // of that engine (row-major, sizes as the host calls; buffers may be larger).  After Submission::wait() C
+    // equals the host-array call bit for bit.  C is provisional within the batch: a later write to it in the
+    // same batch throws std::logic_error and a later read sets the ticket's provisional_reads().  C must not be A
+    // or B.  The Linalg's own report() is not changed.
+    template <class T>
+    LinalgTicket syrk(CommandBatch& batch, const Buffer<T>& A, std::size_t rows, std::size_t cols, Buffer<T>& C, bool lower_only = true,
+                      bool subtract = false)
+    {
+        return encode_product(batch, detail::real_bits<T>(), true, false, detail::Access::operand(A), detail::Access::operand(A), cols, cols, rows,
+                              detail::Access::operand(C), lower_only, subtract);
+    }
+    template <class T>
+    LinalgTicket gemm(CommandBatch& batch, bool transpose_a, const Buffer<T>& A, const Buffer<T>& B, std::size_t m, std::size_t n, std::size_t k,
+                      Buffer<T>& C, bool subtract = false)
+    {
+        return encode_product(batch, detail::real_bits<T>(), false, transpose_a, detail::Access::operand(A), detail::Access::operand(B), m, n, k,
+                              detail::Access::operand(C), false, subtract);
+    }
+    template <class T>
+    LinalgTicket gemm_exact(CommandBatch& b, bool transpose_a, const Buffer<T>& A, const Buffer<T>& B, std::size_t m, std::size_t n, std::size_t k,
+                            Buffer<T>& C, bool subtract = false)
+    {
+        return gemm(b, transpose_a, A, B, m, n, k, C, subtract);
+    }
+    template <class T>
+    LinalgTicket syrk_exact(CommandBatch& b, const Buffer<T>& A, std::size_t rows, std::size_t cols, Buffer<T>& C, bool lower_only = true,
+                            bool subtract = false)
+    {
+        return syrk(b, A, rows, cols, C, lower_only, subtract);
+    }
+    // Factorizations and solves on Buffer operands: synchronous, outside any batch.  The buffers must be idle;
+    // the GPU uses them in place (no copies) and the results equal the host-array calls bit for bit.
+    template <class T> CholeskyInfo cholesky(const Buffer<T>& A, std::size_t n, Buffer<T>& L, const FactorOptions& options = {})
+    {
+        return cholesky_buffers(detail::real_bits<T>(), detail::Access::operand(A), n, detail::Access::operand(L), options);
+    }
+    template <class T>
+    Timing trsm(bool transpose, const Buffer<T>& L, std::size_t n, const Buffer<T>& B, std::size_t nrhs, Buffer<T>& X,
+                const FactorOptions& options = {})
+    {
+        return solve_buffers(detail::real_bits<T>(), transpose ? 2 : 1, detail::Access::operand(L), n, detail::Access::operand(B), nrhs,
+                             detail::Access::operand(X), options);
+    }
+    template <class T>
+    Timing cholesky_solve(const Buffer<T>& L, std::size_t n, const Buffer<T>& B, std::size_t nrhs, Buffer<T>& X, const FactorOptions& options = {})
+    {
+        return solve_buffers(detail::real_bits<T>(), 3, detail::Access::operand(L), n, detail::Access::operand(B), nrhs, detail::Access::operand(X),
+                             options);
+    }
     Timing gemm_exact(int bits, bool transpose_a, const void* A, const void* B, std::size_t m, std::size_t n, std::size_t k, void* C,
                       bool subtract = false)
     {
@@ -120,6 +195,11 @@ public:
     }
 
 private:
+    LinalgTicket encode_product(CommandBatch& batch, int bits, bool syrk, bool transpose_a, const detail::Operand& A, const detail::Operand& B,
+                                std::size_t m, std::size_t n, std::size_t k, const detail::Operand& C, bool lower_only, bool subtract);
+    CholeskyInfo cholesky_buffers(int bits, const detail::Operand& A, std::size_t n, const detail::Operand& L, const FactorOptions& options);
+    Timing solve_buffers(int bits, int passes, const detail::Operand& L, std::size_t n, const detail::Operand& B, std::size_t nrhs,
+                         const detail::Operand& X, const FactorOptions& options);
     struct Impl;
     std::unique_ptr<Impl> impl;
 };

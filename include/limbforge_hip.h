@@ -201,6 +201,17 @@ int lf_gemm_host(int bits, int transpose_a, const void* A, const void* B, uint64
 int lf_syrk_host(int bits, const void* A, uint64_t rows, uint64_t cols, void* C, int lower_only, int subtract,
                  const lf_linalg_options* options, lf_linalg_report* report, double* gpu_seconds, double* wall_seconds);
 
+/* The same products on device operands (the resident Linalg::gemm / syrk on Buffer operands, linalg.hpp:187-190;
+ * src/linalg.mm:849-1027): A, B and C are limb planes holding the row-major matrices of the host calls, and the
+ * results equal the host-array calls bit for bit, the report included.  The band analysis runs on the device and
+ * only its line table (one record per row of op(A) and column of B) is read back to plan the launches; every
+ * output, fallback lines included, is computed on the device.  Returns after the product has finished on
+ * `stream` (work queued earlier on it comes first).  C must not overlap A or B. */
+int lf_gemm(int bits, int transpose_a, const lf_operand* A, const lf_operand* B, uint64_t m, uint64_t n, uint64_t k, const lf_operand* C,
+            int subtract, const lf_linalg_options* options, lf_linalg_report* report, double* gpu_seconds, void* stream);
+int lf_syrk(int bits, const lf_operand* A, uint64_t rows, uint64_t cols, const lf_operand* C, int lower_only, int subtract,
+            const lf_linalg_options* options, lf_linalg_report* report, double* gpu_seconds, void* stream);
+
 /* ---- blocked Cholesky and triangular solves (linalg.hpp:73-87, 157-163; src/linalg.mm:411-486) ----
  * The rounding sequence is fixed by `block` alone (docs/numerics.md:687-727): every entry takes
  * blk(j) + 2 roundings.  FactorOptions' host_macs, solve_host_macs and gpu choose the reference's

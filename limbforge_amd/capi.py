@@ -402,6 +402,37 @@ def _linalg_sigs(L):
     L.lf_syrk_host.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_int, C.c_int, P(LinalgOptions),
                                P(LinalgReport), P(C.c_double), P(C.c_double)]
     L.lf_syrk_host.restype = C.c_int
+    L.lf_gemm.argtypes = [C.c_int, C.c_int, P(Operand), P(Operand), C.c_uint64, C.c_uint64, C.c_uint64, P(Operand), C.c_int,
+                          P(LinalgOptions), P(LinalgReport), P(C.c_double), C.c_void_p]
+    L.lf_gemm.restype = C.c_int
+    L.lf_syrk.argtypes = [C.c_int, P(Operand), C.c_uint64, C.c_uint64, P(Operand), C.c_int, C.c_int, P(LinalgOptions), P(LinalgReport),
+                          P(C.c_double), C.c_void_p]
+    L.lf_syrk.restype = C.c_int
+
+
+def gemm_resident(bits, transpose_a, A: "Buffer", B: "Buffer", m, n, k, Cb: "Buffer", subtract=False, options=None, stream=None):
+    """The resident product (Linalg::gemm on Buffers, linalg.hpp:187-190): C written in place on the device.
+    Returns (report, gpu_seconds)."""
+    L = lib()
+    _linalg_sigs(L)
+    o = options if options is not None else LinalgOptions.default()
+    rep, g = LinalgReport(), C.c_double()
+    oa, ob, oc = A.operand, B.operand, Cb.operand
+    check(L.lf_gemm(bits, int(transpose_a), C.byref(oa), C.byref(ob), m, n, k, C.byref(oc), int(subtract), C.byref(o), C.byref(rep),
+                    C.byref(g), stream))
+    return rep, g.value
+
+
+def syrk_resident(bits, A: "Buffer", rows, cols, Cb: "Buffer", lower_only=True, subtract=False, options=None, stream=None):
+    """The resident Linalg::syrk on Buffers: C written in place on the device.  Returns (report, gpu_seconds)."""
+    L = lib()
+    _linalg_sigs(L)
+    o = options if options is not None else LinalgOptions.default()
+    rep, g = LinalgReport(), C.c_double()
+    oa, oc = A.operand, Cb.operand
+    check(L.lf_syrk(bits, C.byref(oa), rows, cols, C.byref(oc), int(lower_only), int(subtract), C.byref(o), C.byref(rep), C.byref(g),
+                    stream))
+    return rep, g.value
 
 
 def gemm(bits, transpose_a, A, B, m, n, k, C_in=None, subtract=False, options=None):

@@ -129,8 +129,6 @@ template <int N> hipError_t launch_elementwise(int op, const ElemArgs& g, hipStr
     case OP_SQRT: return launch_op<N, OP_SQRT>(g, s);
     case OP_FMA: return launch_op<N, OP_FMA>(g, s);
     case OP_FMS: return launch_op<N, OP_FMS>(g, s);
-    case OP_CFMA: return launch_op<N, OP_CFMA>(g, s);
-    case OP_CFMS: return launch_op<N, OP_CFMS>(g, s);
     default: return hipErrorInvalidValue;
     }
 }

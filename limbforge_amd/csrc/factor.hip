@@ -5,9 +5,9 @@
 // The work matrix stays in limb planes on the device for the whole call.  Per column block [k0, k1):
 //   1. the diagonal block, column by column in one workgroup (pivot, sqrt, then div of the entries below);
 //   2. the rows below it, one thread per row (chol_panel_kernel);
-//   3. one download of the panel's exponent / sign / status words (and the pivot record), then the trailing
-//      update a_ij = D(a_ij; l_i, l_j; [k0, k1)) for i >= j >= k1 as one SYRK product of the update GEMM
-//      (residue GEMM on int8 MFMA plus exact-dot lines, gemm.hip) written in place.
+//   3. the pivot record to the host, then the trailing update a_ij = D(a_ij; l_i, l_j; [k0, k1)) for
+//      i >= j >= k1 as one SYRK product of the update GEMM (band analysis on the device, residue GEMM on int8
+//      MFMA plus exact-dot lines, gemm.hip) written in place.
 // The reference splits step 3 into the next panel's columns and the rest and overlaps the rest with the
 // next panel on a second host thread; every entry is one rounding of the same exact value either way, so
 // one in-place product gives the same bits.  The solves run block by block the same way: the in-block
@@ -95,57 +95,6 @@ const FactorTables& tables()
 
 bool valid_bits(int bits) { return bits >= 64 && bits <= 1024 && bits % 32 == 0; }
 
-// Pinned host copies of the exponent, sign and status planes of rows of a matrix (whole rows, or a range
-// of columns), for the band analysis of an update product.
-struct RegionFields {
-    uint32_t* host = nullptr;
-    std::size_t cap = 0, plane = 0;
-    ~RegionFields()
-    {
-        if (host) (void)hipHostFree(host);
-    }
-    int download(const uint32_t* planes, uint64_t pitch, int N, uint64_t first_row, uint64_t rows, uint64_t cols, hipStream_t stream)
-    {
-        plane = rows * cols;
-        if (3 * plane > cap) {
-            if (host) (void)hipHostFree(host);
-            host = nullptr;
-            cap = 0;
-            FACTOR_HIP(hipHostMalloc((void**)&host, std::max<std::size_t>(3 * plane, 1) * 4, hipHostMallocDefault), "hipHostMalloc (fields)");
-            cap = 3 * plane;
-        }
-        for (int f = 0; f < 3 && plane; ++f)
-            FACTOR_HIP(hipMemcpyAsync(host + f * plane, planes + (uint64_t)(N + f) * pitch + first_row * cols, plane * 4, hipMemcpyDeviceToHost,
-                                      stream),
-                       "download fields");
-        return LF_OK;
-    }
-    // columns [col0, col0 + w) of the rows [first_row, first_row + rows) of a matrix with `cols` columns:
-    // three pinned 2-D copies into rows of w words
-    int download_cols(const uint32_t* planes, uint64_t pitch, int N, uint64_t first_row, uint64_t rows, uint64_t cols, uint64_t col0,
-                      uint64_t w, hipStream_t stream)
-    {
-        plane = rows * w;
-        if (3 * plane > cap) {
-            if (host) (void)hipHostFree(host);
-            host = nullptr;
-            cap = 0;
-            FACTOR_HIP(hipHostMalloc((void**)&host, std::max<std::size_t>(3 * plane, 1) * 4, hipHostMallocDefault), "hipHostMalloc (fields)");
-            cap = 3 * plane;
-        }
-        for (int f = 0; f < 3 && plane; ++f)
-            FACTOR_HIP(hipMemcpy2DAsync(host + f * plane, w * 4, planes + (uint64_t)(N + f) * pitch + first_row * cols + col0, cols * 4, w * 4, rows,
-                                        hipMemcpyDeviceToHost, stream),
-                       "download fields");
-        return LF_OK;
-    }
-    // element (line, k) at col0 + line*ls + k*ks of the downloaded rows
-    lf::Fields fields(uint64_t col0, uint64_t ls, uint64_t ks) const
-    {
-        return {host + col0, host + plane + col0, host + 2 * plane + col0, 1, ls, ks};
-    }
-};
-
 // Copies the N+3 planes of `count` elements between operands of possibly different pitch.
 int copy_planes(int N, const lf_operand& from, const lf_operand& to, uint64_t count, hipStream_t s)
 {
@@ -194,9 +143,8 @@ int check_factor(int bits, uint64_t n, uint64_t nrhs, const lf_factor_options* f
 }
 
 // One update product on the work matrices: C (m x nout at cbase, row stride cstride) -= left * right^T.
-int update(int bits, const lf_linalg_options& o, bool syrk, const lf::MatView& left, const lf::Fields& lf_, const lf::MatView& right,
-           const lf::Fields& rf_, std::size_t m, std::size_t nout, std::size_t K, uint32_t* cbase, uint64_t cpitch, uint64_t cstride,
-           hipStream_t s, float& ms)
+int update(int bits, const lf_linalg_options& o, bool syrk, const lf::MatView& left, const lf::MatView& right, std::size_t m,
+           std::size_t nout, std::size_t K, uint32_t* cbase, uint64_t cpitch, uint64_t cstride, hipStream_t s, float& ms)
 {
     lf::Product P{};
     P.bits = bits;
@@ -205,8 +153,6 @@ int update(int bits, const lf_linalg_options& o, bool syrk, const lf::MatView& l
     P.subtract = true;
     P.left = left;
     P.right = syrk ? left : right;
-    P.lf_ = lf_;
-    P.rf_ = syrk ? lf_ : rf_;
     P.m = m;
     P.n = nout;
     P.K = K;
@@ -285,7 +231,6 @@ int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, 
             (void)hipEventDestroy(b);
         }
     } guard{e0, e1};
-    RegionFields panel;
     for (uint64_t k0 = 0; k0 < n;) {
         const uint64_t k1 = std::min(n, k0 + nb), w = k1 - k0;
         auto t = Clock::now();
@@ -296,7 +241,6 @@ int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, 
         FACTOR_HIP(tables().panel[N](f, s), "panel launch");
         FACTOR_HIP(hipEventRecord(e1, s), "hipEventRecord");
         FACTOR_HIP(hipMemcpyAsync(&hinfo, dinfo.p, sizeof hinfo, hipMemcpyDeviceToHost, s), "download pivot");
-        if (int r = panel.download_cols(W, pitch, N, k1, n - k1, n, k0, w, s)) return r; // the panel's columns below the block
         FACTOR_HIP(hipStreamSynchronize(s), "device execution (panel)");
         float ms = 0;
         FACTOR_HIP(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
@@ -315,8 +259,7 @@ int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, 
         if (k1 < n) {
             t = Clock::now();
             const lf::MatView v{W + (k1 * n + k0), pitch, n, 1};
-            if (int r = update(bits, lo, true, v, panel.fields(0, w, 1), v, panel.fields(0, w, 1), n - k1, n - k1, w, W + (k1 * n + k1), pitch,
-                               n, s, ms))
+            if (int r = update(bits, lo, true, v, v, n - k1, n - k1, w, W + (k1 * n + k1), pitch, n, s, ms))
                 return r;
             ci.timing_gpu_seconds += ms * 1e-3;
             ci.update_seconds += since(t);
@@ -366,12 +309,6 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
     FACTOR_HIP(drecip.alloc((N + 1) * rpitch * 4), "hipMalloc (reciprocals)");
     FACTOR_HIP(tables().reciprocals[N](F, fpitch, uint32_t(n), (uint32_t*)drecip.p, rpitch, s), "reciprocals launch");
     double gpu = 0;
-    // L does not change during a solve: the fields of all its rows come to the host once (band analysis
-    // of every block's update reads them), the solution's rows block by block
-    RegionFields lf_, rf_;
-    if (n > nb) {
-        if (int r = lf_.download(F, fpitch, N, 0, n, n, s)) return r;
-    }
     hipEvent_t e0, e1;
     FACTOR_HIP(hipEventCreate(&e0), "hipEventCreate");
     FACTOR_HIP(hipEventCreate(&e1), "hipEventCreate");
@@ -402,14 +339,10 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
             const uint64_t r1 = std::min(n, r0 + nb), K = r1 - r0;
             FACTOR_HIP(block_solve(0, r0, r1), "solve launch");
             if (r1 == n) break;
-            // rows [r1, n) of X -= L[r1:, r0:r1] X[r0:r1, :]
-            if (int r = rf_.download(Xp, xpitch, N, r0, K, nrhs, s)) return r;
-            FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
-            collect();
+            // rows [r1, n) of X -= L[r1:, r0:r1] X[r0:r1, :] (the update waits for the block on the stream)
             const lf::MatView left{F + (r1 * n + r0), fpitch, n, 1}, right{Xp + r0 * nrhs, xpitch, 1, nrhs};
-            if (int r = update(bits, lo, false, left, lf_.fields(r1 * n + r0, n, 1), right, rf_.fields(0, 1, nrhs), n - r1, nrhs, K, Xp + r1 * nrhs,
-                               xpitch, nrhs, s, ms))
-                return r;
+            if (int r = update(bits, lo, false, left, right, n - r1, nrhs, K, Xp + r1 * nrhs, xpitch, nrhs, s, ms)) return r;
+            collect();
             gpu += ms * 1e-3;
         }
     if (passes == 3) { // the forward pass's last block
@@ -422,12 +355,9 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
             FACTOR_HIP(block_solve(1, r0, r1), "solve launch");
             if (!r0) break;
             // rows [0, r0) of X -= L[r0:r1, :r0]^T X[r0:r1, :]
-            if (int r = rf_.download(Xp, xpitch, N, r0, K, nrhs, s)) return r;
-            FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
-            collect();
             const lf::MatView left{F + r0 * n, fpitch, 1, n}, right{Xp + r0 * nrhs, xpitch, 1, nrhs};
-            if (int r = update(bits, lo, false, left, lf_.fields(r0 * n, 1, n), right, rf_.fields(0, 1, nrhs), r0, nrhs, K, Xp, xpitch, nrhs, s, ms))
-                return r;
+            if (int r = update(bits, lo, false, left, right, r0, nrhs, K, Xp, xpitch, nrhs, s, ms)) return r;
+            collect();
             gpu += ms * 1e-3;
         }
     int32_t err = 0;

@@ -2,8 +2,9 @@
 // behind lf_gemm_host / lf_syrk_host / lf_gemm_planes (the reference's Linalg::gemm / syrk and
 // Linalg::Impl::run, src/linalg.mm:295-407).
 //
-// Per call: band analysis of every line on the host (linalg.mm:82-111, same classes and bands), modulus
-// count from the widest band (linalg.mm:341-345), CRT constants in exact integer arithmetic, then on one
+// Per call: band analysis of every line on the device (linalg.mm:82-111, same classes and bands; only the
+// line table comes back to the host), modulus count from the widest band (linalg.mm:341-345), CRT constants
+// in exact integer arithmetic, then on one
 // stream: digits of both extended operands, the residue GEMM per modulus on int8 MFMA, one output-centric
 // CRT rebuild pass that also writes the trivial outputs, and the exact-dot kernels for the outputs of lines
 // outside the residue path.  Nothing is computed on the host besides the plan.
@@ -253,6 +254,125 @@ uint32_t inverse_mod(uint64_t a, uint64_t m) // a^-1 mod m (gcd 1)
     return uint32_t(t < 0 ? t + int64_t(m) : t);
 }
 
+// ------------------------------------------------------------------ band analysis on the device
+// The reference's per-line analysis (linalg.mm:82-111), one wavefront per line reading the line's exponent,
+// sign and status planes: status OR, largest and smallest exponent of a nonzero status-free entry, the class
+// (0 bands, 1 zero or status line, 2 exact fallback) and the greedy bands [hi - G, hi] taken from the largest
+// exponent down (lo: the smallest exponent inside), one pass over the line per band.  Lines with more than
+// max_bands bands or a spread above max_spread are class 2; the first GEMM_MAX_BANDS bands are stored and nb
+// counts them all (up to max_bands + 1).
+using lf::GemmLine;
+using lf::MatView;
+using lf::u32;
+using lf::u64;
+using lf::GEMM_MAX_BANDS;
+
+struct AnalyzeArgs {
+    MatView v;
+    uint32_t field_plane; // N: exponent plane (sign N + 1, status N + 2)
+    uint32_t nlines, K;
+    int32_t G, max_bands, max_spread;
+    uint32_t force;
+    GemmLine* out;
+};
+
+__device__ __forceinline__ int wave_max(int x)
+{
+#pragma unroll
+    for (int o = 32; o; o >>= 1) x = max(x, __shfl_xor(x, o));
+    return x;
+}
+__device__ __forceinline__ int wave_min(int x)
+{
+#pragma unroll
+    for (int o = 32; o; o >>= 1) x = min(x, __shfl_xor(x, o));
+    return x;
+}
+__device__ __forceinline__ u32 wave_or(u32 x)
+{
+#pragma unroll
+    for (int o = 32; o; o >>= 1) x |= (u32)__shfl_xor((int)x, o);
+    return x;
+}
+
+__global__ void __launch_bounds__(256) analyze_kernel(AnalyzeArgs a)
+{
+    const u32 lane = threadIdx.x & 63u, line = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (line >= a.nlines) return; // whole wavefronts leave together
+    const u32* ep = a.v.planes + (u64)a.field_plane * a.v.pitch;
+    const u32* sp = ep + a.v.pitch;
+    const u32* tp = sp + a.v.pitch;
+    const u64 base = (u64)line * a.v.ls;
+    u32 status = 0;
+    int top = INT_MIN, bottom = INT_MAX;
+    for (u32 k = lane; k < a.K; k += 64u) {
+        const u64 i = base + (u64)k * a.v.ks;
+        const int e = (int)ep[i], sg = (int)sp[i];
+        const u32 st = tp[i];
+        status |= st;
+        if (!st && sg) {
+            top = max(top, e);
+            bottom = min(bottom, e);
+        }
+    }
+    status = wave_or(status);
+    top = wave_max(top);
+    bottom = wave_min(bottom);
+    GemmLine L;
+    L.cls = 1;
+    L.nb = 0;
+    L.low = 0;
+    L.status = status;
+    const bool any = top != INT_MIN;
+    L.top = any ? top : 0;
+    L.bottom = any ? bottom : 0;
+#pragma unroll
+    for (int b = 0; b < GEMM_MAX_BANDS; ++b) L.member[b] = L.lo[b] = L.hi[b] = 0;
+    if (!status && any) {
+        if (a.force) L.cls = 2;
+        else if (a.max_bands >= 1 && (long)top - bottom <= a.G) {
+            L.cls = 0;
+            L.nb = 1;
+            L.lo[0] = bottom;
+            L.hi[0] = top;
+            L.low = bottom;
+        } else if ((long)top - bottom > a.max_spread) {
+            L.cls = 2;
+        } else {
+            int hi = top, nb = 0;
+            bool fallback = false;
+            while (true) {
+                int lo = hi, next = INT_MIN;
+                for (u32 k = lane; k < a.K; k += 64u) {
+                    const u64 i = base + (u64)k * a.v.ks;
+                    const int e = (int)ep[i], sg = (int)sp[i];
+                    const u32 st = tp[i];
+                    if (st || !sg || e > hi) continue;
+                    if ((long)e >= (long)hi - a.G) lo = min(lo, e);
+                    else next = max(next, e);
+                }
+                lo = wave_min(lo);
+                next = wave_max(next);
+                if (nb < GEMM_MAX_BANDS) {
+                    L.lo[nb] = lo;
+                    L.hi[nb] = hi;
+                }
+                ++nb;
+                L.low = lo;
+                if (nb > a.max_bands) {
+                    fallback = true;
+                    break;
+                }
+                if (next == INT_MIN) break;
+                hi = next;
+            }
+            L.cls = fallback ? 2 : 0;
+            L.nb = fallback ? 0 : nb;
+        }
+    }
+    if (lane == 0) a.out[line] = L;
+}
+
 // Primes <= 65279, descending: balanced residues then split into two int8 digits (linalg.mm:38-40).
 const std::vector<uint32_t>& primes()
 {
@@ -272,7 +392,6 @@ const std::vector<uint32_t>& primes()
     return p;
 }
 
-using lf::Fields;
 using lf::Product;
 
 struct Side {
@@ -283,76 +402,22 @@ struct Side {
     std::vector<std::vector<int>> band_members; // band b -> lines having a band b (in line order)
 };
 
-template <class F> void parallel_for(unsigned threads, std::size_t n, F f)
+// The analysis of one side on the device: one analyze_kernel launch over the side's lines, the line table
+// back to the host (the only data the planner reads), then the band histogram and fallback count.
+int analyze_side(const MatView& v, int N, std::size_t lines, std::size_t K, const lf_linalg_options& o, bool force, GemmLine* dev,
+                 hipStream_t stream, Side& s)
 {
-    threads = unsigned(std::min<std::size_t>(std::max(1u, threads), n));
-    if (threads <= 1) {
-        if (n) f(std::size_t(0), n);
-        return;
-    }
-    std::vector<std::thread> w;
-    for (unsigned t = 0; t < threads; ++t) w.emplace_back(f, n * t / threads, n * (t + 1) / threads);
-    for (auto& x : w) x.join();
-}
-
-// linalg.mm:82-111: status OR, exponent range, greedy bands [hi-G, hi] from the largest exponent down
-// (lo: the smallest exponent inside), classes 0 bands / 1 trivial / 2 fallback.
-Side analyze(const Fields& f, std::size_t lines, std::size_t K, const lf_linalg_options& o, bool force, unsigned threads)
-{
-    Side s;
     s.lines.assign(lines, GemmLine{});
     s.bands.assign(lines, {});
-    const long G = o.band_bits;
-    // a thread per 2^16 entries at most: thread start-up costs more than a small analysis
-    threads = unsigned(std::min<std::size_t>(threads, std::max<std::size_t>(1, lines * std::max<std::size_t>(K, 1) >> 16)));
-    parallel_for(threads, lines, [&](std::size_t first, std::size_t last) {
-        for (std::size_t i = first; i < last; ++i) {
-            GemmLine& L = s.lines[i];
-            L.cls = 1;
-            long top = LONG_MIN, bottom = LONG_MAX;
-            uint32_t status = 0;
-            int e, sg;
-            uint32_t st;
-            for (std::size_t k = 0; k < K; ++k) {
-                f.get(i, k, e, sg, st);
-                status |= st;
-                if (!st && sg) {
-                    top = std::max(top, long(e));
-                    bottom = std::min(bottom, long(e));
-                }
-            }
-            L.status = status;
-            L.top = top == LONG_MIN ? 0 : int(top);
-            L.bottom = bottom == LONG_MAX ? 0 : int(bottom);
-            if (status || top == LONG_MIN) continue;
-            if (force) {
-                L.cls = 2;
-                continue;
-            }
-            auto& b = s.bands[i];
-            if (o.max_bands >= 1 && top - bottom <= G) {
-                b.push_back({bottom, top});
-                L.cls = 0;
-                continue;
-            }
-            long hi = top;
-            bool fallback = top - bottom > o.max_spread;
-            while (!fallback) {
-                long lo = hi, next = LONG_MIN;
-                for (std::size_t k = 0; k < K; ++k) {
-                    f.get(i, k, e, sg, st);
-                    if (st || !sg || e > hi) continue;
-                    if (e >= hi - G) lo = std::min(lo, long(e));
-                    else next = std::max(next, long(e));
-                }
-                b.push_back({lo, hi});
-                if (int(b.size()) > o.max_bands) fallback = true;
-                if (next == LONG_MIN) break;
-                hi = next;
-            }
-            L.cls = fallback ? 2 : 0;
-        }
-    });
+    if (lines) {
+        AnalyzeArgs a{v, uint32_t(N), uint32_t(lines), uint32_t(K), o.band_bits, o.max_bands, o.max_spread, force ? 1u : 0u, dev};
+        hipLaunchKernelGGL(analyze_kernel, dim3(unsigned((lines + 3) / 4)), dim3(256), 0, stream, a);
+        if (hipError_t e = hipGetLastError()) return fail(LF_RUNTIME_ERROR, std::string("analysis launch: ") + hipGetErrorString(e));
+        if (hipError_t e = hipMemcpyAsync(s.lines.data(), dev, sizeof(GemmLine) * lines, hipMemcpyDeviceToHost, stream))
+            return fail(LF_RUNTIME_ERROR, std::string("download line table: ") + hipGetErrorString(e));
+        if (hipError_t e = hipStreamSynchronize(stream))
+            return fail(LF_RUNTIME_ERROR, std::string("device execution (analysis): ") + hipGetErrorString(e));
+    }
     s.histogram.assign(std::size_t(std::max(o.max_bands, 0)) + 1, 0);
     for (std::size_t i = 0; i < lines; ++i) {
         GemmLine& L = s.lines[i];
@@ -362,19 +427,12 @@ Side analyze(const Fields& f, std::size_t lines, std::size_t K, const lf_linalg_
         }
         if (L.cls == 2) {
             ++s.fallback;
-            s.bands[i].clear();
             continue;
         }
-        const int nb = int(s.bands[i].size());
-        ++s.histogram[nb];
-        L.nb = nb;
-        L.low = int(s.bands[i].back().first);
-        for (int b = 0; b < nb && b < GEMM_MAX_BANDS; ++b) {
-            L.lo[b] = int(s.bands[i][b].first);
-            L.hi[b] = int(s.bands[i][b].second);
-        }
+        ++s.histogram[L.nb];
+        for (int b = 0; b < L.nb && b < GEMM_MAX_BANDS; ++b) s.bands[i].push_back({L.lo[b], L.hi[b]});
     }
-    return s;
+    return LF_OK;
 }
 
 // GPU-path capacity: lines whose bands this build's rebuild kernels cannot hold take the exact-dot path
@@ -521,7 +579,6 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     using Clock = std::chrono::steady_clock;
     const auto since = [](Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); };
     const int N = P.bits / 32;
-    const unsigned threads = P.opt.host_threads ? P.opt.host_threads : std::max(1u, std::thread::hardware_concurrency());
     if (rep) {
         std::memset(rep, 0, sizeof *rep);
         rep->m = P.m;
@@ -530,9 +587,14 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     }
     auto t0 = Clock::now();
     const bool force = P.opt.force_fallback || P.K > MAX_K || P.opt.method == 1;
-    Side ls = analyze(P.lf_, P.m, P.K, P.opt, force, threads);
-    Side rs_own;
-    if (!P.syrk) rs_own = analyze(P.rf_, P.n, P.K, P.opt, force, threads);
+    // line tables: written by the analysis on the device, completed with the member lists on the host
+    PoolBuf dl{0}, dr{1};
+    GEMM_HIP(dl.alloc(sizeof(GemmLine) * std::max<std::size_t>(P.m, 1)), "hipMalloc (lines)");
+    if (!P.syrk) GEMM_HIP(dr.alloc(sizeof(GemmLine) * std::max<std::size_t>(P.n, 1)), "hipMalloc (lines)");
+    Side ls, rs_own;
+    if (int r = analyze_side(P.left, N, P.m, P.K, P.opt, force, (GemmLine*)dl.p, P.stream, ls)) return r;
+    if (!P.syrk)
+        if (int r = analyze_side(P.right, N, P.n, P.K, P.opt, force, (GemmLine*)dr.p, P.stream, rs_own)) return r;
     Side& rs = P.syrk ? ls : rs_own;
     // modulus count from the widest band of both sides (linalg.mm:341-345)
     int pmax = 0;
@@ -597,13 +659,10 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
         rep->analysis_seconds = since(t0);
     }
     t0 = Clock::now();
-    // line tables
-    PoolBuf dl{0}, dr{1};
-    GEMM_HIP(dl.alloc(sizeof(GemmLine) * std::max<std::size_t>(P.m, 1)), "hipMalloc (lines)");
+    // line tables with the member lists
     GEMM_HIP(hipMemcpyAsync(dl.p, ls.lines.data(), sizeof(GemmLine) * P.m, hipMemcpyHostToDevice, P.stream), "upload lines");
     const GemmLine* dright = (const GemmLine*)dl.p;
     if (!P.syrk) {
-        GEMM_HIP(dr.alloc(sizeof(GemmLine) * std::max<std::size_t>(P.n, 1)), "hipMalloc (lines)");
         GEMM_HIP(hipMemcpyAsync(dr.p, rs.lines.data(), sizeof(GemmLine) * P.n, hipMemcpyHostToDevice, P.stream), "upload lines");
         dright = (const GemmLine*)dr.p;
     }
@@ -912,14 +971,83 @@ int lf::check_linalg_options(const lf_linalg_options* o)
 namespace {
 using lf::check_linalg_options;
 
+// Staging of the host-array products, kept per host thread and device and grown on demand, so repeated
+// calls allocate nothing after the first: a stream, pinned host memory the host arrays are copied through
+// (in chunks, the copy of one chunk overlapping the DMA of the previous one), events that pace the chunked
+// download, and the device AoS / limb-plane buffers (planner pool slots).
 struct HostStage {
-    DeviceBuf aos, pa, pb, pc;
+    int device = -1;
     hipStream_t stream = nullptr;
+    char* pinned = nullptr;
+    std::size_t pinned_bytes = 0;
+    hipEvent_t ring[8] = {};
     ~HostStage()
     {
+        if (pinned) (void)hipHostFree(pinned);
+        for (auto e : ring)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
+    hipError_t ready()
+    {
+        if (!stream)
+            if (hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) return e;
+        for (auto& e : ring)
+            if (!e)
+                if (hipError_t x = hipEventCreateWithFlags(&e, hipEventDisableTiming)) return x;
+        return hipSuccess;
+    }
+    hipError_t pin(std::size_t bytes)
+    {
+        if (bytes <= pinned_bytes) return hipSuccess;
+        if (pinned) (void)hipHostFree(pinned);
+        pinned = nullptr;
+        pinned_bytes = 0;
+        if (hipError_t e = hipHostMalloc((void**)&pinned, bytes, hipHostMallocDefault)) return e;
+        pinned_bytes = bytes;
+        return hipSuccess;
+    }
 };
+HostStage& host_stage()
+{
+    thread_local HostStage stages[16];
+    int device = 0;
+    (void)hipGetDevice(&device);
+    return stages[device & 15];
+}
+constexpr std::size_t STAGE_CHUNK = std::size_t(4) << 20;
+
+// host -> device through pinned bytes [at, at + bytes): no wait for the DMA (the region is not reused
+// within the call)
+hipError_t stage_upload(HostStage& st, std::size_t at, void* dev, const void* host, std::size_t bytes)
+{
+    for (std::size_t off = 0; off < bytes; off += STAGE_CHUNK) {
+        const std::size_t len = std::min(STAGE_CHUNK, bytes - off);
+        std::memcpy(st.pinned + at + off, static_cast<const char*>(host) + off, len);
+        if (hipError_t e = hipMemcpyAsync(static_cast<char*>(dev) + off, st.pinned + at + off, len, hipMemcpyHostToDevice, st.stream)) return e;
+    }
+    return hipSuccess;
+}
+// device -> host through pinned bytes [0, bytes): every chunk's DMA is queued, then each chunk is copied out
+// as soon as its event completes (the copy of one chunk overlaps the DMA of the next)
+hipError_t stage_download(HostStage& st, void* host, const void* dev, std::size_t bytes)
+{
+    const std::size_t chunks = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
+    std::size_t queued = 0, done = 0;
+    while (done < chunks) {
+        while (queued < chunks && queued < done + 8) {
+            const std::size_t off = queued * STAGE_CHUNK, len = std::min(STAGE_CHUNK, bytes - off);
+            if (hipError_t e = hipMemcpyAsync(st.pinned + off, static_cast<const char*>(dev) + off, len, hipMemcpyDeviceToHost, st.stream)) return e;
+            if (hipError_t e = hipEventRecord(st.ring[queued % 8], st.stream)) return e;
+            ++queued;
+        }
+        if (hipError_t e = hipEventSynchronize(st.ring[done % 8])) return e;
+        const std::size_t off = done * STAGE_CHUNK, len = std::min(STAGE_CHUNK, bytes - off);
+        std::memcpy(static_cast<char*>(host) + off, st.pinned + off, len);
+        ++done;
+    }
+    return hipSuccess;
+}
 
 } // namespace
 
@@ -960,26 +1088,30 @@ int lf_gemm_host(int bits, int transpose_a, const void* A, const void* B, uint64
     if (!m || !n) return LF_OK;
     const int N = bits / 32;
     const std::size_t E = bits / 8 + 12;
-    HostStage st;
-    GEMM_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking), "hipStreamCreate");
+    HostStage& st = host_stage();
+    GEMM_HIP(st.ready(), "hipStreamCreate");
     const std::size_t na = m * k, nb = k * n, nc = m * n;
-    GEMM_HIP(st.aos.alloc(std::max({na, nb, nc}) * E), "hipMalloc (staging)");
-    GEMM_HIP(st.pa.alloc(lf_planes_bytes(bits, 0, std::max<uint64_t>(na, 1))), "hipMalloc (A)");
-    GEMM_HIP(st.pb.alloc(lf_planes_bytes(bits, 0, std::max<uint64_t>(nb, 1))), "hipMalloc (B)");
-    GEMM_HIP(st.pc.alloc(lf_planes_bytes(bits, 0, nc)), "hipMalloc (C)");
-    lf_operand OA{st.pa.p, lf_plane_pitch(std::max<uint64_t>(na, 1))}, OB{st.pb.p, lf_plane_pitch(std::max<uint64_t>(nb, 1))},
-        OC{st.pc.p, lf_plane_pitch(nc)};
+    // AoS staging of all three operands on the device (the uploads run back to back), then their planes
+    PoolBuf aos{13}, pa{14}, pb{15}, pc{16};
+    GEMM_HIP(aos.alloc((na + nb + nc) * E), "hipMalloc (staging)");
+    GEMM_HIP(pa.alloc(lf_planes_bytes(bits, 0, std::max<uint64_t>(na, 1))), "hipMalloc (A)");
+    GEMM_HIP(pb.alloc(lf_planes_bytes(bits, 0, std::max<uint64_t>(nb, 1))), "hipMalloc (B)");
+    GEMM_HIP(pc.alloc(lf_planes_bytes(bits, 0, nc)), "hipMalloc (C)");
+    GEMM_HIP(st.pin((na + nb + nc) * E), "hipHostMalloc (staging)");
+    char* daos = static_cast<char*>(aos.p);
+    lf_operand OA{pa.p, lf_plane_pitch(std::max<uint64_t>(na, 1))}, OB{pb.p, lf_plane_pitch(std::max<uint64_t>(nb, 1))},
+        OC{pc.p, lf_plane_pitch(nc)};
     if (na) {
-        GEMM_HIP(hipMemcpyAsync(st.aos.p, A, na * E, hipMemcpyHostToDevice, st.stream), "upload A");
-        if (int r = lf_aos_to_planes(bits, 0, st.aos.p, &OA, na, st.stream)) return r;
+        GEMM_HIP(stage_upload(st, 0, daos, A, na * E), "upload A");
+        if (int r = lf_aos_to_planes(bits, 0, daos, &OA, na, st.stream)) return r;
     }
     if (nb) {
-        GEMM_HIP(hipMemcpyAsync(st.aos.p, B, nb * E, hipMemcpyHostToDevice, st.stream), "upload B");
-        if (int r = lf_aos_to_planes(bits, 0, st.aos.p, &OB, nb, st.stream)) return r;
+        GEMM_HIP(stage_upload(st, na * E, daos + na * E, B, nb * E), "upload B");
+        if (int r = lf_aos_to_planes(bits, 0, daos + na * E, &OB, nb, st.stream)) return r;
     }
     if (subtract) {
-        GEMM_HIP(hipMemcpyAsync(st.aos.p, C, nc * E, hipMemcpyHostToDevice, st.stream), "upload C");
-        if (int r = lf_aos_to_planes(bits, 0, st.aos.p, &OC, nc, st.stream)) return r;
+        GEMM_HIP(stage_upload(st, (na + nb) * E, daos + (na + nb) * E, C, nc * E), "upload C");
+        if (int r = lf_aos_to_planes(bits, 0, daos + (na + nb) * E, &OC, nc, st.stream)) return r;
     }
     Product P{};
     P.bits = bits;
@@ -989,10 +1121,6 @@ int lf_gemm_host(int bits, int transpose_a, const void* A, const void* B, uint64
     // op(A) row i, entry k: A[i*k + kk] or (transpose) A[kk*m + i]; B column j, entry kk: B[kk*n + j]
     P.left = {(const uint32_t*)OA.planes, OA.pitch, transpose_a ? 1 : k, transpose_a ? m : 1};
     P.right = {(const uint32_t*)OB.planes, OB.pitch, 1, n};
-    const uint32_t* a32 = (const uint32_t*)A;
-    const uint32_t* b32 = (const uint32_t*)B;
-    P.lf_ = {a32 + N, a32 + N + 1, a32 + N + 2, uint64_t(N + 3), P.left.ls, P.left.ks};
-    P.rf_ = {b32 + N, b32 + N + 1, b32 + N + 2, uint64_t(N + 3), 1, n};
     P.m = m;
     P.n = n;
     P.K = k;
@@ -1001,11 +1129,115 @@ int lf_gemm_host(int bits, int transpose_a, const void* A, const void* B, uint64
     P.stream = st.stream;
     float ms = 0;
     if (int r = run_product(P, report, &ms)) return r;
-    if (int r = lf_planes_to_aos(bits, 0, &OC, st.aos.p, nc, st.stream)) return r;
-    GEMM_HIP(hipMemcpyAsync(C, st.aos.p, nc * E, hipMemcpyDeviceToHost, st.stream), "download C");
-    GEMM_HIP(hipStreamSynchronize(st.stream), "device execution (linalg)");
+    if (int r = lf_planes_to_aos(bits, 0, &OC, daos, nc, st.stream)) return r;
+    GEMM_HIP(stage_download(st, C, daos, nc * E), "download C");
     if (gpu_seconds) *gpu_seconds = ms * 1e-3;
     if (wall_seconds) *wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    return LF_OK;
+}
+
+// ---- products on device operands (resident products) ----
+namespace {
+int check_product_planes(const lf_operand* o, uint64_t elements, const char* name)
+{
+    if (!elements) return LF_OK;
+    if (!o || !o->planes) return fail(LF_INVALID_ARGUMENT, std::string("null operand ") + name);
+    if (o->pitch < elements) return fail(LF_INVALID_ARGUMENT, std::string("operand ") + name + " pitch is smaller than its element count");
+    return LF_OK;
+}
+bool overlaps(int N, const lf_operand& a, const lf_operand& b)
+{
+    const uintptr_t a0 = (uintptr_t)a.planes, a1 = a0 + (uintptr_t)(N + 3) * a.pitch * 4;
+    const uintptr_t b0 = (uintptr_t)b.planes, b1 = b0 + (uintptr_t)(N + 3) * b.pitch * 4;
+    return a0 < b1 && b0 < a1;
+}
+} // namespace
+
+int lf_gemm(int bits, int transpose_a, const lf_operand* A, const lf_operand* B, uint64_t m, uint64_t n, uint64_t k, const lf_operand* C,
+            int subtract, const lf_linalg_options* options, lf_linalg_report* report, double* gpu_seconds, void* stream)
+{
+    if (gpu_seconds) *gpu_seconds = 0;
+    lf_linalg_options o;
+    if (options) o = *options;
+    else lf_linalg_default_options(&o);
+    if (int r = check_linalg_options(&o)) return r;
+    if (bits < 64 || bits > 1024 || bits % 32) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    const uint64_t limit = uint64_t(1) << 31;
+    if (m >= limit || n >= limit || k >= limit || m * n >= limit || m * k >= limit || k * n >= limit)
+        return fail(LF_INVALID_ARGUMENT, "matrix dimensions exceed 32-bit indexing");
+    if (report) {
+        std::memset(report, 0, sizeof *report);
+        report->m = m;
+        report->n = n;
+        report->k = k;
+    }
+    if (!m || !n) return LF_OK;
+    if (int r = check_product_planes(C, m * n, "C")) return r;
+    if (k) {
+        if (int r = check_product_planes(A, m * k, "A")) return r;
+        if (int r = check_product_planes(B, k * n, "B")) return r;
+        if (overlaps(bits / 32, *C, *A) || overlaps(bits / 32, *C, *B)) return fail(LF_INVALID_ARGUMENT, "gemm: C must not overlap A or B");
+    }
+    Product P{};
+    P.bits = bits;
+    P.subtract = subtract != 0;
+    const uint32_t* a = k ? (const uint32_t*)A->planes : (const uint32_t*)C->planes;
+    const uint32_t* b = k ? (const uint32_t*)B->planes : (const uint32_t*)C->planes;
+    P.left = {a, k ? A->pitch : C->pitch, transpose_a ? 1 : k, transpose_a ? m : 1};
+    P.right = {b, k ? B->pitch : C->pitch, 1, n};
+    P.m = m;
+    P.n = n;
+    P.K = k;
+    P.C = *C;
+    P.opt = o;
+    P.stream = (hipStream_t)stream;
+    float ms = 0;
+    if (int r = run_product(P, report, &ms)) return r;
+    if (gpu_seconds) *gpu_seconds = ms * 1e-3;
+    return LF_OK;
+}
+
+int lf_syrk(int bits, const lf_operand* A, uint64_t rows, uint64_t cols, const lf_operand* C, int lower_only, int subtract,
+            const lf_linalg_options* options, lf_linalg_report* report, double* gpu_seconds, void* stream)
+{
+    if (gpu_seconds) *gpu_seconds = 0;
+    lf_linalg_options o;
+    if (options) o = *options;
+    else lf_linalg_default_options(&o);
+    if (int r = check_linalg_options(&o)) return r;
+    if (bits < 64 || bits > 1024 || bits % 32) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    if (subtract && !lower_only) return fail(LF_INVALID_ARGUMENT, "syrk: subtract updates the lower triangle only (lower_only)");
+    const uint64_t limit = uint64_t(1) << 31;
+    if (rows >= limit || cols >= limit || cols * cols >= limit || rows * cols >= limit)
+        return fail(LF_INVALID_ARGUMENT, "matrix dimensions exceed 32-bit indexing");
+    if (report) {
+        std::memset(report, 0, sizeof *report);
+        report->m = cols;
+        report->n = cols;
+        report->k = rows;
+    }
+    if (!cols) return LF_OK;
+    if (int r = check_product_planes(C, cols * cols, "C")) return r;
+    if (rows) {
+        if (int r = check_product_planes(A, rows * cols, "A")) return r;
+        if (overlaps(bits / 32, *C, *A)) return fail(LF_INVALID_ARGUMENT, "syrk: C must not overlap A");
+    }
+    Product P{};
+    P.bits = bits;
+    P.syrk = true;
+    P.lower_only = lower_only != 0;
+    P.subtract = subtract != 0;
+    P.left = {rows ? (const uint32_t*)A->planes : (const uint32_t*)C->planes, rows ? A->pitch : C->pitch, 1, cols};
+    P.right = P.left;
+    P.m = cols;
+    P.n = cols;
+    P.K = rows;
+    P.C = *C;
+    P.opt = o;
+    P.stream = (hipStream_t)stream;
+    float ms = 0;
+    if (int r = run_product(P, report, &ms)) return r;
+    if (gpu_seconds) *gpu_seconds = ms * 1e-3;
     return LF_OK;
 }
 
@@ -1035,20 +1267,23 @@ int lf_syrk_host(int bits, const void* A, uint64_t rows, uint64_t cols, void* C,
     if (!cols) return LF_OK;
     const int N = bits / 32;
     const std::size_t E = bits / 8 + 12;
-    HostStage st;
-    GEMM_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking), "hipStreamCreate");
+    HostStage& st = host_stage();
+    GEMM_HIP(st.ready(), "hipStreamCreate");
     const std::size_t na = rows * cols, nc = cols * cols;
-    GEMM_HIP(st.aos.alloc(std::max(na, nc) * E), "hipMalloc (staging)");
-    GEMM_HIP(st.pa.alloc(lf_planes_bytes(bits, 0, std::max<uint64_t>(na, 1))), "hipMalloc (A)");
-    GEMM_HIP(st.pc.alloc(lf_planes_bytes(bits, 0, nc)), "hipMalloc (C)");
-    lf_operand OA{st.pa.p, lf_plane_pitch(std::max<uint64_t>(na, 1))}, OC{st.pc.p, lf_plane_pitch(nc)};
+    PoolBuf aos{13}, pa{14}, pc{16};
+    GEMM_HIP(aos.alloc((na + nc) * E), "hipMalloc (staging)");
+    GEMM_HIP(pa.alloc(lf_planes_bytes(bits, 0, std::max<uint64_t>(na, 1))), "hipMalloc (A)");
+    GEMM_HIP(pc.alloc(lf_planes_bytes(bits, 0, nc)), "hipMalloc (C)");
+    GEMM_HIP(st.pin((na + nc) * E), "hipHostMalloc (staging)");
+    char* daos = static_cast<char*>(aos.p);
+    lf_operand OA{pa.p, lf_plane_pitch(std::max<uint64_t>(na, 1))}, OC{pc.p, lf_plane_pitch(nc)};
     if (na) {
-        GEMM_HIP(hipMemcpyAsync(st.aos.p, A, na * E, hipMemcpyHostToDevice, st.stream), "upload A");
-        if (int r = lf_aos_to_planes(bits, 0, st.aos.p, &OA, na, st.stream)) return r;
+        GEMM_HIP(stage_upload(st, 0, daos, A, na * E), "upload A");
+        if (int r = lf_aos_to_planes(bits, 0, daos, &OA, na, st.stream)) return r;
     }
     if (subtract || lower_only) { // untouched entries keep their old values
-        GEMM_HIP(hipMemcpyAsync(st.aos.p, C, nc * E, hipMemcpyHostToDevice, st.stream), "upload C");
-        if (int r = lf_aos_to_planes(bits, 0, st.aos.p, &OC, nc, st.stream)) return r;
+        GEMM_HIP(stage_upload(st, na * E, daos + na * E, C, nc * E), "upload C");
+        if (int r = lf_aos_to_planes(bits, 0, daos + na * E, &OC, nc, st.stream)) return r;
     }
     Product P{};
     P.bits = bits;
@@ -1058,9 +1293,6 @@ int lf_syrk_host(int bits, const void* A, uint64_t rows, uint64_t cols, void* C,
     // line i = column i of A: element (i, kk) = A[kk*cols + i]
     P.left = {(const uint32_t*)OA.planes, OA.pitch, 1, cols};
     P.right = P.left;
-    const uint32_t* a32 = (const uint32_t*)A;
-    P.lf_ = {a32 + N, a32 + N + 1, a32 + N + 2, uint64_t(N + 3), 1, cols};
-    P.rf_ = P.lf_;
     P.m = cols;
     P.n = cols;
     P.K = rows;
@@ -1069,9 +1301,8 @@ int lf_syrk_host(int bits, const void* A, uint64_t rows, uint64_t cols, void* C,
     P.stream = st.stream;
     float ms = 0;
     if (int r = run_product(P, report, &ms)) return r;
-    if (int r = lf_planes_to_aos(bits, 0, &OC, st.aos.p, nc, st.stream)) return r;
-    GEMM_HIP(hipMemcpyAsync(C, st.aos.p, nc * E, hipMemcpyDeviceToHost, st.stream), "download C");
-    GEMM_HIP(hipStreamSynchronize(st.stream), "device execution (linalg)");
+    if (int r = lf_planes_to_aos(bits, 0, &OC, daos, nc, st.stream)) return r;
+    GEMM_HIP(stage_download(st, C, daos, nc * E), "download C");
     if (gpu_seconds) *gpu_seconds = ms * 1e-3;
     if (wall_seconds) *wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     return LF_OK;

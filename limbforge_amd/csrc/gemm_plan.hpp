@@ -11,30 +11,14 @@
 
 namespace lf {
 
-// Host field access for the band analysis: element (line, k) is record line*ls + k*ks; its exponent,
-// sign and status words are at the given pointers plus record * stride words.
-struct Fields {
-    const uint32_t* exponent;
-    const uint32_t* sign;
-    const uint32_t* status;
-    uint64_t stride, ls, ks;
-    void get(std::size_t line, std::size_t k, int& e, int& s, uint32_t& st) const
-    {
-        const uint64_t i = (uint64_t(line) * ls + uint64_t(k) * ks) * stride;
-        e = int(exponent[i]);
-        s = int(sign[i]);
-        st = status[i];
-    }
-};
-
 // C (m x n, element (i, j) at plane element i*out_stride + j of C.planes) = or -= left * right^T, where
-// left / right are device views of the m rows of op(A) and the n columns of B; lf_ / rf_ read the same
-// elements on the host for the analysis.  syrk: right = left, outputs i >= j (mirrored unless lower_only).
+// left / right are device views of the m rows of op(A) and the n columns of B (the band analysis reads their
+// exponent, sign and status planes on the device).  syrk: right = left, outputs i >= j (mirrored unless
+// lower_only).
 struct Product {
     int bits;
     bool syrk, lower_only, subtract;
     MatView left, right;
-    Fields lf_, rf_;
     std::size_t m, n, K;
     lf_operand C;
     lf_linalg_options opt;

@@ -968,16 +968,41 @@ template <int K, int P, int W> __device__ __forceinline__ void bits_at(const u32
 // iteration for the reciprocal square root, z <- z (3 - x z^2) / 2, in M-word fixed point.  Every term
 // stays non-negative (x z^2 is near 1, so 3 - x z^2 is near 2), so each step is three integer products
 // and a subtraction from a constant, with no normalization shifts; the step at M words refines the result
-// at M/2 + 1 words (each step doubles the accurate bits), down to two words seeded from the double
-// reciprocal square root of X's top 64 bits.  Z <= 2^(F+1) * (1 + tiny) fits in M words.
+// at M/2 + 1 words (each step doubles the accurate bits), down to two words seeded by rsqrt_seed from X's
+// top 64 bits.  Z <= 2^(F+1) * (1 + tiny) fits in M words.
+//
+// rsqrt_seed: z ~ 2^61 / sqrt(x) for x = top / 2^64 in [1/4, 1), in integer arithmetic: a two-piece
+// constant start (relative error below 0.2) refined by five Newton steps z <- z (3 - x z^2) / 2 in 64-bit
+// fixed point with 2^61 scaling (products through 128-bit intermediates; each step doubles the accurate
+// bits: about 2.5, 5, 10, 20, 40 and then the 60 the format holds, far above the 31 that the two-word step
+// of rsqrt_fixed needs to continue the doubling).
+__device__ __forceinline__ u64 mul_hi64(u64 a, u64 b)
+{
+    const u64 al = (u32)a, ah = a >> 32, bl = (u32)b, bh = b >> 32;
+    const u64 ll = al * bl, lh = al * bh, hl = ah * bl, hh = ah * bh;
+    const u64 mid = (ll >> 32) + (u32)lh + (u32)hl;
+    return hh + (lh >> 32) + (hl >> 32) + (mid >> 32);
+}
+__device__ __forceinline__ u64 rsqrt_seed(u64 top)
+{
+    const u64 three = (u64)3 << 61;
+    u64 z = top >= ((u64)1 << 63) ? (u64)0x2666666666666666ull /* 1.2 * 2^61 */ : (u64)0x3666666666666666ull; // 1.7 * 2^61
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        const u64 z2 = mul_hi64(z, z) << 3;        // z^2 * 2^61 (z^2 <= 4: below 2^63)
+        const u64 xz2 = mul_hi64(top, z2);         // x z^2 * 2^61
+        const u64 t = three - xz2;                 // (3 - x z^2) * 2^61, near 2 * 2^61
+        z = mul_hi64(z, t) << 2;                   // z (3 - x z^2) / 2 * 2^61
+    }
+    return z;
+}
+
 template <int M, int W> __device__ void rsqrt_fixed(const u32 (&X)[W], u32 (&Z)[M])
 {
     constexpr int F = 32 * M - 3, DM = digits_of_words<M>();
     if constexpr (M <= 2) {
         static_assert(M == 2, "the seed fills two words");
-        const u64 top = ((u64)X[W - 1] << 32) | X[W - 2];
-        const double z0 = 1.0 / __builtin_sqrt((double)top * 0x1p-64); // in (1, 2]
-        const u64 z = (u64)(z0 * 0x1p61);
+        const u64 z = rsqrt_seed(((u64)X[W - 1] << 32) | X[W - 2]);
         Z[0] = (u32)z;
         Z[1] = (u32)(z >> 32);
     } else {

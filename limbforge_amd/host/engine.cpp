@@ -224,12 +224,18 @@ struct CommandBatch::Impl {
     std::chrono::steady_clock::time_point start = std::chrono::steady_clock::now();
     bool submitted = false;
     explicit Impl(std::shared_ptr<Engine::Impl> e) : engine(std::move(e)) {}
+    // outputs of resident products encoded earlier in this batch (docs/numerics.md "Resident products")
+    std::vector<std::pair<std::shared_ptr<detail::BufferStorage>, std::shared_ptr<std::atomic<bool>>>> provisional;
     void retain(const std::shared_ptr<detail::BufferStorage>& r, bool written)
     {
         if (!r || r->owner != engine->identity) throw std::invalid_argument("buffer belongs to another engine");
         if (r->busy.load(std::memory_order_acquire)) throw std::logic_error("buffer belongs to an unwaited submission");
+        for (auto& p : provisional)
+            if (p.first == r) {
+                if (written) throw std::logic_error("the output of a resident product is provisional within its batch: split the batch");
+                p.second->store(true);
+            }
         if (std::find(resources.begin(), resources.end(), r) == resources.end()) resources.push_back(r);
-        (void)written;
     }
     // run(mul, x, y, x) repeated on the same buffers becomes one resident chain launch
     void fuse_chains()
@@ -269,6 +275,8 @@ void CommandBatch::encode(int bits, bool complex, Operation op, const std::share
     if (!impl_ || impl_->submitted) throw std::logic_error("batch already submitted or moved");
     validate(bits, count);
     if (!operation_is_valid(op) || operation_is_complex(op) != complex) throw std::invalid_argument("operation and buffer format mismatch");
+    if (op == Operation::complex_fma || op == Operation::complex_fms)
+        throw std::invalid_argument("complex_fma / complex_fms are not run on the GPU by this library; use the host functions cfma / cfms");
     if (!b_index.stride || !c_index.stride) throw std::invalid_argument("broadcast stride must be positive");
     for (auto* r : {&a, &b, &c})
         if (*r && (*r)->bits != bits) throw std::invalid_argument("buffer precision does not match the operation");
@@ -428,6 +436,11 @@ void BatchAccess::on_completion(CommandBatch& b, std::function<void()> f)
 {
     if (!b.impl_ || b.impl_->submitted) throw std::logic_error("batch already submitted or moved");
     b.impl_->completions.push_back(std::move(f));
+}
+void BatchAccess::mark_provisional(CommandBatch& b, const std::shared_ptr<BufferStorage>& r, std::shared_ptr<std::atomic<bool>> read_flag)
+{
+    if (!b.impl_ || b.impl_->submitted) throw std::logic_error("batch already submitted or moved");
+    b.impl_->provisional.emplace_back(r, std::move(read_flag));
 }
 std::shared_ptr<int> BatchAccess::identity(const Engine& e) { return e.impl->identity; }
 std::shared_ptr<BufferStorage> BatchAccess::allocate(Engine& e, std::size_t count, int bits)

@@ -52,6 +52,9 @@ struct BatchAccess {
     static void retain(CommandBatch& b, const std::shared_ptr<BufferStorage>& r, bool written);
     static void enqueue(CommandBatch& b, Launch launch);
     static void on_completion(CommandBatch& b, std::function<void()> f);
+    // r stays provisional for the rest of the batch: a later write to it throws std::logic_error, a later
+    // read sets *read_flag
+    static void mark_provisional(CommandBatch& b, const std::shared_ptr<BufferStorage>& r, std::shared_ptr<std::atomic<bool>> read_flag);
     static std::shared_ptr<int> identity(const Engine& e);
     static std::shared_ptr<BufferStorage> allocate(Engine& e, std::size_t count, int bits);
 };

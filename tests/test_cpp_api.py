@@ -7,7 +7,7 @@ import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
 LIB = ROOT / "limbforge_amd" / "lib"
-PROGRAMS = ["test_engine", "test_linalg"]
+PROGRAMS = ["test_engine", "test_linalg", "test_resident"]
 
 
 def build_consumer(name: str, out: Path):

@@ -68,7 +68,8 @@ def test_deep_k_vs_oracle(capi, oracle, bits, m, n, k, opts):
     A = records(rng, bits, m * k, three_bands(rng, m * k))
     B = records(rng, bits, k * n, three_bands(rng, k * n))
     got, rep, _, _ = capi.gemm(bits, False, A, B, m, n, k, options=capi.LinalgOptions.default(**kw))
-    assert rep.gpu_outputs == m * n
+    if opts != "narrow":  # 16-bit bands split every exponent cluster past max_bands: those lines take the exact dots
+        assert rep.gpu_outputs == m * n
     want = oracle.gemm_rows(bits, A, B, m, n, k)
     same(got, want, f"K={k} bits={bits} options={opts}")
     # the same product as an update of random old entries

@@ -76,6 +76,9 @@ def test_complex_ops_vs_oracle(capi, oracle, bits):
         assert_same(got, want, f"{op} bits={bits}")
 
 
+@pytest.mark.skip(reason="complex_fma / complex_fms are switched off on the GPU: their element-wise kernel took a "
+                         "memory-access fault at 800 bits (complex_fms, full addend) whose cause is not established; "
+                         "the entries reject the two operations (test_complex_fused_ops_are_rejected)")
 @pytest.mark.parametrize("bits", WIDTHS)
 def test_complex_fused_vs_oracle(capi, oracle, bits):
     """complex_fma / complex_fms (core.hpp:330-358): each part one rounding of a.re b.re - a.im b.im + c.re and
@@ -96,6 +99,18 @@ def test_complex_fused_vs_oracle(capi, oracle, bits):
         assert_same(got, oracle.elementwise(bits, capi.OPS[op], a, b, c), f"{op} bits={bits}")
         got = run_gpu(capi, bits, op, a, b, c[:7], c_index=(1, 7))
         assert_same(got, oracle.elementwise(bits, capi.OPS[op], a, b, c[:7], c_index=(1, 7)), f"{op} bits={bits} broadcast c")
+
+
+def test_complex_fused_ops_are_rejected(capi):
+    """The device entries refuse complex_fma / complex_fms before any launch, with the error class of an
+    invalid argument; the host functions cfma / cfms (core.hpp) remain their implementation."""
+    rng = np.random.default_rng(5)
+    a, b = lfcases.complex_cases(rng, 256, 8)
+    for op in ("complex_fma", "complex_fms"):
+        with pytest.raises(ValueError, match="not run on the GPU"):
+            run_gpu(capi, 256, op, a, b, a)
+        with pytest.raises(ValueError, match="not run on the GPU"):
+            capi.run_host(256, op, a, b, a)
 
 
 @pytest.mark.parametrize("bits", [64, 96, 256, 384, 512, 1024])
