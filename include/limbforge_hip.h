@@ -120,6 +120,18 @@ int lf_mul_chain(int bits, const lf_operand* x, const lf_operand* f, const lf_op
 int lf_run_host(int bits, int op, const void* a, const void* b, const void* c, void* out, uint64_t count,
                 const lf_broadcast* b_index, const lf_broadcast* c_index, double* gpu_seconds, double* wall_seconds);
 
+/* No counterpart in the reference.  lf_run_host for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc): the same operation and results, with the device staging in a
+ * static per-GPU array of lf_noalloc_scratch_bytes() bytes (64 MiB) held until the process ends; no call
+ * allocates or frees device memory.  A batch whose staging (a, b, c and out rounded up to 256 bytes each)
+ * exceeds the array is rejected with LF_INVALID_ARGUMENT before any device work.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race.  stream may be NULL (the default stream);
+ * returns after the stream has finished. */
+int lf_run_host_noalloc(int bits, int op, const void* a, const void* b, const void* c, void* out, uint64_t count,
+                        const lf_broadcast* b_index, const lf_broadcast* c_index, void* stream);
+/* The size of lf_run_host_noalloc's per-GPU staging array, in bytes. */
+int lf_noalloc_scratch_bytes(uint64_t* bytes);
+
 /* Precision cast between resident operands: CommandBatch::cast (engine.hpp:170-176) and the kernels
  * cast_real / cast_complex (kernels.metal:251-266): exact widening, RN-even narrowing, componentwise for
  * complex values; statuses kept, exponent_overflow when rounding leaves the range. */
