@@ -12,6 +12,7 @@
 #include "gemm_kernels.hpp"
 #include "gemm_launch.hpp"
 #include "gemm_plan.hpp"
+#include "residue_reduce.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -34,26 +35,17 @@ constexpr int PT = 128;       // output tile per workgroup (rows and columns)
 constexpr int PK = 32;        // K per MFMA step
 constexpr u32 XCDS = 8;       // MI355X: 8 XCDs, workgroups dealt to them round-robin by linear id
 
-// One 32x32 block of residues: (65536 G11 + 256 Gmid + G00) mod m, written as uint16.  The combination
-// is an integer below 2^48 in magnitude, exact in double precision (as are the two fused multiply-adds
-// forming it); q = floor(t / m) from the double reciprocal is off by at most one, t - q m is then exact
-// (an integer below 2m in magnitude, from one fused multiply-add) and one correction each way lands it
-// in [0, m).  About 13 VALU instructions per residue, FP64 at full rate on this GPU, against 45 for
-// three 32-bit Barrett reductions and their combination.
+// One 32x32 block of residues: (65536 G11 + 256 Gmid + G00) mod m, written as uint16, in 32-bit integer
+// and FP32 arithmetic (residue_combine).
 __device__ __forceinline__ void store_block(uint16_t* out, u32 Np, u32 row, u32 col, u32 lane, const v16i& c11, const v16i& c00,
-                                            const v16i& cmid, int m, double md, double inv)
+                                            const v16i& cmid, int m, float inv, int c32)
 {
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         // 32x32 accumulator layout: lane l, element i -> row 8(i/4) + 4(l/32) + i%4, column l%32
         const u32 rr = row + 8u * (u32)(i / 4) + 4u * (lane >> 5) + (u32)(i % 4);
         const u32 cc = col + (lane & 31u);
-        const double t = __builtin_fma((double)c11[i], 65536.0, __builtin_fma((double)cmid[i], 256.0, (double)c00[i]));
-        const double q = __builtin_floor(t * inv);
-        int r = (int)__builtin_fma(-q, md, t);
-        r += r < 0 ? m : 0;
-        r -= r >= m ? m : 0;
-        out[(u64)rr * Np + cc] = (uint16_t)r;
+        out[(u64)rr * Np + cc] = (uint16_t)residue_combine(c11[i], cmid[i], c00[i], m, inv, c32);
     }
 }
 
@@ -138,16 +130,17 @@ __global__ void __launch_bounds__(256, 2) product_kernel(ProductArgs g)
             }
     }
     const int m = (int)g.moduli[q];
-    const double md = (double)m, inv = 1.0 / md;
+    const float inv = 1.0f / (float)m;
+    const int c32u = (int)(((u64)1 << 32) % (u64)m), c32 = c32u > m / 2 ? c32u - m : c32u; // balanced 2^32 mod m
     uint16_t* out = g.res + (u64)q * g.Mp * g.Np;
     // one block at a time (the scheduling barriers keep the accumulators in AGPRs until their turn)
-    store_block(out, g.Np, row0 + wr, col0 + wc, lane, c11[0][0], c00[0][0], cmid[0][0], m, md, inv);
+    store_block(out, g.Np, row0 + wr, col0 + wc, lane, c11[0][0], c00[0][0], cmid[0][0], m, inv, c32);
     __builtin_amdgcn_sched_barrier(0);
-    store_block(out, g.Np, row0 + wr, col0 + wc + 32u, lane, c11[0][1], c00[0][1], cmid[0][1], m, md, inv);
+    store_block(out, g.Np, row0 + wr, col0 + wc + 32u, lane, c11[0][1], c00[0][1], cmid[0][1], m, inv, c32);
     __builtin_amdgcn_sched_barrier(0);
-    store_block(out, g.Np, row0 + wr + 32u, col0 + wc, lane, c11[1][0], c00[1][0], cmid[1][0], m, md, inv);
+    store_block(out, g.Np, row0 + wr + 32u, col0 + wc, lane, c11[1][0], c00[1][0], cmid[1][0], m, inv, c32);
     __builtin_amdgcn_sched_barrier(0);
-    store_block(out, g.Np, row0 + wr + 32u, col0 + wc + 32u, lane, c11[1][1], c00[1][1], cmid[1][1], m, md, inv);
+    store_block(out, g.Np, row0 + wr + 32u, col0 + wc + 32u, lane, c11[1][1], c00[1][1], cmid[1][1], m, inv, c32);
 }
 
 hipError_t launch_product(const ProductArgs& g, hipStream_t s)

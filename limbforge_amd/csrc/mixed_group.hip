@@ -3,8 +3,7 @@
 // of its widths: workgroup b finds its bucket in the table of per-bucket block offsets and runs that
 // bucket's width.  The four group kernels of a batch are the kernel nodes of one hipGraph, so a batch of
 // up to 31 precisions is four concurrent launches, each sized by its own widths' register needs.
-#include "elementwise_impl.hpp"
-#include "lf_internal.hpp"
+#include "mixed_chain.hpp"
 
 #ifndef LF_GROUP
 #error "compile with -DLF_GROUP=<0..3>"
@@ -15,22 +14,6 @@ namespace lf {
 namespace {
 constexpr int GLO = LF_GROUP == 0 ? 2 : 8 * LF_GROUP + 1;
 constexpr int GHI = 8 * LF_GROUP + 8;
-
-template <int N> __device__ __forceinline__ void chain_value(const MixedArgs& a, u32 i)
-{
-    Num<N> x = load_num<N>(a.x, a.pitch, i);
-    const Num<N> f = load_num<N>(a.f, a.pitch, i);
-    const Num<N> g = load_num<N>(a.g, a.pitch, i);
-    const u32 len = a.len[i];
-    const u64 mask = a.mask[i];
-    if constexpr (use_digits<N>()) {
-        u32 fd[digit_count<N>()];
-        to_digits<N>(f.w, fd);
-        for (u32 t = 0; t < len; ++t) x = fma_by_digits<N>(x, f, fd, ((mask >> t) & 1u) ? g : zero<N>(ST_OK));
-    } else
-        for (u32 t = 0; t < len; ++t) x = fma<N>(x, f, ((mask >> t) & 1u) ? g : zero<N>(ST_OK));
-    store_num<N>(a.out, a.pitch, i, x);
-}
 
 template <int N> __device__ __forceinline__ void dispatch(int n, const MixedArgs& a, u32 i)
 {

@@ -3,11 +3,13 @@
 // tests/test_device_core_cpu.py with LF_HOST_BUILD defined.
 #include "dot2_core.hpp"
 #include "limb_core.hpp"
+#include "residue_reduce.hpp"
 #include "lf_oracle.h"
 #include "lfo_cases.h"
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
+#include <vector>
 
 static long failures = 0, checks = 0;
 
@@ -236,10 +238,57 @@ template <int N> void run_all(int lo, int hi, int cases)
     if constexpr (N < 32) run_all<N + 1>(lo, hi, cases);
 }
 
+// The residue GEMM's epilogue, (65536 G11 + 256 Gmid + G00) mod m, against exact 64-bit remainders over
+// every modulus the planner can use (the 1200 largest primes <= 65279), at the accumulators' extremes
+// (|G11|, |G00| <= 65472 * 2^14, |Gmid| <= 2 * 65472 * 2^14), near multiples of m and random values.
+static void check_residue_combine()
+{
+    std::vector<int> primes;
+    for (int q = 65279; primes.size() < 1200; --q) {
+        bool p = true;
+        for (int d = 2; d * d <= q; ++d)
+            if (q % d == 0) {
+                p = false;
+                break;
+            }
+        if (p) primes.push_back(q);
+    }
+    uint64_t state = 0x9E3779B97F4A7C15ull;
+    auto next = [&] {
+        state ^= state << 13;
+        state ^= state >> 7;
+        state ^= state << 17;
+        return state;
+    };
+    const int64_t B = 65472LL * 16384, BM = 2 * B;
+    const int64_t ends[] = {0, 1, -1, B, -B, B - 1, -B + 1, 1 << 29, -(1 << 29)};
+    const int64_t mids[] = {0, 1, -1, BM, -BM, BM - 1, -BM + 1, 1 << 30, -(1 << 30)};
+    auto check = [&](int64_t a, int64_t b, int64_t c, int m) {
+        ++checks;
+        const int c32 = int((uint64_t(1) << 32) % uint64_t(m)), cb = c32 > m / 2 ? c32 - m : c32;
+        const uint32_t r = lf::residue_combine((int)a, (int)b, (int)c, m, 1.0f / (float)m, cb);
+        const int64_t t = a * 65536 + b * 256 + c, want = ((t % m) + m) % m;
+        if ((int64_t)r != want && ++failures <= 30)
+            std::printf("MISMATCH residue_combine g11=%lld gmid=%lld g00=%lld m=%d: %u, want %lld\n", (long long)a, (long long)b, (long long)c, m, r,
+                        (long long)want);
+    };
+    for (int m : primes) {
+        for (int64_t a : ends)
+            for (int64_t b : mids)
+                for (int64_t c : ends) check(a, b, c, m);
+        for (int k = 0; k < 200; ++k) {
+            check(int64_t(next() % uint64_t(2 * B + 1)) - B, int64_t(next() % uint64_t(2 * BM + 1)) - BM, int64_t(next() % uint64_t(2 * B + 1)) - B, m);
+            const int64_t a = int64_t(next() % 32768) * m + int64_t(next() % 3) - 1;
+            check(a <= B ? a : -m, int64_t(next() % 1024) * m + m / 2, int64_t(next() % 4096) * m + (m - 1) / 2, m);
+        }
+    }
+}
+
 int main(int argc, char** argv)
 {
     int lo = argc > 1 ? std::atoi(argv[1]) : 2, hi = argc > 2 ? std::atoi(argv[2]) : 32, cases = argc > 3 ? std::atoi(argv[3]) : 300;
     run_all<2>(lo, hi, cases);
+    check_residue_combine();
     std::printf("device core (host build): %ld checks, %ld failures\n", checks, failures);
     return failures ? 1 : 0;
 }
