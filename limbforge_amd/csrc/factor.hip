@@ -4,14 +4,15 @@
 //
 // The work matrix stays in limb planes on the device for the whole call.  Per column block [k0, k1):
 //   1. the diagonal block, column by column in one workgroup (pivot, sqrt, then div of the entries below);
-//   2. the rows below it, one thread per row (chol_panel_kernel);
-//   3. the pivot record to the host, then the trailing update a_ij = D(a_ij; l_i, l_j; [k0, k1)) for
-//      i >= j >= k1 as one SYRK product of the update GEMM (band analysis on the device, residue GEMM on int8
-//      MFMA plus exact-dot lines, gemm.hip) written in place.
-// The reference splits step 3 into the next panel's columns and the rest and overlaps the rest with the
-// next panel on a second host thread; every entry is one rounding of the same exact value either way, so
-// one in-place product gives the same bits.  The solves run block by block the same way: the in-block
-// recurrence on one thread per right-hand side, the off-block updates as products of the update GEMM.
+//   2. the rows below it, one wavefront per row (chol_panel_kernel);
+//   3. the trailing update a_ij = D(a_ij; l_i, l_j; [k0, k1)) for i >= j >= k1, in two parts as the
+//      reference splits it (linalg.mm:440-452): the next panel's columns [k1, k2) by the look-ahead kernel
+//      on the caller's stream, followed there by the next panel, while the rest (i >= j >= k2) runs as one
+//      SYRK product of the update GEMM (band analysis on the device, residue GEMM on int8 MFMA plus
+//      exact-dot lines, gemm.hip) on a second stream.
+// Every entry is one rounding of the same exact value in either part, so the bits are those of the serial
+// order.  The solves run block by block: the in-block recurrence on one wavefront per right-hand side, the
+// off-block updates as products of the update GEMM.
 #include "factor_launch.hpp"
 #include "gemm_plan.hpp"
 #include "limbforge_hip.h"
@@ -63,6 +64,7 @@ struct OwnedStream {
 
 typedef hipError_t (*diag_fn)(const lf::FactorArgs&, uint32_t, hipStream_t);
 typedef hipError_t (*panel_fn)(const lf::FactorArgs&, hipStream_t);
+typedef hipError_t (*lookahead_fn)(const lf::FactorArgs&, uint32_t, uint32_t, hipStream_t);
 typedef hipError_t (*invalid_fn)(uint32_t*, uint64_t, uint32_t, uint32_t, hipStream_t);
 typedef hipError_t (*solve_fn)(int, const lf::SolveArgs&, hipStream_t);
 typedef hipError_t (*recip_fn)(const uint32_t*, uint64_t, uint32_t, uint32_t*, uint64_t, hipStream_t);
@@ -71,6 +73,7 @@ template <int N> void fill_factor(FactorTables& t);
 struct FactorTables {
     diag_fn diag[33] = {};
     panel_fn panel[33] = {};
+    lookahead_fn lookahead[33] = {};
     invalid_fn invalid[33] = {};
     hipError_t (*zero_upper[33])(uint32_t*, uint64_t, uint32_t, hipStream_t) = {};
     solve_fn solve[33] = {};
@@ -81,6 +84,7 @@ template <int N> void fill_factor(FactorTables& t)
 {
     t.diag[N] = lf::launch_chol_diag<N>;
     t.panel[N] = lf::launch_chol_panel<N>;
+    t.lookahead[N] = lf::launch_chol_lookahead<N>;
     t.invalid[N] = lf::launch_chol_invalid<N>;
     t.zero_upper[N] = lf::launch_zero_upper<N>;
     t.solve[N] = lf::launch_solve<N>;
@@ -218,55 +222,124 @@ int lf_cholesky(int bits, const lf_operand* A, uint64_t n, const lf_operand* L, 
     DevBuf drecip;
     const uint64_t rpitch = (n + 63) / 64 * 64;
     FACTOR_HIP(drecip.alloc((N + 1) * rpitch * 4), "hipMalloc (reciprocals)");
-    lf::FactorInfo hinfo{-1, 0, 0, 0};
-    FACTOR_HIP(hipMemcpyAsync(dinfo.p, &hinfo, sizeof hinfo, hipMemcpyHostToDevice, s), "upload pivot");
-    hipEvent_t e0, e1;
-    FACTOR_HIP(hipEventCreate(&e0), "hipEventCreate");
-    FACTOR_HIP(hipEventCreate(&e1), "hipEventCreate");
-    struct EventGuard {
-        hipEvent_t a, b;
-        ~EventGuard()
+    // the pivot record comes back through pinned memory, so that reading it does not wait on the stream
+    struct PinnedInfo {
+        lf::FactorInfo* p = nullptr;
+        ~PinnedInfo()
         {
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
+            if (p) (void)hipHostFree(p);
         }
-    } guard{e0, e1};
-    for (uint64_t k0 = 0; k0 < n;) {
-        const uint64_t k1 = std::min(n, k0 + nb), w = k1 - k0;
-        auto t = Clock::now();
-        lf::FactorArgs f{W, pitch, n32, uint32_t(k0), uint32_t(k1), (lf::FactorInfo*)dinfo.p, (uint32_t*)dscratch.p, swords,
-                         (uint32_t*)drecip.p, rpitch};
-        FACTOR_HIP(hipEventRecord(e0, s), "hipEventRecord");
-        FACTOR_HIP(tables().diag[N](f, uint32_t(std::min<uint64_t>(256, (w + 63) / 64 * 64)), s), "panel launch");
+    } hinfo;
+    FACTOR_HIP(hipHostMalloc((void**)&hinfo.p, sizeof(lf::FactorInfo), hipHostMallocDefault), "hipHostMalloc (pivot)");
+    *hinfo.p = lf::FactorInfo{-1, 0, 0, 0};
+    FACTOR_HIP(hipMemcpyAsync(dinfo.p, hinfo.p, sizeof(lf::FactorInfo), hipMemcpyHostToDevice, s), "upload pivot");
+    // Look-ahead (the reference overlaps the next panel with the rest of the update on a second host thread,
+    // linalg.mm:440-452): after the panel of block [k0, k1), the entries of the next panel's columns
+    // [k1, k2) are updated by a look-ahead kernel on stream s, the next panel follows on s, and the rest of
+    // the trailing update (rows and columns from k2) runs on a second stream at the same time.  Every entry
+    // still gets one rounding of the same exact value per block, so the bits are those of the serial order.
+    OwnedStream upd;
+    FACTOR_HIP(hipStreamCreateWithFlags(&upd.s, hipStreamNonBlocking), "hipStreamCreate");
+    hipEvent_t ev[6];
+    for (auto& e : ev) e = nullptr;
+    struct EventsGuard {
+        hipEvent_t* e;
+        ~EventsGuard()
+        {
+            for (int i = 0; i < 6; ++i)
+                if (e[i]) (void)hipEventDestroy(e[i]);
+        }
+    } guard{ev};
+    for (auto& e : ev) FACTOR_HIP(hipEventCreate(&e), "hipEventCreate");
+    hipEvent_t &p0 = ev[0], &p1 = ev[1], &panel_done = ev[2], &rest_done = ev[3], &l0 = ev[4], &l1 = ev[5];
+    const uint32_t waves = uint32_t(threads);
+    bool rest_pending = false;
+    auto args = [&](uint64_t a, uint64_t b) {
+        return lf::FactorArgs{W, pitch, n32, uint32_t(a), uint32_t(b), (lf::FactorInfo*)dinfo.p, (uint32_t*)dscratch.p, swords,
+                              (uint32_t*)drecip.p, rpitch};
+    };
+    // the panel of block [a, b) on s, its pivot record on its way to the host, panel_done recorded
+    auto panel = [&](uint64_t a, uint64_t b) -> int {
+        const lf::FactorArgs f = args(a, b);
+        FACTOR_HIP(hipEventRecord(p0, s), "hipEventRecord");
+        FACTOR_HIP(tables().diag[N](f, uint32_t(std::min<uint64_t>(256, (b - a + 63) / 64 * 64)), s), "panel launch");
         FACTOR_HIP(tables().panel[N](f, s), "panel launch");
-        FACTOR_HIP(hipEventRecord(e1, s), "hipEventRecord");
-        FACTOR_HIP(hipMemcpyAsync(&hinfo, dinfo.p, sizeof hinfo, hipMemcpyDeviceToHost, s), "download pivot");
-        FACTOR_HIP(hipStreamSynchronize(s), "device execution (panel)");
+        FACTOR_HIP(hipEventRecord(p1, s), "hipEventRecord");
+        FACTOR_HIP(hipMemcpyAsync(hinfo.p, dinfo.p, sizeof(lf::FactorInfo), hipMemcpyDeviceToHost, s), "download pivot");
+        FACTOR_HIP(hipEventRecord(panel_done, s), "hipEventRecord");
+        return LF_OK;
+    };
+    // waits for the panel just enqueued; LF_OK and `failed` set when its pivot failed
+    auto finish_panel = [&](Clock::time_point t, bool& failed) -> int {
+        FACTOR_HIP(hipEventSynchronize(panel_done), "device execution (panel)");
         float ms = 0;
-        FACTOR_HIP(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        FACTOR_HIP(hipEventElapsedTime(&ms, p0, p1), "hipEventElapsedTime");
         ci.timing_gpu_seconds += ms * 1e-3;
         ci.panel_seconds += since(t);
         ++ci.blocks;
-        if (hinfo.error) return fail(LF_LOGIC_ERROR, "cholesky: exact-dot scratch overflow");
-        if (hinfo.pivot >= 0) {
-            ci.pivot = uint64_t(hinfo.pivot);
-            ci.pivot_sign = hinfo.sign;
-            ci.pivot_status = hinfo.status;
-            FACTOR_HIP(tables().invalid[N](W, pitch, n32, uint32_t(hinfo.pivot), s), "invalid launch");
+        if (hinfo.p->error) return fail(LF_LOGIC_ERROR, "cholesky: exact-dot scratch overflow");
+        failed = hinfo.p->pivot >= 0;
+        if (failed) {
+            ci.pivot = uint64_t(hinfo.p->pivot);
+            ci.pivot_sign = hinfo.p->sign;
+            ci.pivot_status = hinfo.p->status;
+            if (rest_pending) FACTOR_HIP(hipStreamWaitEvent(s, rest_done, 0), "hipStreamWaitEvent");
+            FACTOR_HIP(tables().invalid[N](W, pitch, n32, uint32_t(hinfo.p->pivot), s), "invalid launch");
             FACTOR_HIP(hipStreamSynchronize(s), "device execution");
-            break;
         }
-        if (k1 < n) {
-            t = Clock::now();
-            const lf::MatView v{W + (k1 * n + k0), pitch, n, 1};
-            if (int r = update(bits, lo, true, v, v, n - k1, n - k1, w, W + (k1 * n + k1), pitch, n, s, ms))
-                return r;
+        return LF_OK;
+    };
+    {
+        const auto t = Clock::now();
+        if (int r = panel(0, std::min(n, nb))) return r;
+        bool failed = false;
+        if (int r = finish_panel(t, failed)) return r;
+        if (failed) {
+            ci.timing_wall_seconds = since(start);
+            if (info) *info = ci;
+            return LF_OK;
+        }
+    }
+    for (uint64_t k0 = 0; k0 < n;) {
+        const uint64_t k1 = std::min(n, k0 + nb), w = k1 - k0;
+        if (k1 >= n) break;
+        const uint64_t k2 = std::min(n, k1 + nb);
+        // (1) the next panel's columns, after the previous block's rest of the update
+        auto t = Clock::now();
+        if (rest_pending) FACTOR_HIP(hipStreamWaitEvent(s, rest_done, 0), "hipStreamWaitEvent");
+        FACTOR_HIP(hipEventRecord(l0, s), "hipEventRecord");
+        FACTOR_HIP(tables().lookahead[N](args(k0, k1), uint32_t(k2), waves, s), "look-ahead launch");
+        FACTOR_HIP(hipEventRecord(l1, s), "hipEventRecord");
+        // (2) the next panel on s
+        const auto tp = Clock::now();
+        if (int r = panel(k1, k2)) return r;
+        // (3) the rest of this block's update on the second stream, after this block's panel (the look-ahead
+        // and the next panel write only columns [k1, k2), which this part neither reads nor writes)
+        rest_pending = false;
+        if (k2 < n) {
+            FACTOR_HIP(hipStreamWaitEvent(upd.s, l0, 0), "hipStreamWaitEvent");
+            const lf::MatView v{W + (k2 * n + k0), pitch, n, 1};
+            float ms = 0;
+            if (int r = update(bits, lo, true, v, v, n - k2, n - k2, w, W + (k2 * n + k2), pitch, n, upd.s, ms)) return r;
+            FACTOR_HIP(hipEventRecord(rest_done, upd.s), "hipEventRecord");
+            rest_pending = true;
             ci.timing_gpu_seconds += ms * 1e-3;
-            ci.update_seconds += since(t);
             ++ci.gpu_updates;
         }
+        ci.update_seconds += since(t);
+        {
+            float ms = 0;
+            FACTOR_HIP(hipEventSynchronize(l1), "device execution (look-ahead)");
+            FACTOR_HIP(hipEventElapsedTime(&ms, l0, l1), "hipEventElapsedTime");
+            ci.timing_gpu_seconds += ms * 1e-3;
+        }
+        // (4) the next panel's pivot
+        bool failed = false;
+        if (int r = finish_panel(tp, failed)) return r;
+        if (failed) break;
         k0 = k1;
     }
+    if (rest_pending) FACTOR_HIP(hipStreamSynchronize(upd.s), "device execution (update)");
     ci.timing_wall_seconds = since(start);
     if (info) *info = ci;
     return LF_OK;

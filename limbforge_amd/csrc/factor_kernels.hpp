@@ -589,6 +589,27 @@ template <int N> __global__ void __launch_bounds__(64) chol_panel_kernel(FactorA
     }
 }
 
+// Look-ahead part of a block's trailing update: after the panel [k0, k1), the entries (i, j), j <= i, of the
+// next panel's columns j in [k1, k2) and rows i in [k1, n) become D(a_ij; l_i, l_j; [k0, k1)) in place -- the
+// same one rounding of the same exact value as the update product gives them -- so that the next panel can
+// start while the rest of the update (rows and columns from k2) runs on another stream.  One wavefront per
+// entry, grid-stride over the (n - k1) x (k2 - k1) rectangle; wavefront b uses scratch slot b.
+template <int N> __global__ void __launch_bounds__(64) chol_lookahead_kernel(FactorArgs f, u32 k2)
+{
+    __shared__ WaveLds<N> L;
+    const u32 lane = threadIdx.x, cols = k2 - f.k1;
+    const u64 entries = (u64)(f.n - f.k1) * cols;
+    const DotScratch sc{f.scratch + (u64)blockIdx.x * 2 * f.swords, f.swords, &f.info->error};
+    for (u64 e = blockIdx.x; e < entries; e += gridDim.x) {
+        const u32 i = f.k1 + (u32)(e / cols), j = f.k1 + (u32)(e % cols);
+        if (j > i) continue; // the whole wavefront skips the strict upper part
+        const VecView ri{f.w, f.pitch, (i64)i * f.n + f.k0, 1};
+        const VecView rj{f.w, f.pitch, (i64)j * f.n + f.k0, 1};
+        const Num<N> v = wave_dot_sub<N>(element<N>(f, i, j), ri, rj, f.k1 - f.k0, lane, L, sc);
+        if (lane == 0) store_out<N>(f.w, f.pitch, f.n, i, j, false, v);
+    }
+}
+
 // Entries (i, j) with i >= j >= p after a failing pivot p: zero with status invalid.  Thread per entry of
 // the trailing (n-p) x (n-p) square.
 template <int N> __global__ void __launch_bounds__(256) chol_invalid_kernel(u32* w, u64 pitch, u32 n, u32 p)

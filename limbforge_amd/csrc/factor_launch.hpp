@@ -53,6 +53,9 @@ inline uint64_t factor_scratch_words(int N, uint64_t K)
 
 template <int N> hipError_t launch_chol_diag(const FactorArgs& f, uint32_t threads, hipStream_t s);
 template <int N> hipError_t launch_chol_panel(const FactorArgs& f, hipStream_t s);
+// entries (i, j), j <= i, of columns [k1, k2) and rows [k1, n) updated by the block [k0, k1); at most
+// `waves` wavefronts (scratch slots)
+template <int N> hipError_t launch_chol_lookahead(const FactorArgs& f, uint32_t k2, uint32_t waves, hipStream_t s);
 template <int N> hipError_t launch_chol_invalid(uint32_t* w, uint64_t pitch, uint32_t n, uint32_t p, hipStream_t s);
 template <int N> hipError_t launch_zero_upper(uint32_t* w, uint64_t pitch, uint32_t n, hipStream_t s);
 template <int N> hipError_t launch_solve(int backward, const SolveArgs& a, hipStream_t s);

@@ -2,6 +2,8 @@
 #include "factor_kernels.hpp"
 #include "factor_launch.hpp"
 
+#include <algorithm>
+
 #ifndef LF_WORDS
 #error "compile with -DLF_WORDS=<2..32>"
 #endif
@@ -20,6 +22,14 @@ template <int N> hipError_t launch_chol_panel(const FactorArgs& f, hipStream_t s
 {
     if (f.k1 >= f.n) return hipSuccess;
     hipLaunchKernelGGL((chol_panel_kernel<N>), dim3(f.n - f.k1), dim3(WAVE), 0, s, f);
+    return hipGetLastError();
+}
+
+template <int N> hipError_t launch_chol_lookahead(const FactorArgs& f, uint32_t k2, uint32_t waves, hipStream_t s)
+{
+    if (f.k1 >= f.n || k2 <= f.k1 || !waves) return hipSuccess;
+    const uint64_t entries = (uint64_t)(f.n - f.k1) * (k2 - f.k1);
+    hipLaunchKernelGGL((chol_lookahead_kernel<N>), dim3((unsigned)std::min<uint64_t>(entries, waves)), dim3(WAVE), 0, s, f, k2);
     return hipGetLastError();
 }
 
@@ -57,6 +67,7 @@ template <int N> hipError_t launch_reciprocals(const uint32_t* w, uint64_t pitch
 
 template hipError_t launch_chol_diag<LF_WORDS>(const FactorArgs&, uint32_t, hipStream_t);
 template hipError_t launch_chol_panel<LF_WORDS>(const FactorArgs&, hipStream_t);
+template hipError_t launch_chol_lookahead<LF_WORDS>(const FactorArgs&, uint32_t, uint32_t, hipStream_t);
 template hipError_t launch_chol_invalid<LF_WORDS>(uint32_t*, uint64_t, uint32_t, uint32_t, hipStream_t);
 template hipError_t launch_zero_upper<LF_WORDS>(uint32_t*, uint64_t, uint32_t, hipStream_t);
 template hipError_t launch_solve<LF_WORDS>(int, const SolveArgs&, hipStream_t);
