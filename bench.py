@@ -603,6 +603,117 @@ def cholesky_section(oracle_check: bool):
     return res
 
 
+# The reference's published times for its own linalg workloads (Apple M5 Max GPU, milliseconds):
+# benchmarks/results/round20_linalg.csv:16-17 (256-bit QSC-like SYRK and GEMM A^T B, n = 1000, K = 2000) and
+# round23_cholesky.csv:7,9 (normal equations J^T J + damping, Cholesky block 32 and cholesky_solve, 4 RHS).
+REFERENCE_LINALG = {
+    "syrk_1000": {"device_ms": 18.501, "wall_ms": 22.177},
+    "gemm_1000": {"device_ms": 31.945, "wall_ms": 38.345},
+    "cholesky_1000": {"syrk_wall_ms": 31.064, "factor_wall_ms": 163.2, "solve_wall_ms": 73.788, "e2e_wall_ms": 268.09},
+    "cholesky_400": {"syrk_wall_ms": 5.8664, "factor_wall_ms": 37.132, "solve_wall_ms": 8.8681, "e2e_wall_ms": 51.386},
+}
+
+
+def qsc_jacobian(rng, bits: int, rows: int, cols: int) -> np.ndarray:
+    """QSC-like Jacobian columns as the reference's linalg and Cholesky benchmarks draw them
+    (benchmarks/linalg.mm:39-46): column scale in [-60, 60], entries random_number(15) (tests/reference.hpp:21-26)
+    shifted by scale - 10, a quarter of them zero, and in every 17th column (c % 17 == 5) one entry 150 below
+    the column scale.  Row-major rows x cols records; numpy's generator stands in for mt19937_64."""
+    w = bits // 32
+    r = np.zeros((rows, cols, w + 3), dtype=np.uint32)
+    r[:, :, :w] = rng.integers(0, 1 << 32, size=(rows, cols, w), dtype=np.uint64).astype(np.uint32)
+    r[:, :, w - 1] |= np.uint32(0x80000000)
+    scale = rng.integers(-60, 61, size=cols)
+    e = rng.integers(-15, 16, size=(rows, cols)) + scale[None, :] - 10
+    odd = np.nonzero(np.arange(cols) % 17 == 5)[0]
+    e[rng.integers(0, rows, size=odd.size), odd] = scale[odd] - 150
+    r[:, :, w] = (e & 0xFFFFFFFF).astype(np.uint32)
+    r[:, :, w + 1] = (np.where(rng.integers(0, 2, size=(rows, cols)) == 1, 1, -1) & 0xFFFFFFFF).astype(np.uint32)
+    r[rng.integers(0, 4, size=(rows, cols)) == 0] = 0
+    return r.reshape(rows * cols, w + 3)
+
+
+def reference_linalg_section():
+    """The reference's own linalg benchmark workloads at 256 bits, timed as it times them (median of 5 after
+    a warm-up): SYRK C = J^T J (lower) and GEMM C = A^T B of QSC-like Jacobians with n = 1000, K = 2000, and the
+    normal-equation solve: A = J^T J with Marquardt damping a_ii = RN(a_ii + 2^(e_ii - 10)), Cholesky
+    (block 32) and cholesky_solve with 4 right-hand sides (exponent span 20), n = 1000 and 400.  Reports
+    this GPU's times beside the reference's published M5 Max times."""
+    from limbforge_amd import capi
+
+    bits, w = 256, 8
+    rng = np.random.default_rng(20261007 + 20)
+    out = {"data": "QSC-like Jacobians (benchmarks/linalg.mm:39-46), 256 bits", "reference_gpu": "Apple M5 Max"}
+
+    def med(xs):
+        return sorted(xs)[len(xs) // 2]
+
+    n, K = 1000, 2000
+    J = qsc_jacobian(rng, bits, K, n)
+    J2 = qsc_jacobian(rng, bits, K, n)
+    capi.syrk(bits, J, K, n, lower_only=True)
+    runs = [capi.syrk(bits, J, K, n, lower_only=True) for _ in range(5)]
+    sd, sw = med([r[2] for r in runs]) * 1e3, med([r[3] for r in runs]) * 1e3
+    ref = REFERENCE_LINALG["syrk_1000"]
+    out["syrk_1000"] = {"device_ms": round(sd, 3), "wall_ms": round(sw, 3), "moduli": int(runs[0][1].moduli),
+                        "reference_device_ms": ref["device_ms"], "reference_wall_ms": ref["wall_ms"],
+                        "speedup_device": round(ref["device_ms"] / sd, 2), "speedup_wall": round(ref["wall_ms"] / sw, 2)}
+    log(f"[bench] reference workload SYRK n=1000: {out['syrk_1000']}")
+    capi.gemm(bits, True, J, J2, n, n, K)
+    runs = [capi.gemm(bits, True, J, J2, n, n, K) for _ in range(5)]
+    gd, gw = med([r[2] for r in runs]) * 1e3, med([r[3] for r in runs]) * 1e3
+    ref = REFERENCE_LINALG["gemm_1000"]
+    out["gemm_1000"] = {"device_ms": round(gd, 3), "wall_ms": round(gw, 3), "moduli": int(runs[0][1].moduli),
+                        "reference_device_ms": ref["device_ms"], "reference_wall_ms": ref["wall_ms"],
+                        "speedup_device": round(ref["device_ms"] / gd, 2), "speedup_wall": round(ref["wall_ms"] / gw, 2)}
+    log(f"[bench] reference workload GEMM A^T B n=1000: {out['gemm_1000']}")
+    del J2
+
+    for n in (1000, 400):
+        Jn = J if n == 1000 else qsc_jacobian(rng, bits, 2 * n, n)
+        B = np.zeros((n * 4, w + 3), np.uint32)
+        B[:, :w] = rng.integers(0, 1 << 32, size=(n * 4, w), dtype=np.uint64).astype(np.uint32)
+        B[:, w - 1] |= np.uint32(0x80000000)
+        B[:, w] = (rng.integers(-20, 21, size=n * 4) & 0xFFFFFFFF).astype(np.uint32)
+        B[:, w + 1] = (np.where(rng.integers(0, 2, size=n * 4) == 1, 1, -1) & 0xFFFFFFFF).astype(np.uint32)
+        diag = np.arange(n) * (n + 1)
+
+        def normal():
+            t = time.perf_counter()
+            A, _, _, _ = capi.syrk(bits, Jn, 2 * n, n, lower_only=True)
+            d = np.zeros((n, w + 3), np.uint32)
+            nz = A[diag, w + 1] != 0
+            d[nz, w - 1] = 0x80000000
+            d[nz, w] = ((A[diag, w].view(np.int32)[nz].astype(np.int64) - 10) & 0xFFFFFFFF).astype(np.uint32)
+            d[nz, w + 1] = 1
+            A[diag], _, _ = capi.run_host(bits, "add", A[diag], d)
+            return A, time.perf_counter() - t
+
+        A, _ = normal()
+        L, info = capi.cholesky(bits, A, n, block=32)
+        capi.solve(bits, 3, L, n, B, 4, block=32)
+        rows = []
+        for _ in range(5):
+            t = time.perf_counter()
+            A, ts = normal()
+            tc = time.perf_counter()
+            L, info = capi.cholesky(bits, A, n, block=32)
+            tf = time.perf_counter() - tc
+            if int(info.pivot) != n:
+                raise RuntimeError(f"normal matrix n={n} is not positive definite (pivot {int(info.pivot)})")
+            X, _, sw_ = capi.solve(bits, 3, L, n, B, 4, block=32)
+            rows.append((ts, tf, sw_, time.perf_counter() - t))
+        ref = REFERENCE_LINALG[f"cholesky_{n}"]
+        res = {"syrk_wall_ms": round(med([r[0] for r in rows]) * 1e3, 3), "factor_wall_ms": round(med([r[1] for r in rows]) * 1e3, 3),
+               "solve_wall_ms": round(med([r[2] for r in rows]) * 1e3, 3), "e2e_wall_ms": round(med([r[3] for r in rows]) * 1e3, 3)}
+        res.update({f"reference_{k}": v for k, v in ref.items()})
+        res["speedup_factor"] = round(ref["factor_wall_ms"] / res["factor_wall_ms"], 2)
+        res["speedup_e2e"] = round(ref["e2e_wall_ms"] / res["e2e_wall_ms"], 2)
+        out[f"cholesky_{n}"] = res
+        log(f"[bench] reference workload Cholesky n={n} + 4-RHS solve: {res}")
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -616,6 +727,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cholesky", action="store_true")
     ap.add_argument("--no-elementwise", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-reference-linalg", action="store_true")
     ap.add_argument("--elementwise-reps", type=int, default=5)
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -713,6 +825,11 @@ def main(argv=None) -> int:
         log("[bench] 256-bit SYRK + Cholesky n=512 + solve (64 right-hand sides)")
         chol = cholesky_section(not a.no_parity)
 
+    ref_linalg = None
+    if not a.no_reference_linalg:
+        log("[bench] the reference's linalg workloads: 256-bit QSC-like SYRK / GEMM n=1000, normal-equation Cholesky n=1000 and 400")
+        ref_linalg = reference_linalg_section()
+
     base = None
     if not a.no_cpu_baseline:
         log("[bench] timing the CPU baseline")
@@ -741,6 +858,7 @@ def main(argv=None) -> int:
         "cholesky": chol,
         "elementwise": ew,
         "mixed": mixed,
+        "reference_linalg": ref_linalg,
     }
     print(json.dumps(line), flush=True)
     return 0

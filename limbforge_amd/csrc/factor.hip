@@ -67,6 +67,7 @@ typedef hipError_t (*panel_fn)(const lf::FactorArgs&, hipStream_t);
 typedef hipError_t (*lookahead_fn)(const lf::FactorArgs&, uint32_t, uint32_t, hipStream_t);
 typedef hipError_t (*invalid_fn)(uint32_t*, uint64_t, uint32_t, uint32_t, hipStream_t);
 typedef hipError_t (*solve_fn)(int, const lf::SolveArgs&, hipStream_t);
+typedef hipError_t (*solve_update_fn)(int, const lf::SolveArgs&, uint32_t, hipStream_t);
 typedef hipError_t (*recip_fn)(const uint32_t*, uint64_t, uint32_t, uint32_t*, uint64_t, hipStream_t);
 struct FactorTables;
 template <int N> void fill_factor(FactorTables& t);
@@ -77,6 +78,7 @@ struct FactorTables {
     invalid_fn invalid[33] = {};
     hipError_t (*zero_upper[33])(uint32_t*, uint64_t, uint32_t, hipStream_t) = {};
     solve_fn solve[33] = {};
+    solve_update_fn solve_update[33] = {};
     recip_fn reciprocals[33] = {};
     FactorTables() { fill_factor<2>(*this); }
 };
@@ -88,6 +90,7 @@ template <int N> void fill_factor(FactorTables& t)
     t.invalid[N] = lf::launch_chol_invalid<N>;
     t.zero_upper[N] = lf::launch_zero_upper<N>;
     t.solve[N] = lf::launch_solve<N>;
+    t.solve_update[N] = lf::launch_solve_update<N>;
     t.reciprocals[N] = lf::launch_reciprocals<N>;
     if constexpr (N < 32) fill_factor<N + 1>(t);
 }
@@ -168,6 +171,9 @@ int update(int bits, const lf_linalg_options& o, bool syrk, const lf::MatView& l
 }
 
 double since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+// wavefronts (and scratch slots) of a direct off-block solve update
+constexpr uint64_t SOLVE_UPDATE_WAVES = 2048;
 
 } // namespace
 
@@ -371,9 +377,12 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
     if (int r = copy_planes(N, *B, *X, n * nrhs, s)) return r;
     DevBuf dscratch, derr;
     const uint64_t swords = lf::factor_scratch_words(N, nb);
-    if (2 * swords * nrhs * 4 > (uint64_t(32) << 30))
+    // scratch slots: one per right-hand side for the in-block kernels, and the wavefronts of the direct
+    // off-block updates
+    const uint64_t slots = std::max<uint64_t>(nrhs, SOLVE_UPDATE_WAVES);
+    if (2 * swords * slots * 4 > (uint64_t(32) << 30))
         return fail(LF_INVALID_ARGUMENT, "solve: the block is too large for the exact-dot scratch (use a smaller block)");
-    FACTOR_HIP(dscratch.alloc(2 * swords * nrhs * 4), "hipMalloc (scratch)");
+    FACTOR_HIP(dscratch.alloc(2 * swords * slots * 4), "hipMalloc (scratch)");
     FACTOR_HIP(derr.alloc(4), "hipMalloc");
     FACTOR_HIP(hipMemsetAsync(derr.p, 0, 4, s), "hipMemsetAsync");
     // reciprocals of the n diagonal entries, for every division of both passes
@@ -382,61 +391,76 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
     FACTOR_HIP(drecip.alloc((N + 1) * rpitch * 4), "hipMalloc (reciprocals)");
     FACTOR_HIP(tables().reciprocals[N](F, fpitch, uint32_t(n), (uint32_t*)drecip.p, rpitch, s), "reciprocals launch");
     double gpu = 0;
-    hipEvent_t e0, e1;
-    FACTOR_HIP(hipEventCreate(&e0), "hipEventCreate");
-    FACTOR_HIP(hipEventCreate(&e1), "hipEventCreate");
-    struct EventGuard {
-        hipEvent_t a, b;
-        ~EventGuard()
+    // device time: an event pair around every in-block kernel and direct update, read after the last
+    // synchronization (the update GEMM reports its own kernels' time)
+    struct Events {
+        std::vector<hipEvent_t> e;
+        ~Events()
         {
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
+            for (auto x : e) (void)hipEventDestroy(x);
         }
-    } guard{e0, e1};
-    // the in-block kernel between two events; its time is collected after the stream's next synchronization
+    } events;
+    auto timed = [&](auto&& launch) -> hipError_t {
+        hipEvent_t a = nullptr, b = nullptr;
+        hipError_t e = hipEventCreate(&a);
+        if (e != hipSuccess) return e;
+        events.e.push_back(a);
+        if ((e = hipEventCreate(&b)) != hipSuccess) return e;
+        events.e.push_back(b);
+        if ((e = hipEventRecord(a, s)) != hipSuccess) return e;
+        if ((e = launch()) != hipSuccess) return e;
+        return hipEventRecord(b, s);
+    };
     auto block_solve = [&](int backward, uint64_t r0, uint64_t r1) {
         lf::SolveArgs a{F, fpitch, Xp, xpitch, uint32_t(n), uint32_t(nrhs), uint32_t(r0), uint32_t(r1), (uint32_t*)dscratch.p, swords,
                         (int32_t*)derr.p, (const uint32_t*)drecip.p, rpitch};
-        hipError_t e = hipEventRecord(e0, s);
-        if (e == hipSuccess) e = tables().solve[N](backward, a, s);
-        if (e == hipSuccess) e = hipEventRecord(e1, s);
-        return e;
-    };
-    auto collect = [&]() {
-        float t = 0;
-        if (hipEventElapsedTime(&t, e0, e1) == hipSuccess) gpu += t * 1e-3;
+        return timed([&] { return tables().solve[N](backward, a, s); });
     };
     float ms = 0;
+    // An off-block update of fewer multiply-adds than solve_host_macs (the reference's threshold for taking
+    // such an update off the GPU, linalg.hpp:73-78) runs as direct exact dots on the stream, without the
+    // update GEMM's planning and its synchronisation; larger ones are products of the update GEMM.
+    auto direct = [&](uint64_t rows, uint64_t K) { return double(rows) * double(nrhs) * double(K) < fo.solve_host_macs; };
+    auto dot_update = [&](int backward, uint64_t r0, uint64_t r1) {
+        lf::SolveArgs a{F, fpitch, Xp, xpitch, uint32_t(n), uint32_t(nrhs), uint32_t(r0), uint32_t(r1), (uint32_t*)dscratch.p, swords,
+                        (int32_t*)derr.p, (const uint32_t*)drecip.p, rpitch};
+        return timed([&] { return tables().solve_update[N](backward, a, uint32_t(slots), s); });
+    };
     if (passes & 1)
         for (uint64_t r0 = 0; r0 < n; r0 += nb) {
             const uint64_t r1 = std::min(n, r0 + nb), K = r1 - r0;
             FACTOR_HIP(block_solve(0, r0, r1), "solve launch");
             if (r1 == n) break;
             // rows [r1, n) of X -= L[r1:, r0:r1] X[r0:r1, :] (the update waits for the block on the stream)
+            if (direct(n - r1, K)) {
+                FACTOR_HIP(dot_update(0, r0, r1), "solve update launch");
+                continue;
+            }
             const lf::MatView left{F + (r1 * n + r0), fpitch, n, 1}, right{Xp + r0 * nrhs, xpitch, 1, nrhs};
             if (int r = update(bits, lo, false, left, right, n - r1, nrhs, K, Xp + r1 * nrhs, xpitch, nrhs, s, ms)) return r;
-            collect();
             gpu += ms * 1e-3;
         }
-    if (passes == 3) { // the forward pass's last block
-        FACTOR_HIP(hipEventSynchronize(e1), "device execution (solve)");
-        collect();
-    }
     if (passes & 2)
         for (uint64_t r0 = (n - 1) / nb * nb;; r0 -= nb) {
             const uint64_t r1 = std::min(n, r0 + nb), K = r1 - r0;
             FACTOR_HIP(block_solve(1, r0, r1), "solve launch");
             if (!r0) break;
             // rows [0, r0) of X -= L[r0:r1, :r0]^T X[r0:r1, :]
+            if (direct(r0, K)) {
+                FACTOR_HIP(dot_update(1, r0, r1), "solve update launch");
+                continue;
+            }
             const lf::MatView left{F + r0 * n, fpitch, 1, n}, right{Xp + r0 * nrhs, xpitch, 1, nrhs};
             if (int r = update(bits, lo, false, left, right, r0, nrhs, K, Xp, xpitch, nrhs, s, ms)) return r;
-            collect();
             gpu += ms * 1e-3;
         }
     int32_t err = 0;
     FACTOR_HIP(hipMemcpyAsync(&err, derr.p, 4, hipMemcpyDeviceToHost, s), "download status");
     FACTOR_HIP(hipStreamSynchronize(s), "device execution (solve)");
-    collect(); // the last block of the last pass
+    for (std::size_t k = 0; k + 1 < events.e.size(); k += 2) {
+        float t = 0;
+        if (hipEventElapsedTime(&t, events.e[k], events.e[k + 1]) == hipSuccess) gpu += t * 1e-3;
+    }
     if (err) return fail(LF_LOGIC_ERROR, "solve: exact-dot scratch overflow");
     if (gpu_seconds) *gpu_seconds = gpu;
     if (wall_seconds) *wall_seconds = since(start);

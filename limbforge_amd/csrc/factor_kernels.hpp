@@ -671,6 +671,26 @@ template <int N> __global__ void __launch_bounds__(64) solve_backward_kernel(Sol
     }
 }
 
+// Off-block update of a solve by the block [r0, r1), for updates too small to pay for the update GEMM's
+// planning: x_ic = D(x_ic; l, x_c; [r0, r1)) with l = row i of L (forward, rows i in [r1, n)) or column i of L
+// (backward, rows i in [0, r0)), one wavefront per entry, grid-stride over the rows x nrhs entries; wavefront
+// b uses scratch slot b.  The same one rounding of the same exact value as the update product.
+template <int N> __global__ void __launch_bounds__(64) solve_update_kernel(SolveArgs a, u32 backward)
+{
+    __shared__ WaveLds<N> L;
+    const u32 lane = threadIdx.x;
+    const u32 row0 = backward ? 0u : a.r1, rows = backward ? a.r0 : a.n - a.r1;
+    const u64 entries = (u64)rows * a.nrhs;
+    const DotScratch sc{a.scratch + (u64)blockIdx.x * 2 * a.swords, a.swords, a.error};
+    for (u64 e = blockIdx.x; e < entries; e += gridDim.x) {
+        const u32 i = row0 + (u32)(e / a.nrhs), c = (u32)(e % a.nrhs);
+        const VecView l = backward ? VecView{a.f, a.fpitch, (i64)a.r0 * a.n + i, (i64)a.n} : VecView{a.f, a.fpitch, (i64)i * a.n + a.r0, 1};
+        const VecView xc{a.x, a.xpitch, (i64)a.r0 * a.nrhs + c, (i64)a.nrhs};
+        const Num<N> v = wave_dot_sub<N>(load_out<N>(a.x, a.xpitch, (u64)i * a.nrhs + c), l, xc, a.r1 - a.r0, lane, L, sc);
+        if (lane == 0) store_out<N>(a.x, a.xpitch, a.nrhs, i, c, false, v);
+    }
+}
+
 // Reciprocals of the n diagonal entries of the factor, one thread each (for the solves' divisions).
 template <int N> __global__ void __launch_bounds__(64) reciprocals_kernel(const u32* w, u64 pitch, u32 n, u32* recip, u64 rpitch)
 {

@@ -59,6 +59,9 @@ template <int N> hipError_t launch_chol_lookahead(const FactorArgs& f, uint32_t 
 template <int N> hipError_t launch_chol_invalid(uint32_t* w, uint64_t pitch, uint32_t n, uint32_t p, hipStream_t s);
 template <int N> hipError_t launch_zero_upper(uint32_t* w, uint64_t pitch, uint32_t n, hipStream_t s);
 template <int N> hipError_t launch_solve(int backward, const SolveArgs& a, hipStream_t s);
+// the off-block update of block [r0, r1) by exact dots (rows [r1, n) forward, [0, r0) backward); at most
+// `waves` wavefronts (scratch slots)
+template <int N> hipError_t launch_solve_update(int backward, const SolveArgs& a, uint32_t waves, hipStream_t s);
 // recip entry i = reciprocal of the mantissa of the diagonal element (i, i) of the n x n planes, i < n
 template <int N> hipError_t launch_reciprocals(const uint32_t* w, uint64_t pitch, uint32_t n, uint32_t* recip, uint64_t rpitch, hipStream_t s);
 

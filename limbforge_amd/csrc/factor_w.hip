@@ -58,6 +58,16 @@ template <int N> hipError_t launch_solve(int backward, const SolveArgs& a, hipSt
     return hipGetLastError();
 }
 
+template <int N> hipError_t launch_solve_update(int backward, const SolveArgs& a, uint32_t waves, hipStream_t s)
+{
+    const uint64_t rows = backward ? a.r0 : a.n - a.r1;
+    if (!a.nrhs || !rows || a.r1 <= a.r0 || !waves) return hipSuccess;
+    const uint64_t entries = rows * a.nrhs;
+    hipLaunchKernelGGL((solve_update_kernel<N>), dim3((unsigned)std::min<uint64_t>(entries, waves)), dim3(WAVE), 0, s, a,
+                       backward ? 1u : 0u);
+    return hipGetLastError();
+}
+
 template <int N> hipError_t launch_reciprocals(const uint32_t* w, uint64_t pitch, uint32_t n, uint32_t* recip, uint64_t rpitch, hipStream_t s)
 {
     if (!n) return hipSuccess;
@@ -71,6 +81,7 @@ template hipError_t launch_chol_lookahead<LF_WORDS>(const FactorArgs&, uint32_t,
 template hipError_t launch_chol_invalid<LF_WORDS>(uint32_t*, uint64_t, uint32_t, uint32_t, hipStream_t);
 template hipError_t launch_zero_upper<LF_WORDS>(uint32_t*, uint64_t, uint32_t, hipStream_t);
 template hipError_t launch_solve<LF_WORDS>(int, const SolveArgs&, hipStream_t);
+template hipError_t launch_solve_update<LF_WORDS>(int, const SolveArgs&, uint32_t, hipStream_t);
 template hipError_t launch_reciprocals<LF_WORDS>(const uint32_t*, uint64_t, uint32_t, uint32_t*, uint64_t, hipStream_t);
 
 } // namespace lf
