@@ -633,6 +633,51 @@ def qsc_jacobian(rng, bits: int, rows: int, cols: int) -> np.ndarray:
     return r.reshape(rows * cols, w + 3)
 
 
+# The reference's published element-wise times for 65,536 values (Apple M5 Max; milliseconds: GPU device
+# and wall of Engine::run on host arrays), benchmarks/results/final.csv:40,46,134,136,142 and
+# round12_reduction_policy.csv:52,172.
+REFERENCE_ELEMENTWISE = {
+    ("mul", 1024): (0.482, 1.622), ("div", 1024): (0.787, 2.088), ("complex_div", 1024): (6.04, 7.96),
+    ("div", 256): (0.278, 0.681), ("complex_div", 256): (0.767, 1.705), ("sqrt", 256): (1.50, 2.13), ("sqrt", 1024): (5.47, 6.74),
+}
+
+
+def reference_elementwise_section():
+    """The reference's element-wise benchmark (benchmarks/benchmark.cpp:3-60): 65,536 values with
+    random_number(5) operands (exponents in [-5, 5]), sqrt of |a|, Engine::run on host arrays; device time
+    (the kernel, HIP events) and wall time (host arrays in and out) of lf_run_host, median of 5 after two
+    warm-ups, beside the reference's published M5 Max times."""
+    from limbforge_amd import capi
+
+    count = 65536
+    rng = np.random.default_rng(20261007 + 5)
+    out = {}
+    for (op, bits), (ref_dev, ref_wall) in REFERENCE_ELEMENTWISE.items():
+        n = bits // 32
+        cplx = op.startswith("complex")
+
+        def records(positive=False):
+            r = np.zeros((count, n + 3), dtype=np.uint32)
+            r[:, :n] = rng.integers(0, 1 << 32, size=(count, n), dtype=np.uint64).astype(np.uint32)
+            r[:, n - 1] |= np.uint32(0x80000000)
+            r[:, n] = (rng.integers(-5, 6, size=count) & 0xFFFFFFFF).astype(np.uint32)
+            sg = np.ones(count, np.int64) if positive else np.where(rng.integers(0, 2, size=count) == 1, 1, -1)
+            r[:, n + 1] = (sg & 0xFFFFFFFF).astype(np.uint32)
+            return r
+
+        a = np.concatenate([records(), records()], axis=1) if cplx else records(positive=op == "sqrt")
+        b = None if op == "sqrt" else (np.concatenate([records(), records()], axis=1) if cplx else records())
+        runs = [capi.run_host(bits, op, a, b) for _ in range(7)][2:]
+        dev = sorted(r[1] for r in runs)[2] * 1e3
+        wall = sorted(r[2] for r in runs)[2] * 1e3
+        row = {"device_ms": round(dev, 4), "wall_ms": round(wall, 4), "reference_device_ms": ref_dev, "reference_wall_ms": ref_wall,
+               "speedup_device": round(ref_dev / dev, 1), "speedup_wall": round(ref_wall / wall, 2)}
+        out[f"{op}_{bits}"] = row
+        log(f"[bench] reference element-wise {op} {bits}: {row}")
+    return {"workload": "65,536 values, random_number(5) operands, host arrays (benchmarks/benchmark.cpp:3-60)",
+            "reference_gpu": "Apple M5 Max", "ops": out}
+
+
 def reference_linalg_section():
     """The reference's own linalg benchmark workloads at 256 bits, timed as it times them (median of 5 after
     a warm-up): SYRK C = J^T J (lower) and GEMM C = A^T B of QSC-like Jacobians with n = 1000, K = 2000, and the
@@ -825,6 +870,11 @@ def main(argv=None) -> int:
         log("[bench] 256-bit SYRK + Cholesky n=512 + solve (64 right-hand sides)")
         chol = cholesky_section(not a.no_parity)
 
+    ref_ew = None
+    if not a.no_reference_linalg:
+        log("[bench] the reference's element-wise workloads: 65,536 values, host arrays")
+        ref_ew = reference_elementwise_section()
+
     ref_linalg = None
     if not a.no_reference_linalg:
         log("[bench] the reference's linalg workloads: 256-bit QSC-like SYRK / GEMM n=1000, normal-equation Cholesky n=1000 and 400")
@@ -859,6 +909,7 @@ def main(argv=None) -> int:
         "elementwise": ew,
         "mixed": mixed,
         "reference_linalg": ref_linalg,
+        "reference_elementwise": ref_ew,
     }
     print(json.dumps(line), flush=True)
     return 0
