@@ -678,6 +678,25 @@ def reference_elementwise_section():
             "reference_gpu": "Apple M5 Max", "ops": out}
 
 
+def small_batch_chain_section():
+    """Multiplication chains on batches too small to fill the GPU with one lane per value (1,024 to 16,384
+    values, 64 steps, 512 and 1024 bits) at 1, 2 and 4 lanes per value, timed by the library's tuner
+    (lf_autotune_chain: HIP events, best of three after a warm-up); the 65,536-value row is the headline
+    batch size, where one lane per value is fastest.  The cooperative forms are checked bit for bit against
+    one lane by the GPU tests."""
+    from limbforge_amd import capi
+
+    out = {}
+    for bits in (512, 1024):
+        for count in (1024, 4096, 16384, 65536):
+            lanes, ms = capi.autotune_chain(bits, count, 64)
+            row = {"ms_lanes_1": round(ms[0], 4), "ms_lanes_2": round(ms[1], 4), "ms_lanes_4": round(ms[2], 4), "tuned_lanes": lanes,
+                   "speedup_vs_one_lane": round(ms[0] / min(ms), 2)}
+            out[f"{bits}_{count}"] = row
+            log(f"[bench] small-batch chain {bits} bits x {count}: {row}")
+    return {"workload": "mul chain, 64 steps, random operands (exponents in [-64, 64])", "rows": out}
+
+
 def reference_linalg_section():
     """The reference's own linalg benchmark workloads at 256 bits, timed as it times them (median of 5 after
     a warm-up): SYRK C = J^T J (lower) and GEMM C = A^T B of QSC-like Jacobians with n = 1000, K = 2000, and the
@@ -870,6 +889,11 @@ def main(argv=None) -> int:
         log("[bench] 256-bit SYRK + Cholesky n=512 + solve (64 right-hand sides)")
         chol = cholesky_section(not a.no_parity)
 
+    small = None
+    if not a.no_mixed:
+        log("[bench] small-batch chains at 1, 2 and 4 lanes per value")
+        small = small_batch_chain_section()
+
     ref_ew = None
     if not a.no_reference_linalg:
         log("[bench] the reference's element-wise workloads: 65,536 values, host arrays")
@@ -908,6 +932,7 @@ def main(argv=None) -> int:
         "cholesky": chol,
         "elementwise": ew,
         "mixed": mixed,
+        "small_batch_chains": small,
         "reference_linalg": ref_linalg,
         "reference_elementwise": ref_ew,
     }

@@ -112,6 +112,19 @@ int lf_elementwise(int bits, int op, const lf_operand* a, const lf_operand* b, c
  * as benchmarks/benchmark.cpp:84-92 issues them).  out may alias x. */
 int lf_mul_chain(int bits, const lf_operand* x, const lf_operand* f, const lf_operand* out, uint64_t count,
                  const lf_broadcast* f_index, int steps, void* stream);
+/* The same chain with `lanes` lanes of a wavefront per value (no reference counterpart; the reference
+ * spreads values over lanes only in its recurrence kernels, src/cooperative.metal:40-61): 1, or 2 and 4 from
+ * 256 bits on (the lanes split each product's digit columns, exchange them inside a quad and round
+ * together), bit for bit the one-lane result; 0 takes the choice lf_autotune_chain made for this width
+ * and batch size, else 4 lanes from 512 bits for batches of at most 16384 values and one lane otherwise.
+ * lf_mul_chain is lanes 0. */
+int lf_mul_chain_lanes(int bits, const lf_operand* x, const lf_operand* f, const lf_operand* out, uint64_t count,
+                       const lf_broadcast* f_index, int steps, unsigned lanes, void* stream);
+/* Times the chain at 1, 2 and 4 lanes per value on a synthetic batch of `count` values and `steps` steps
+ * (count <= 2^22, steps <= 1024), records the fastest for this device, width and power-of-two class of the
+ * batch size, and returns it in *lanes; ms (may be NULL) receives the three times in milliseconds (-1 where
+ * the width has no such form).  No reference counterpart (the project's autotuner). */
+int lf_autotune_chain(int bits, uint64_t count, int steps, unsigned* lanes, float* ms);
 
 /* Host-array element-wise call: Engine::run / run_unary / run_ternary (engine.hpp:202-204;
  * engine.mm:254-280, 479-499).  a, b, c, out are host arrays of AoS elements; b holds

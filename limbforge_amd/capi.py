@@ -82,6 +82,8 @@ def lib() -> C.CDLL:
         "lf_planes_to_aos": (i32, [i32, i32, P(Operand), vp, u64, vp]),
         "lf_elementwise": (i32, [i32, i32, P(Operand), P(Operand), P(Operand), P(Operand), u64, P(Broadcast), P(Broadcast), vp]),
         "lf_mul_chain": (i32, [i32, P(Operand), P(Operand), P(Operand), u64, P(Broadcast), i32, vp]),
+        "lf_mul_chain_lanes": (i32, [i32, P(Operand), P(Operand), P(Operand), u64, P(Broadcast), i32, C.c_uint, vp]),
+        "lf_autotune_chain": (i32, [i32, u64, i32, P(C.c_uint), P(C.c_float)]),
         "lf_calibrate_mad64": (i32, [u64, i32, P(C.c_double)]),
         "lf_calibrate_copy": (i32, [u64, P(C.c_double)]),
         "lf_cast": (i32, [i32, i32, i32, P(Operand), P(Operand), u64, vp]),
@@ -216,9 +218,20 @@ def elementwise(bits: int, op, a: Buffer, b: Buffer | None, c: Buffer | None, ou
                                _bcast(b_index), _bcast(c_index), stream))
 
 
-def mul_chain(bits: int, x: Buffer, f: Buffer, out: Buffer, steps: int, f_index=None, stream=None) -> None:
+def mul_chain(bits: int, x: Buffer, f: Buffer, out: Buffer, steps: int, f_index=None, stream=None, lanes: int = 0) -> None:
+    """x <- RN(x * f) `steps` times in one launch; lanes per value 1, 2 or 4, 0 = the tuned choice."""
     ox, of, oo = x.operand, f.operand, out.operand
-    check(lib().lf_mul_chain(bits, C.byref(ox), C.byref(of), C.byref(oo), x.count, _bcast(f_index), int(steps), stream))
+    check(lib().lf_mul_chain_lanes(bits, C.byref(ox), C.byref(of), C.byref(oo), x.count, _bcast(f_index), int(steps),
+                                   int(lanes), stream))
+
+
+def autotune_chain(bits: int, count: int, steps: int = 64):
+    """Times the chain at 1, 2 and 4 lanes per value, records the fastest for this width and batch-size
+    class, and returns (lanes, [ms at 1, 2, 4 lanes] with -1 where the width has no such form)."""
+    lanes = C.c_uint(0)
+    ms = (C.c_float * 3)()
+    check(lib().lf_autotune_chain(bits, int(count), int(steps), C.byref(lanes), ms))
+    return lanes.value, list(ms)
 
 
 def broadcast_elements(count: int, index) -> int:

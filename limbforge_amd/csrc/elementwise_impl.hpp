@@ -4,6 +4,7 @@
 // operands are loaded plane by plane (coalesced), computed in VGPRs and stored plane by plane.
 #pragma once
 #include "lf_internal.hpp"
+#include "coop_core.hpp"
 #include "dot2_core.hpp"
 #include "limb_core.hpp"
 
@@ -107,6 +108,24 @@ template <int N> __global__ void __launch_bounds__(256) mul_chain_kernel(ChainAr
     store_num<N>(g.out, g.po, i, x);
 }
 
+// The chain with G lanes per value (coop_core.hpp): lanes G i .. G i + G - 1 hold value i, split each
+// product's digit columns and round together; lane G i stores the result.  Bit for bit the single-lane
+// chain; it shortens each step (about 1/G of the multiply-adds per lane) for batches too small to fill
+// the GPU with one lane per value.
+template <int N, int G> __global__ void __launch_bounds__(256) mul_chain_coop_kernel(ChainArgs g)
+{
+    const u64 t = (u64)blockIdx.x * 256u + threadIdx.x;
+    const u32 i = (u32)(t / G), k = (u32)(t % G);
+    if (i >= g.count) return; // whole groups leave together: G divides the wavefront
+    Num<N> x = load_num<N>(g.x, g.px, i);
+    const Num<N> f = load_num<N>(g.f, g.pf, operand_index(i, g.f_stride, g.f_period));
+    u32 fd[digit_count<N>()], fs[coop_factor_digits<N, G>()];
+    to_digits<N>(f.w, fd);
+    coop_shift_factor<N, G>(fd, k, fs);
+    for (int s = 0; s < g.steps; ++s) x = coop_mul_by_digits<N, G>(x, f, fd, fs);
+    if (k == 0) store_num<N>(g.out, g.po, i, x);
+}
+
 template <int N, int OP> inline hipError_t launch_op(const ElemArgs& g, hipStream_t s)
 {
     const dim3 grid(grid_blocks(g.count, 256)), block(256);
@@ -139,6 +158,21 @@ template <int N> hipError_t launch_mul_chain(const ChainArgs& g, hipStream_t s)
     const dim3 grid(grid_blocks(g.count, 256)), block(256);
     hipLaunchKernelGGL((mul_chain_kernel<N>), grid, block, 0, s, g);
     return hipGetLastError();
+}
+
+// The chain with `lanes` lanes per value: 1, or 2 and 4 from 8 limbs on (the digit products).
+template <int N> hipError_t launch_mul_chain_lanes(const ChainArgs& g, unsigned lanes, hipStream_t s)
+{
+    if (lanes == 1) return launch_mul_chain<N>(g, s);
+    if constexpr (use_digits<N>()) {
+        if (!g.count) return hipSuccess;
+        const dim3 grid(grid_blocks((u64)g.count * lanes, 256)), block(256);
+        if (lanes == 2) hipLaunchKernelGGL((mul_chain_coop_kernel<N, 2>), grid, block, 0, s, g);
+        else if (lanes == 4) hipLaunchKernelGGL((mul_chain_coop_kernel<N, 4>), grid, block, 0, s, g);
+        else return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
 }
 
 } // namespace lf

@@ -9,4 +9,5 @@
 namespace lf {
 template hipError_t launch_elementwise<LF_WORDS>(int op, const ElemArgs& g, hipStream_t s);
 template hipError_t launch_mul_chain<LF_WORDS>(const ChainArgs& g, hipStream_t s);
+template hipError_t launch_mul_chain_lanes<LF_WORDS>(const ChainArgs& g, unsigned lanes, hipStream_t s);
 } // namespace lf

@@ -52,6 +52,8 @@ struct MixedArgs {
 template <int N> hipError_t launch_elementwise(int op, const ElemArgs& g, hipStream_t s);
 // x <- RN(x * f) `steps` times with x held in registers (one launch for the whole chain).
 template <int N> hipError_t launch_mul_chain(const ChainArgs& g, hipStream_t s);
+// the same chain with `lanes` (1, 2 or 4; 2 and 4 from 8 limbs on) lanes per value, bit for bit
+template <int N> hipError_t launch_mul_chain_lanes(const ChainArgs& g, unsigned lanes, hipStream_t s);
 // The mixed chains of one bucket with `block` lanes per workgroup (64, 128 or 256).
 template <int N> hipError_t launch_mixed_chain(const MixedArgs& g, unsigned block, hipStream_t s);
 // The kernel entry of launch_mixed_chain, for graph kernel nodes (arguments: one MixedArgs).

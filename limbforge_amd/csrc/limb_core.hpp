@@ -428,27 +428,13 @@ template <int N> constexpr int short_cut()
 // product P' = a * b - delta, 0 <= delta < 2^(32N-32), from the columns from short_cut<N>() up.  With
 // HIGH, only what a rounding to N words reads is formed: words N-1 .. 2N-1 exactly, word 0 nonzero
 // exactly when a bit below word N-1 is set, words 1 .. N-2 zero (no repacking of the low half).
-template <int N, bool HIGH = false>
-__device__ __forceinline__ void digit_columns(const u32 (&da)[digit_count<N>()], const u32 (&db)[digit_count<N>()], u32 (&p)[2 * N],
-                                              bool full)
+// Carry pass and repacking of digit column sums: col[k] (k < 2D) holds column k of a product of 29-bit
+// digits, columns below short_cut<N>() only when full (zero otherwise); p as digit_columns describes.
+template <int N, bool HIGH = false> __device__ __forceinline__ void pack_columns(u64 (&col)[2 * digit_count<N>()], u32 (&p)[2 * N], bool full)
 {
     constexpr int D = digit_count<N>(), C0 = short_cut<N>();
-    static_assert(D <= 63, "column sums of 29-bit digit products must fit in 64 bits");
-    u64 col[2 * D];
-#pragma unroll
-    for (int k = 0; k < 2 * D; ++k) col[k] = 0;
-#pragma unroll
-    for (int i = 0; i < D; ++i)
-#pragma unroll
-        for (int j = 0; j < D; ++j)
-            if (i + j >= C0) col[i + j] += (u64)da[i] * db[j];
     u32 dg[2 * D];
     if (full) {
-#pragma unroll
-        for (int i = 0; i < D; ++i)
-#pragma unroll
-            for (int j = 0; j < D; ++j)
-                if (i + j < C0) col[i + j] += (u64)da[i] * db[j];
 #pragma unroll
         for (int k = 0; k < C0; ++k) {
             dg[k] = (u32)col[k] & DIGIT_MASK;
@@ -482,6 +468,34 @@ __device__ __forceinline__ void digit_columns(const u32 (&da)[digit_count<N>()],
         if (k + 2 < 2 * D && 2 * DIGIT_BITS - s < 32) v |= dg[k + 2] << (2 * DIGIT_BITS - s);
         p[w] = v;
     }
+}
+
+// p (2N words) = a * b from the digits of a and b (each < 2^(32N)) when full; otherwise the short
+// product P' = a * b - delta, 0 <= delta < 2^(32N-32), from the columns from short_cut<N>() up.  With
+// HIGH, only what a rounding to N words reads is formed: words N-1 .. 2N-1 exactly, word 0 nonzero
+// exactly when a bit below word N-1 is set, words 1 .. N-2 zero (no repacking of the low half).
+template <int N, bool HIGH = false>
+__device__ __forceinline__ void digit_columns(const u32 (&da)[digit_count<N>()], const u32 (&db)[digit_count<N>()], u32 (&p)[2 * N],
+                                              bool full)
+{
+    constexpr int D = digit_count<N>(), C0 = short_cut<N>();
+    static_assert(D <= 63, "column sums of 29-bit digit products must fit in 64 bits");
+    u64 col[2 * D];
+#pragma unroll
+    for (int k = 0; k < 2 * D; ++k) col[k] = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+            if (i + j >= C0) col[i + j] += (u64)da[i] * db[j];
+    if (full) {
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+                if (i + j < C0) col[i + j] += (u64)da[i] * db[j];
+    }
+    pack_columns<N, HIGH>(col, p, full);
 }
 
 // p (2N words) = a * b from the digits of a and b (each < 2^(32N)).

@@ -4,6 +4,7 @@
 #include "dot2_core.hpp"
 #include "limb_core.hpp"
 #include "residue_reduce.hpp"
+#include "coop_core.hpp"
 #include "lf_oracle.h"
 #include "lfo_cases.h"
 #include <cstdio>
@@ -284,11 +285,68 @@ static void check_residue_combine()
     }
 }
 
+// Cooperative digit columns (coop_core.hpp): for every lane k of a group of G, the lane's share of the
+// columns from its shifted factor digits, assembled over the group, equals the product's columns; packed,
+// they give digit_columns' words (short and exact forms), on random mantissas of every digit width.
+template <int N, int G, bool FULL> static void check_coop_once(uint64_t& state)
+{
+    constexpr int D = lf::digit_count<N>(), LO = lf::coop_lo<N, G, FULL>(), M = lf::coop_cols<N, G, FULL>();
+    auto next = [&] {
+        state ^= state << 13;
+        state ^= state >> 7;
+        state ^= state << 17;
+        return state;
+    };
+    uint32_t a[N], b[N];
+    for (int i = 0; i < N; ++i) {
+        a[i] = (uint32_t)next();
+        b[i] = (uint32_t)next();
+    }
+    a[N - 1] |= 0x80000000u;
+    b[N - 1] |= 0x80000000u;
+    if (next() % 4 == 0) // all-ones and sparse mantissas exercise the carries
+        for (int i = 0; i < N - 1; ++i) a[i] = (next() & 1) ? 0xffffffffu : 0u;
+    uint32_t ad[D], bd[D];
+    lf::to_digits<N>(a, ad);
+    lf::to_digits<N>(b, bd);
+    uint64_t col[2 * D] = {};
+    for (uint32_t k = 0; k < (uint32_t)G; ++k) {
+        uint32_t bs[lf::coop_factor_digits<N, G>()];
+        lf::coop_shift_factor<N, G>(bd, k, bs);
+        uint64_t S[M];
+        lf::coop_partial<N, G, FULL>(ad, bs, S);
+        for (int m = 0; m < M; ++m)
+            if (LO + G * m + (int)k < 2 * D - 1) col[LO + G * m + k] = S[m];
+    }
+    uint32_t got[2 * N], want[2 * N];
+    lf::pack_columns<N, true>(col, got, FULL);
+    lf::digit_columns<N, true>(ad, bd, want, FULL);
+    ++checks;
+    bool same = true;
+    for (int w = 0; w < 2 * N; ++w) same = same && got[w] == want[w];
+    if (!same && ++failures <= 30) std::printf("MISMATCH coop columns N=%d G=%d full=%d\n", N, G, FULL ? 1 : 0);
+}
+
+template <int N> static void check_coop(uint64_t& state, int cases)
+{
+    if constexpr (lf::use_digits<N>()) {
+        for (int c = 0; c < cases; ++c) {
+            check_coop_once<N, 2, false>(state);
+            check_coop_once<N, 4, false>(state);
+            check_coop_once<N, 2, true>(state);
+            check_coop_once<N, 4, true>(state);
+        }
+    }
+    if constexpr (N < 32) check_coop<N + 1>(state, cases);
+}
+
 int main(int argc, char** argv)
 {
     int lo = argc > 1 ? std::atoi(argv[1]) : 2, hi = argc > 2 ? std::atoi(argv[2]) : 32, cases = argc > 3 ? std::atoi(argv[3]) : 300;
     run_all<2>(lo, hi, cases);
     check_residue_combine();
+    uint64_t coop_state = 0x2545F4914F6CDD1Dull;
+    check_coop<8>(coop_state, 200);
     std::printf("device core (host build): %ld checks, %ld failures\n", checks, failures);
     return failures ? 1 : 0;
 }
