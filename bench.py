@@ -365,7 +365,8 @@ def mixed_section(peak_tps: float, oracle_check: bool, cpu: bool, reps: int = 3)
     """10^6 values with precisions 64 + 32 k bits, k in [0, 30] drawn with P(k) ~ 0.85^k, chains of uniform
     length 1-64 whose steps are fma or mul by a seeded mask, operands as the chain's.  Bucketed by precision
     (longest chains first inside a bucket); timed as one launch per precision on a stream (workgroup 256,
-    then the autotuned sizes) and as one hipGraph of concurrent bucket nodes with the autotuned sizes."""
+    then the autotuned sizes) and as one hipGraph of concurrent bucket nodes with the autotuned sizes and
+    lanes per value."""
     from limbforge_amd import capi
 
     count, p = 10 ** 6, 0.15
@@ -413,7 +414,9 @@ def mixed_section(peak_tps: float, oracle_check: bool, cpu: bool, reps: int = 3)
 
     plain_ms = timed(lambda: capi.mixed_chains(buckets, block=[256] * len(buckets), stream=stream.handle))
     tuned_ms = timed(lambda: capi.mixed_chains(buckets, block=[tuned[b.bits] for b in buckets], stream=stream.handle))
-    graph = capi.mixed_graph(buckets, block=[tuned[b.bits] for b in buckets])
+    # lanes per value of the two widest groups, tuned on this batch's graph (1 lane keeps the narrower groups)
+    lanes, _ = capi.autotune_mixed_lanes(buckets, block=[tuned[b.bits] for b in buckets])
+    graph = capi.mixed_graph(buckets, block=[tuned[b.bits] for b in buckets], lanes=lanes)
     try:
         graph_ms = timed(lambda: capi.graph_launch(graph, stream.handle))
     finally:
@@ -429,6 +432,7 @@ def mixed_section(peak_tps: float, oracle_check: bool, cpu: bool, reps: int = 3)
         "speedup_graph_vs_per_precision": round(plain_ms / graph_ms, 3),
         "valu_frac_graph": round(work / (graph_ms * 1e-3) / peak_tps, 4),
         "autotuned_workgroup": {str(k): v for k, v in sorted(tuned.items())},
+        "autotuned_lanes_per_group": lanes,
     }
     res["roofline"] = {"bound": "valu", "achieved": round(work / (graph_ms * 1e-3) / 1e12, 3), "peak": round(peak_tps / 1e12, 3),
                        "unit": "Tword-products/s", "frac": round(work / (graph_ms * 1e-3) / peak_tps, 4),

@@ -299,8 +299,21 @@ typedef struct lf_chain_bucket {
 int lf_mixed_chains(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, void* stream);
 /* One hipGraph with one persistent kernel node per group of eight widths (2-8, 9-16, 17-24, 25-32 limbs) and
  * no edges between them; each node runs all of its group's buckets (launch with lf_graph_launch, free with
- * lf_graph_destroy).  The graph keeps the buckets' device pointers. */
+ * lf_graph_destroy).  The graph keeps the buckets' device pointers.  One lane per value unless
+ * lf_autotune_mixed_lanes has run on this device. */
 int lf_mixed_graph_create(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, void** graph_exec);
+/* The same graph with group_lanes[k] lanes per value in group k (0 or 1 for the 2-8 and 9-16-limb groups;
+ * 0, 1, 2 or 4 for 17-24 and 25-32 limbs: the lanes split each step's product columns, as
+ * lf_mul_chain_lanes does; 0 = 1).
+ * group_lanes NULL takes the choice of lf_autotune_mixed_lanes on this device, else one lane everywhere
+ * (what lf_mixed_graph_create does).  Results are the same bit for bit whatever the lanes. */
+int lf_mixed_graph_create_lanes(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, const unsigned* group_lanes,
+                                void** graph_exec);
+/* Lanes tuner: times the batch's graph (writing the buckets' out planes, with the results the batch
+ * computes) from one lane everywhere, trying 1, 2 and 4 lanes per value in the 25-32-limb group, then in
+ * the 17-24-limb group with the first choice held; keeps the fastest for this device and returns it in group_lanes[4]
+ * and its time in *best_ms (may be NULL). */
+int lf_autotune_mixed_lanes(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, unsigned* group_lanes, float* best_ms);
 /* Launch-shape autotuner: times the bucket kernel of `bits` for each workgroup size on `sample` synthetic
  * values (32-step chains, half fma) and keeps the fastest for this device and width. */
 int lf_autotune_mixed(int bits, uint64_t sample, unsigned* block, double* best_ms);

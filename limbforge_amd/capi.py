@@ -563,6 +563,8 @@ def _mixed_sigs(L):
     L.lf_mixed_chains.argtypes = [P(ChainBucket), C.c_int, P(C.c_uint), C.c_void_p]
     L.lf_mixed_graph_create.argtypes = [P(ChainBucket), C.c_int, P(C.c_uint), P(C.c_void_p)]
     L.lf_autotune_mixed.argtypes = [C.c_int, C.c_uint64, P(C.c_uint), P(C.c_double)]
+    L.lf_mixed_graph_create_lanes.argtypes = [P(ChainBucket), C.c_int, P(C.c_uint), P(C.c_uint), P(C.c_void_p)]
+    L.lf_autotune_mixed_lanes.argtypes = [P(ChainBucket), C.c_int, P(C.c_uint), P(C.c_uint), P(C.c_float)]
     L._mixed = True
 
 
@@ -600,14 +602,28 @@ def mixed_chains(buckets, block=None, stream=None) -> None:
     check(L.lf_mixed_chains(arr, len(buckets), blk, stream))
 
 
-def mixed_graph(buckets, block=None) -> int:
-    """A graph with one kernel node per bucket (lf_mixed_graph_create); launch with graph_launch."""
+def mixed_graph(buckets, block=None, lanes=None) -> int:
+    """A graph with one kernel node per group of widths (lf_mixed_graph_create_lanes); launch with
+    graph_launch.  lanes: lanes per value of the four groups (None = the tuned choice, else one lane)."""
     L = lib()
     _mixed_sigs(L)
     arr, blk = _bucket_array(buckets, block)
     g = C.c_void_p()
-    check(L.lf_mixed_graph_create(arr, len(buckets), blk, C.byref(g)))
+    gl = None if lanes is None else (C.c_uint * 4)(*[int(x) for x in lanes])
+    check(L.lf_mixed_graph_create_lanes(arr, len(buckets), blk, gl, C.byref(g)))
     return g.value
+
+
+def autotune_mixed_lanes(buckets, block=None):
+    """lf_autotune_mixed_lanes: the fastest lanes per value of each group for this batch's graph (cached per
+    device); returns ([lanes of the four groups], ms)."""
+    L = lib()
+    _mixed_sigs(L)
+    arr, blk = _bucket_array(buckets, block)
+    gl = (C.c_uint * 4)()
+    ms = C.c_float()
+    check(L.lf_autotune_mixed_lanes(arr, len(buckets), blk, gl, C.byref(ms)))
+    return list(gl), ms.value
 
 
 def graph_launch(graph, stream=None) -> None:

@@ -488,30 +488,28 @@ int lf_autotune_chain(int bits, uint64_t count, int steps, unsigned* lanes, floa
     LF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
     hipError_t err = hipEventCreate(&e0);
     if (err == hipSuccess) err = hipEventCreate(&e1);
-    unsigned best = 1;
-    float best_t = 1e30f;
     const unsigned options[3] = {1, 2, 4};
-    for (int o = 0; o < 3 && err == hipSuccess; ++o) {
-        const unsigned l = options[o];
-        if (ms) ms[o] = -1.0f;
-        if (l > 1 && N < 8) continue;
-        float t_min = 1e30f;
-        for (int rep = 0; rep < 4 && err == hipSuccess; ++rep) { // the first run warms
+    const int forms = N >= 8 ? 3 : 1;
+    float t_min[3] = {1e30f, 1e30f, 1e30f};
+    // every form runs once untimed, then the forms take turns (5 rounds, best time each), so that clock
+    // ramps and first launches weigh on no form alone
+    for (int rep = 0; rep < 6 && err == hipSuccess; ++rep) {
+        for (int o = 0; o < forms && err == hipSuccess; ++o) {
             err = hipEventRecord(e0, s);
-            if (err == hipSuccess) err = tables().chain_lanes[N](g, l, s);
+            if (err == hipSuccess) err = tables().chain_lanes[N](g, options[o], s);
             if (err == hipSuccess) err = hipEventRecord(e1, s);
             if (err == hipSuccess) err = hipEventSynchronize(e1);
             float t = 0;
             if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
-            if (rep && t < t_min) t_min = t;
-        }
-        if (err != hipSuccess) break;
-        if (ms) ms[o] = t_min;
-        if (t_min < best_t) {
-            best_t = t_min;
-            best = l;
+            if (rep && t < t_min[o]) t_min[o] = t;
         }
     }
+    int fastest = 0;
+    for (int o = 0; o < 3; ++o) {
+        if (ms) ms[o] = o < forms ? t_min[o] : -1.0f;
+        if (o < forms && t_min[o] < t_min[fastest]) fastest = o;
+    }
+    const unsigned best = options[fastest];
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(s);

@@ -108,6 +108,41 @@ __device__ __forceinline__ Num<N> coop_mul_by_digits(const Num<N>& a, const Num<
     if (!kill && !short_decides<N>(p)) digit_columns<N, true>(ad, bd, p, true);
     return round_product<N>(p, (i64)a.e + b.e, a.s * b.s, st, kill);
 }
+
+// The digit-column product of ad and bd, split over the group's lanes: the exact product p (2N words)
+// when full, else the short product (the digit_columns contract, without HIGH).
+template <int N, int G, bool FULL>
+__device__ __forceinline__ void coop_columns(const u32 (&ad)[digit_count<N>()], const u32 (&bs)[coop_factor_digits<N, G>()], u32 (&p)[2 * N])
+{
+    u64 S[coop_cols<N, G, FULL>()], col[2 * digit_count<N>()];
+    coop_partial<N, G, FULL>(ad, bs, S);
+    coop_gather<N, G, FULL>(S, col);
+    pack_columns<N, false>(col, p, FULL);
+}
+
+// RN(a * b + c) with one rounding (the fma_by_digits contract), the product's columns split over the
+// group's lanes: the exact product when c is nonzero, else the short product, made exact when it does
+// not decide the rounding.  Every lane of the group returns the same value.
+template <int N, int G>
+__device__ __forceinline__ Num<N> coop_fma_by_digits(const Num<N>& a, const Num<N>& b, const u32 (&bs)[coop_factor_digits<N, G>()],
+                                                     const Num<N>& c)
+{
+    const u32 st = a.st | b.st | c.st;
+    const int ps = a.s * b.s;
+    const bool kill = st || !ps;
+    u32 ad[digit_count<N>()], p[2 * N];
+    to_digits<N>(a.w, ad);
+    // the group's lanes hold the same operands and columns, so they take these branches together
+    bool full = c.s != 0;
+    if (!full) {
+        coop_columns<N, G, false>(ad, bs, p);
+        full = !kill && !short_decides<N>(p);
+    }
+    if (full) coop_columns<N, G, true>(ad, bs, p);
+    if (!c.s || st) return round_product<N>(p, (i64)a.e + b.e, ps, st, kill);
+    if (!ps) return c; // RN(c) = c
+    return fma_tail<N>(p, (i64)a.e + b.e, ps, c);
+}
 #endif
 
 } // namespace lf

@@ -60,7 +60,8 @@ template <int N> hipError_t launch_mixed_chain(const MixedArgs& g, unsigned bloc
 template <int N> const void* mixed_chain_entry();
 // The persistent kernel of width group G (0: 2-8 limbs, 1: 9-16, 2: 17-24, 3: 25-32); arguments: a device
 // table of MixedArgs, the limb count of each, and the first workgroup of each bucket (plus the grid size).
-template <int G> const void* mixed_group_entry();
+// the group kernel of group G with `lanes` lanes per value (1; 2 and 4 for groups 2 and 3), or null
+template <int G> const void* mixed_group_entry(int lanes);
 constexpr int mixed_group_of(int N) { return N <= 8 ? 0 : (N - 1) / 8; }
 
 } // namespace lf

@@ -86,50 +86,39 @@ __device__ __forceinline__ u32 sub_borrow(u32 x, u32 y, u32& bw)
     return r;
 }
 
-// x >>= d bits for a runtime d (any size); returns whether a set bit was shifted out.
-template <int W> __device__ __forceinline__ bool shr_sticky(u32 (&x)[W], u64 d)
+// x + y + cy (cy 0 or 1) in 32 bits, with the carry out in cy: one v_add_co / v_addc_co pair per word in a
+// chain, as sub_borrow is for differences.
+__device__ __forceinline__ u32 add_carry(u32 x, u32 y, u32& cy)
 {
-    bool sticky = false;
-    if (d >= 32ull * W) {
+    const u32 t = x + y;
+    const u32 r = t + cy;
+    cy = (u32)(t < x) | (u32)(r < t);
+    return r;
+}
+
+// x >>= d bits for a runtime d < 32 (KMAX + 1) (the caller bounds d); returns whether a set bit was
+// shifted out.  One select stage per bit of the word count d >> 5 up to KMAX; each stage ORs the words it
+// drops into the sticky flag (about W/2 instructions over all stages, where a select tree over prefix ORs
+// of the words costs one select per word per stage).  A right funnel shift by 0 bits is the identity, so
+// the bit shift needs no select.
+template <int W, int KMAX> __device__ __forceinline__ bool shr_sticky(u32 (&x)[W], u32 d)
+{
+    const u32 k = d >> 5, b = d & 31u;
+    u32 lost = 0;
 #pragma unroll
-        for (int i = 0; i < W; ++i) {
-            sticky = sticky || x[i] != 0;
-            x[i] = 0;
-        }
-        return sticky;
-    }
-    const int k = (int)(d >> 5);
-    const u32 b = (u32)(d & 31);
-    // the whole words shifted out are x[0 .. k-1]: their OR is the prefix OR pre[k], picked from the prefix
-    // ORs by a select tree over the bits of k (about 2W instructions; OR-ing the words each shifter stage
-    // drops costs about four per word per stage)
-    {
-        constexpr int P = W + 1;
-        u32 pre[P];
-        pre[0] = 0;
+    for (int s = 1; s <= KMAX; s <<= 1) {
+        const bool on = (k & (u32)s) != 0;
+        u32 o = 0;
 #pragma unroll
-        for (int j = 0; j < W; ++j) pre[j + 1] = pre[j] | x[j];
-#pragma unroll
-        for (int s = 1; s < P; s <<= 1) { // after this level, pre[j] holds the prefix OR at j + (k mod 2s) rounded
-            const bool on = (k & s) != 0;  // to the level: entries j stepped by s while the bit is set
-#pragma unroll
-            for (int j = 0; j < P; ++j) pre[j] = on ? (j + s < P ? pre[j + s] : pre[P - 1]) : pre[j];
-        }
-        sticky = pre[0] != 0;
-    }
-#pragma unroll
-    for (int s = 1; s < W; s <<= 1) {
-        const bool on = (k & s) != 0;
+        for (int i = 0; i < s && i < W; ++i) o |= x[i];
+        lost |= on ? o : 0u;
 #pragma unroll
         for (int i = 0; i < W; ++i) x[i] = on ? (i + s < W ? x[i + s] : 0u) : x[i];
     }
-    sticky = sticky || (b && (x[0] << (32u - b)) != 0);
+    lost |= b ? x[0] << (32u - b) : 0u;
 #pragma unroll
-    for (int i = 0; i < W; ++i) {
-        const u32 v = __builtin_amdgcn_alignbit(i + 1 < W ? x[i + 1] : 0u, x[i], b);
-        x[i] = b ? v : x[i];
-    }
-    return sticky;
+    for (int i = 0; i < W; ++i) x[i] = __builtin_amdgcn_alignbit(i + 1 < W ? x[i + 1] : 0u, x[i], b);
+    return lost != 0;
 }
 
 // dst (W words) = src (S words) * 2^off for a runtime bit offset; bits that fall below bit 0 of dst
@@ -1236,18 +1225,23 @@ template <int N> __device__ __forceinline__ Num<N> fma_tail(const u32 (&p)[2 * N
     if (c_msb < p_lsb - 1) return round_product<N>(p, eab, ps, ST_OK, false, ps * c.s);
     if (p_msb < c_lsb - 2) return c; // |p| < 2^(c_lsb-2)
     // Near case: a window of W = 2N+3 words with the larger term's top bit (top bit of P or of c, the one
-    // with the higher exponent) at window bit 32W-3; the smaller term, aligned to it, may reach below bit 0
-    // only when it lies 95 or more bits below the larger one's top (the larger term's lowest bit is at
-    // window bit 94 or higher), and then its bits below the window only decide a sticky unit: with T =
-    // 2 (big +/- small) +/- sticky, RN(T 2^(E0-1)) is the result (no rounding boundary of the N-word result
-    // lies within one window unit, and no cancellation beyond one bit is possible there).
+    // with the higher exponent) at window bit 32W-2; the smaller term, aligned to it, may reach below bit 0
+    // only when it lies 96 or more bits below the larger one's top (the larger term's lowest bit is at
+    // window bit 95 or higher), and then its bits below the window only decide the sticky flag.  The sum is
+    // formed once for both signs: T = big + (small or its complement) + carry-in, which is floor of the
+    // exact window value (a subtraction with a sticky remainder is one unit lower), and a sticky remainder
+    // sets T's last bit, so T lies in the same rounding cell as the exact value and is no tie (no rounding
+    // boundary of the N-word result lies within one window unit, and no cancellation beyond one bit is
+    // possible when a sticky remainder is).
     constexpr int W = 2 * N + 3;
     const bool p_big = p_msb >= c_msb;
     const i64 H = p_big ? p_msb : c_msb;
-    const u64 d = (u64)(p_big ? p_msb - c_msb : c_msb - p_msb);
-    // top-aligned images: P shifted left by 64 + pb bits (its top bit, 64N-1 or 64N-2, to 64N+93); c shifted
-    // left by 32(N+2) + 30 bits (its top bit 32N-1 to 64N+93)
-    const u32 pb = top ? 30u : 31u;
+    // the exponent gap: at most 64N with P the larger (c_msb >= p_lsb - 1) and 32N + 1 with c the larger
+    // (p_msb >= c_lsb - 2), so the small term moves by at most 2N words
+    const u32 d = (u32)(p_big ? p_msb - c_msb : c_msb - p_msb);
+    // top-aligned images: P shifted left by 64 + pb bits (its top bit, 64N-1 or 64N-2, to 64N+94); c shifted
+    // left by 32(N+2) + 31 bits (its top bit 32N-1 to 64N+94)
+    const u32 pb = top ? 31u : 32u;
     u32 big[W], sm[W];
 #pragma unroll
     for (int i = 0; i < W; ++i) {
@@ -1255,52 +1249,30 @@ template <int N> __device__ __forceinline__ Num<N> fma_tail(const u32 (&p)[2 * N
         const u32 pa = __builtin_amdgcn_alignbit(phi, plo, 32u - pb);
         const u32 chi = (i - (N + 2) >= 0 && i - (N + 2) < N) ? c.w[i - (N + 2)] : 0u;
         const u32 clo = (i - (N + 3) >= 0 && i - (N + 3) < N) ? c.w[i - (N + 3)] : 0u;
-        const u32 ca = __builtin_amdgcn_alignbit(chi, clo, 2u);
+        const u32 ca = __builtin_amdgcn_alignbit(chi, clo, 1u);
         big[i] = p_big ? pa : ca;
         sm[i] = p_big ? ca : pa;
     }
-    const bool sticky = shr_sticky<W>(sm, d);
+    const bool sticky = shr_sticky<W, 2 * N>(sm, d);
     const int sb = p_big ? ps : c.s, ss = p_big ? c.s : ps;
+    const bool sub = sb != ss;
+    const u32 flip = sub ? 0xffffffffu : 0u;
+    u32 T[W], cy = (sub && !sticky) ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < W; ++i) T[i] = add_carry(big[i], sm[i] ^ flip, cy);
+    T[0] |= sticky ? 1u : 0u;
     int sign = sb;
-    u32 T[W];
-    if (sb == ss) { // T = 2 (big + sm) + sticky
-        u32 cy = 0;
+    if (sub && (T[W - 1] >> 31)) { // the smaller-exponent term had the larger mantissa (d <= 1, no sticky): negate
+        sign = ss;
+        u32 c1 = 1;
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-            const u32 t = big[i] + sm[i];
-            T[i] = t + cy;
-            cy = (u32)(t < big[i]) | (u32)(T[i] < t);
-        }
-        shl_bits<W>(T, 1);
-        T[0] |= sticky ? 1u : 0u;
-    } else { // T = 2 (big - sm) - sticky, negated when negative
-        u32 bw = 0;
-#pragma unroll
-        for (int i = 0; i < W; ++i) {
-            const u32 d = big[i] - sm[i];
-            T[i] = d - bw;
-            bw = (u32)(big[i] < sm[i]) | (u32)(d < bw);
-        }
-        shl_bits<W>(T, 1);
-        bw = sticky ? 1u : 0u;
-#pragma unroll
-        for (int i = 0; i < W; ++i) {
-            const u32 t = T[i];
-            T[i] = t - bw;
-            bw = (u32)(t < bw);
-        }
-        if (T[W - 1] >> 31) { // the smaller-exponent term had the larger mantissa (d <= 1): negate
-            sign = ss;
-            u32 cy = 1;
-#pragma unroll
-            for (int i = 0; i < W; ++i) {
-                const u64 v = (u64)(u32)~T[i] + cy;
-                T[i] = (u32)v;
-                cy = (u32)(v >> 32);
-            }
+            const u64 v = (u64)(u32)~T[i] + c1;
+            T[i] = (u32)v;
+            c1 = (u32)(v >> 32);
         }
     }
-    return pack<N, W>(T, H - (64 * N + 93) - 1, sign, ST_OK);
+    return pack<N, W>(T, H - (64 * N + 94), sign, ST_OK);
 }
 
 template <int N> __device__ __forceinline__ Num<N> fma(const Num<N>& a, const Num<N>& b, const Num<N>& c)

@@ -8,7 +8,7 @@
 // (workgroups find their bucket in a device table), so one graph launch runs every precision concurrently
 // within the runtime's hardware queues.  The autotuner times the bucket
 // kernel of a width on a synthetic sample for each workgroup size and keeps the fastest per device and
-// width.
+// width; the lanes tuner times a batch's graph with 1, 2 or 4 lanes per value in each group of widths.
 #include "lf_internal.hpp"
 #include "limbforge_hip.h"
 
@@ -98,8 +98,39 @@ struct DevBuf {
     }
 };
 
-typedef const void* (*group_fn)();
+typedef const void* (*group_fn)(int);
 const group_fn group_entries[4] = {lf::mixed_group_entry<0>, lf::mixed_group_entry<1>, lf::mixed_group_entry<2>, lf::mixed_group_entry<3>};
+
+// Lanes per value of each group chosen by lf_autotune_mixed_lanes, per device; untuned: one lane.
+std::map<int, std::vector<unsigned>> tuned_lanes;
+
+void lanes_for(const unsigned* group_lanes, unsigned (&out)[4])
+{
+    for (int k = 0; k < 4; ++k) out[k] = 1;
+    if (group_lanes) {
+        for (int k = 0; k < 4; ++k) out[k] = group_lanes[k] ? group_lanes[k] : 1;
+        return;
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lock(tuned_mutex);
+    auto it = tuned_lanes.find(dev);
+    if (it != tuned_lanes.end())
+        for (int k = 0; k < 4; ++k) out[k] = it->second[k];
+}
+
+int check_lanes(const unsigned* group_lanes)
+{
+    if (!group_lanes) return LF_OK;
+    for (int k = 0; k < 4; ++k) {
+        const unsigned l = group_lanes[k];
+        if (l > 1 && !group_entries[k](int(l)))
+            return fail(LF_INVALID_ARGUMENT, "lanes per value must be 0 or 1 for the groups of 2-8 and 9-16 limbs, and 0, 1, 2 or 4 for the others");
+        if (l != 0 && l != 1 && l != 2 && l != 4)
+            return fail(LF_INVALID_ARGUMENT, "lanes per value must be 0, 1, 2 or 4");
+    }
+    return LF_OK;
+}
 
 // device tables of the graphs made by lf_mixed_graph_create, freed by lf_graph_destroy
 std::mutex tables_mutex;
@@ -139,7 +170,16 @@ int lf_mixed_chains(const lf_chain_bucket* buckets, int nbuckets, const unsigned
 
 int lf_mixed_graph_create(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, void** graph_exec)
 {
+    return lf_mixed_graph_create_lanes(buckets, nbuckets, block, nullptr, graph_exec);
+}
+
+int lf_mixed_graph_create_lanes(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, const unsigned* group_lanes,
+                                void** graph_exec)
+{
     if (!graph_exec) return fail(LF_INVALID_ARGUMENT, "null pointer");
+    if (int r = check_lanes(group_lanes)) return r;
+    unsigned lanes[4];
+    lanes_for(group_lanes, lanes);
     *graph_exec = nullptr;
     if (nbuckets < 0 || (nbuckets && !buckets)) return fail(LF_INVALID_ARGUMENT, "bad bucket list");
     for (int i = 0; i < nbuckets; ++i) {
@@ -169,7 +209,9 @@ int lf_mixed_graph_create(const lf_chain_bucket* buckets, int nbuckets, const un
         Group& G = grp[lf::mixed_group_of(b.bits / 32)];
         G.args.push_back(args_of(b));
         G.words.push_back(b.bits / 32);
-        G.first.push_back(G.first.back() + lf::grid_blocks(b.count, G.block));
+        const uint64_t blocks = (uint64_t)G.first.back() + lf::grid_blocks((uint64_t)b.count * lanes[lf::mixed_group_of(b.bits / 32)], G.block);
+        if (blocks > 0x7fffffffull) return fail(LF_INVALID_ARGUMENT, "a precision group of the batch needs more than 2^31 - 1 workgroups");
+        G.first.push_back(uint32_t(blocks));
     }
     std::size_t bytes = 0, off[4][3] = {};
     for (int k = 0; k < 4; ++k) {
@@ -210,7 +252,7 @@ int lf_mixed_graph_create(const lf_chain_bucket* buckets, int nbuckets, const un
         void* params[] = {&a, &w, &f, &nb};
         hipKernelNodeParams p;
         std::memset(&p, 0, sizeof p);
-        p.func = const_cast<void*>(group_entries[k]());
+        p.func = const_cast<void*>(group_entries[k](int(lanes[k])));
         p.gridDim = dim3(grp[k].first.back());
         p.blockDim = dim3(grp[k].block);
         p.sharedMemBytes = 0;
@@ -298,6 +340,74 @@ int lf_autotune_mixed(int bits, uint64_t sample, unsigned* block, double* best_m
         tuned[{dev, bits}] = best;
     }
     *block = best;
+    if (best_ms) *best_ms = best_t;
+    return LF_OK;
+}
+
+int lf_autotune_mixed_lanes(const lf_chain_bucket* buckets, int nbuckets, const unsigned* block, unsigned* group_lanes, float* best_ms)
+{
+    if (!group_lanes) return fail(LF_INVALID_ARGUMENT, "null pointer");
+    if (nbuckets < 1 || !buckets) return fail(LF_INVALID_ARGUMENT, "bad bucket list");
+    bool present[4] = {};
+    for (int i = 0; i < nbuckets; ++i) {
+        if (int r = check_bucket(buckets[i])) return r;
+        if (buckets[i].count) present[lf::mixed_group_of(buckets[i].bits / 32)] = true;
+    }
+    hipStream_t s = nullptr;
+    MIXED_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    // the batch's graph with the given lanes: best of three replays after a warm-up
+    auto time_graph = [&](const unsigned (&lanes)[4], float& ms) -> int {
+        void* x = nullptr;
+        if (int r = lf_mixed_graph_create_lanes(buckets, nbuckets, block, lanes, &x)) return r;
+        ms = 1e30f;
+        for (int rep = 0; rep < 4 && err == hipSuccess; ++rep) {
+            err = hipEventRecord(e0, s);
+            if (err == hipSuccess) err = hipGraphLaunch((hipGraphExec_t)x, s);
+            if (err == hipSuccess) err = hipEventRecord(e1, s);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float t = 0;
+            if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+            if (rep && t < ms) ms = t;
+        }
+        (void)hipGraphExecDestroy((hipGraphExec_t)x);
+        lf_mixed_graph_release(x);
+        return LF_OK;
+    };
+    // coordinate descent from one lane everywhere, widest group first (it bounds the batch)
+    unsigned best[4] = {1, 1, 1, 1};
+    float best_t = 1e30f;
+    int rc = err == hipSuccess ? time_graph(best, best_t) : LF_OK;
+    for (int k = 3; k >= 2 && rc == LF_OK && err == hipSuccess; --k) {
+        if (!present[k]) continue;
+        unsigned base = best[k];
+        for (unsigned l : {1u, 2u, 4u}) {
+            if (l == base) continue;
+            unsigned trial[4] = {best[0], best[1], best[2], best[3]};
+            trial[k] = l;
+            float t = 0;
+            rc = time_graph(trial, t);
+            if (rc != LF_OK || err != hipSuccess) break;
+            if (t < best_t) {
+                best_t = t;
+                best[k] = l;
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    if (rc != LF_OK) return rc;
+    if (err != hipSuccess) return fail(LF_RUNTIME_ERROR, std::string("autotune: ") + hipGetErrorString(err));
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    {
+        std::lock_guard<std::mutex> lock(tuned_mutex);
+        tuned_lanes[dev] = std::vector<unsigned>(best, best + 4);
+    }
+    for (int k = 0; k < 4; ++k) group_lanes[k] = best[k];
     if (best_ms) *best_ms = best_t;
     return LF_OK;
 }

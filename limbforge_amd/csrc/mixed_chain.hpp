@@ -8,6 +8,7 @@
 // step without the loop).  The called step needs 50-170 registers and no spills; from 15 limbs on the
 // inlined loop needs no spills and the call would only add its argument traffic.
 #pragma once
+#include "coop_core.hpp"
 #include "elementwise_impl.hpp"
 #include "lf_internal.hpp"
 
@@ -40,6 +41,23 @@ template <int N> __device__ __forceinline__ void chain_value(const MixedArgs& a,
         for (u32 t = 0; t < len; ++t) x = fma_by_digits<N>(x, f, fd, ((mask >> t) & 1u) ? g : zero<N>(ST_OK));
     }
     store_num<N>(a.out, a.pitch, i, x);
+}
+
+// Value i's chain on lane k of a group of G lanes (coop_core.hpp): every step's product columns are
+// split over the group; lane 0 stores the result.
+template <int N, int G> __device__ __forceinline__ void chain_value_coop(const MixedArgs& a, u32 i, u32 k)
+{
+    static_assert(use_digits<N>(), "cooperative chains need the digit product");
+    Num<N> x = load_num<N>(a.x, a.pitch, i);
+    const Num<N> f = load_num<N>(a.f, a.pitch, i);
+    const Num<N> g = load_num<N>(a.g, a.pitch, i);
+    const u32 len = a.len[i];
+    const u64 mask = a.mask[i];
+    u32 fd[digit_count<N>()], fs[coop_factor_digits<N, G>()];
+    to_digits<N>(f.w, fd);
+    coop_shift_factor<N, G>(fd, k, fs);
+    for (u32 t = 0; t < len; ++t) x = coop_fma_by_digits<N, G>(x, f, fs, ((mask >> t) & 1u) ? g : zero<N>(ST_OK));
+    if (k == 0) store_num<N>(a.out, a.pitch, i, x);
 }
 
 } // namespace lf

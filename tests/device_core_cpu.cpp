@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
+#include <gmp.h>
 #include <vector>
 
 static long failures = 0, checks = 0;
@@ -318,6 +319,8 @@ template <int N, int G, bool FULL> static void check_coop_once(uint64_t& state)
         for (int m = 0; m < M; ++m)
             if (LO + G * m + (int)k < 2 * D - 1) col[LO + G * m + k] = S[m];
     }
+    uint64_t col2[2 * D];
+    for (int c = 0; c < 2 * D; ++c) col2[c] = col[c];
     uint32_t got[2 * N], want[2 * N];
     lf::pack_columns<N, true>(col, got, FULL);
     lf::digit_columns<N, true>(ad, bd, want, FULL);
@@ -325,6 +328,13 @@ template <int N, int G, bool FULL> static void check_coop_once(uint64_t& state)
     bool same = true;
     for (int w = 0; w < 2 * N; ++w) same = same && got[w] == want[w];
     if (!same && ++failures <= 30) std::printf("MISMATCH coop columns N=%d G=%d full=%d\n", N, G, FULL ? 1 : 0);
+    // every word of the product (the form the fused multiply-add rounds against its addend)
+    lf::pack_columns<N, false>(col2, got, FULL);
+    lf::digit_columns<N, false>(ad, bd, want, FULL);
+    ++checks;
+    same = true;
+    for (int w = 0; w < 2 * N; ++w) same = same && got[w] == want[w];
+    if (!same && ++failures <= 30) std::printf("MISMATCH coop columns (all words) N=%d G=%d full=%d\n", N, G, FULL ? 1 : 0);
 }
 
 template <int N> static void check_coop(uint64_t& state, int cases)
@@ -340,6 +350,146 @@ template <int N> static void check_coop(uint64_t& state, int cases)
     if constexpr (N < 32) check_coop<N + 1>(state, cases);
 }
 
+// RN to N words (ties to even) of ps * P * 2^pl + c, exactly, with GMP: the reference for fma_tail on
+// arbitrary 2N-word P (P need not be a product of two N-word mantissas).
+template <int N> static lf::Num<N> exact_round_sum(const uint32_t (&p)[2 * N], long pl, int ps, const lf::Num<N>& c)
+{
+    const long cl = (long)c.e - (32L * N - 1), m = pl < cl ? pl : cl;
+    mpz_t P, C, V, q, r, half;
+    mpz_inits(P, C, V, q, r, half, NULL);
+    mpz_import(P, 2 * N, -1, 4, 0, 0, p);
+    mpz_import(C, N, -1, 4, 0, 0, c.w);
+    mpz_mul_2exp(P, P, (unsigned long)(pl - m));
+    mpz_mul_2exp(C, C, (unsigned long)(cl - m));
+    if (ps < 0) mpz_neg(P, P);
+    if (c.s < 0) mpz_neg(C, C);
+    mpz_add(V, P, C);
+    lf::Num<N> res = lf::zero<N>(0);
+    const int sg = mpz_sgn(V);
+    if (sg) {
+        mpz_abs(V, V);
+        long nb = (long)mpz_sizeinbase(V, 2);
+        const long sh = nb - 32L * N;
+        if (sh <= 0) mpz_mul_2exp(q, V, (unsigned long)-sh);
+        else {
+            mpz_fdiv_q_2exp(q, V, (unsigned long)sh);
+            mpz_fdiv_r_2exp(r, V, (unsigned long)sh);
+            mpz_set_ui(half, 1);
+            mpz_mul_2exp(half, half, (unsigned long)(sh - 1));
+            const int cmp = mpz_cmp(r, half);
+            if (cmp > 0 || (cmp == 0 && mpz_odd_p(q))) mpz_add_ui(q, q, 1);
+            if ((long)mpz_sizeinbase(q, 2) > 32L * N) {
+                mpz_fdiv_q_2exp(q, q, 1);
+                ++nb;
+            }
+        }
+        uint32_t w[N] = {};
+        size_t cnt = 0;
+        mpz_export(w, &cnt, -1, 4, 0, 0, q);
+        for (int i = 0; i < N; ++i) res.w[i] = w[i];
+        res.e = (int)(m + nb - 1);
+        res.s = sg;
+    }
+    mpz_clears(P, C, V, q, r, half, NULL);
+    return res;
+}
+
+// fma_tail's window path where the result sits next to a rounding tie and only the sticky remainder of
+// the smaller term (its bits below the window, with a zero gap above them inside the window) decides it:
+// P the larger term with P + C_hi or P - C_hi an exact tie (c's bits just below P's last bit zero, its
+// lowest bit set), and c the larger term with C + P_hi carrying to an odd 32N+1-bit value or C - P_hi
+// leaving exactly half a unit less the remainder; plus c anywhere in the window range.  Against GMP.
+template <int N> static void check_fma_tail_sticky(uint64_t& state, int cases)
+{
+    if constexpr (N >= 4) {
+        auto next = [&] {
+            state ^= state << 13;
+            state ^= state >> 7;
+            state ^= state << 17;
+            return state;
+        };
+        mpz_t P, C, X, t, u;
+        mpz_inits(P, C, X, t, u, NULL);
+        for (int it = 0; it < cases; ++it)
+            for (int scen = 0; scen < 5; ++scen) {
+                uint32_t p[2 * N];
+                for (int i = 0; i < 2 * N; ++i) p[i] = (uint32_t)next();
+                const int top = (int)(next() & 1);
+                p[2 * N - 1] = top ? (p[2 * N - 1] | 0x80000000u) : ((p[2 * N - 1] & 0x7fffffffu) | 0x40000000u);
+                const long eab = (long)(next() % 401) - 200, pl = eab - 2 * (32L * N - 1), pmsb = eab + top;
+                lf::Num<N> c;
+                for (int i = 0; i < N; ++i) c.w[i] = (uint32_t)next();
+                c.w[N - 1] |= 0xc0000000u;
+                c.st = 0;
+                const bool add = scen % 2 == 0;
+                const int ps = (next() & 1) ? 1 : -1;
+                c.s = add ? ps : -ps;
+                mpz_import(P, 2 * N, -1, 4, 0, 0, p);
+                if (scen < 2) { // P the larger term
+                    const long d = 32L * N + 96 + (long)(next() % (uint64_t)(32 * N - 97));
+                    c.e = (int)(pmsb - d);
+                    const long cl = (long)c.e - (32L * N - 1);
+                    mpz_import(C, N, -1, 4, 0, 0, c.w);
+                    for (long j = pl - 96 - cl; j < pl - cl; ++j)
+                        if (j >= 0 && j < 32L * N) mpz_clrbit(C, (unsigned long)j);
+                    mpz_setbit(C, 0);
+                    mpz_fdiv_q_2exp(t, C, (unsigned long)(pl - cl)); // C_hi at P's scale
+                    if (add) mpz_add(X, P, t);
+                    else mpz_sub(X, P, t);
+                    const long sh = (long)mpz_sizeinbase(X, 2) - 32L * N;
+                    mpz_fdiv_r_2exp(u, X, (unsigned long)sh);
+                    mpz_sub(P, P, u);
+                    mpz_setbit(t, 0); // any value: reuse t for 2^(sh-1)
+                    mpz_set_ui(t, 1);
+                    mpz_mul_2exp(t, t, (unsigned long)(sh - 1));
+                    mpz_add(P, P, t);
+                    uint32_t w[N] = {};
+                    size_t cnt = 0;
+                    mpz_export(w, &cnt, -1, 4, 0, 0, C);
+                    for (int i = 0; i < N; ++i) c.w[i] = w[i];
+                } else if (scen < 4) { // c the larger term
+                    const long d = 97 + (long)(next() % (uint64_t)(32 * N - 95));
+                    c.e = (int)(pmsb + d);
+                    const long cl = (long)c.e - (32L * N - 1), w0 = (long)c.e - (64L * N + 94);
+                    for (long j = w0 - pl; j < cl - pl; ++j)
+                        if (j >= 0 && j < 64L * N) mpz_clrbit(P, (unsigned long)j);
+                    mpz_setbit(P, 0);
+                    mpz_fdiv_q_2exp(t, P, (unsigned long)(cl - pl)); // P_hi at c's scale
+                    if (add) { // C = 2^(32N) - P_hi + j, j odd: C + P_hi = 2^(32N) + j
+                        mpz_set_ui(C, 1);
+                        mpz_mul_2exp(C, C, 32UL * N);
+                        mpz_sub(C, C, t);
+                        mpz_add_ui(C, C, 1 + 2 * (next() % 1000));
+                    } else { // P's window bits below c: only the half unit, so C - P = (C - P_hi - 1) + half - rem
+                        mpz_import(C, N, -1, 4, 0, 0, c.w);
+                        if (cl - 1 - pl >= 0) mpz_setbit(P, (unsigned long)(cl - 1 - pl));
+                    }
+                    uint32_t w[N] = {};
+                    size_t cnt = 0;
+                    mpz_export(w, &cnt, -1, 4, 0, 0, C);
+                    for (int i = 0; i < N; ++i) c.w[i] = w[i];
+                } else { // c anywhere in the window range
+                    c.e = (int)(pl - 3 + (long)(next() % (uint64_t)(96 * N + 6)));
+                }
+                uint32_t q[2 * N] = {};
+                size_t cnt = 0;
+                mpz_export(q, &cnt, -1, 4, 0, 0, P);
+                if (cnt > (size_t)(2 * N) || (q[2 * N - 1] >> 30) != (top ? 2u + (q[2 * N - 1] >> 30 & 1u) : 1u)) continue;
+                if (!(c.w[N - 1] >> 31)) continue;
+                for (int i = 0; i < 2 * N; ++i) p[i] = q[i];
+                const lf::Num<N> got = lf::fma_tail<N>(p, eab, ps, c), want = exact_round_sum<N>(p, pl, ps, c);
+                ++checks;
+                bool ok = got.s == want.s && got.st == want.st && (!got.s || got.e == want.e);
+                for (int i = 0; i < N && ok && got.s; ++i) ok = got.w[i] == want.w[i];
+                if (!ok && ++failures <= 30)
+                    std::printf("MISMATCH fma_tail sticky N=%d scenario=%d case=%d got(s=%d e=%d top=%08x lo=%08x) want(s=%d e=%d top=%08x lo=%08x)\n",
+                                N, scen, it, got.s, got.e, got.w[N - 1], got.w[0], want.s, want.e, want.w[N - 1], want.w[0]);
+            }
+        mpz_clears(P, C, X, t, u, NULL);
+    }
+    if constexpr (N < 32) check_fma_tail_sticky<N + 1>(state, cases);
+}
+
 int main(int argc, char** argv)
 {
     int lo = argc > 1 ? std::atoi(argv[1]) : 2, hi = argc > 2 ? std::atoi(argv[2]) : 32, cases = argc > 3 ? std::atoi(argv[3]) : 300;
@@ -347,6 +497,8 @@ int main(int argc, char** argv)
     check_residue_combine();
     uint64_t coop_state = 0x2545F4914F6CDD1Dull;
     check_coop<8>(coop_state, 200);
+    uint64_t tail_state = 0x9E3779B97F4A7C15ull;
+    check_fma_tail_sticky<4>(tail_state, cases);
     std::printf("device core (host build): %ld checks, %ld failures\n", checks, failures);
     return failures ? 1 : 0;
 }

@@ -94,6 +94,59 @@ def test_graph_replays_with_new_inputs(capi, oracle):
         capi.graph_destroy(graph)
 
 
+def test_graph_lanes_exact_products(capi, oracle):
+    """The 17-24 and 25-32-limb groups can split each value's products over 2 or 4 lanes: exact products with
+    ties (factor 1.5 on odd mantissas, the short product's undecided case), factors 1 + 2^-(32n-1), addends
+    that cancel the product exactly or lie far above or below it, with value counts that leave partial
+    wavefronts; every lanes choice gives the oracle's values, as do the per-precision (one-lane) launches."""
+    rng = np.random.default_rng(6)
+    widths = [576, 736, 800, 832, 960, 1024]
+    batch = {}
+    for bits, k in zip(widths, (45, 200, 61, 130, 257, 1000)):
+        n = bits // 32
+        x = random_records(rng, bits, k, 64)
+        f = random_records(rng, bits, k, 64)
+        g = random_records(rng, bits, k, 64)
+        x[:, 0] |= 1
+        f[::3, :n] = 0
+        f[::3, n - 1] = 0xC0000000  # 1.5 * 2^(e-1)
+        f[1::7, :n] = 0
+        f[1::7, n - 1] = 0x80000000
+        f[1::7, 0] = 1
+        g[2::5] = x[2::5]  # an addend equal to the start value: x f + x
+        g[2::5, n + 1] = (-g[2::5, n + 1].view(np.int32)).view(np.uint32)
+        g[3::11, n] = (g[3::11, n].view(np.int32) + 3000).view(np.uint32)  # far above
+        g[4::13, n] = (g[4::13, n].view(np.int32) - 3000).view(np.uint32)  # far below
+        x[::37] = zero(bits)
+        length = rng.integers(0, 65, size=k).astype(np.uint32)
+        length[:2] = [64, 1]
+        mask = rng.integers(0, 1 << 63, size=k, dtype=np.uint64) * 2 + rng.integers(0, 2, size=k).astype(np.uint64)
+        order = np.argsort(-length.astype(np.int64), kind="stable")
+        batch[bits] = tuple(a[order] for a in (x, f, g, length, mask))
+    buckets = [capi.MixedBucket(bits, *batch[bits]) for bits in widths]
+    stream = capi.Stream()
+    for lanes in ([1, 1, 1, 1], [1, 1, 2, 4], [0, 0, 4, 2]):
+        print(f"[gpu] mixed graph, lanes per group {lanes}, exact products", flush=True)
+        graph = capi.mixed_graph(buckets, lanes=lanes)
+        try:
+            capi.graph_launch(graph, stream.handle)
+            stream.synchronize()
+        finally:
+            capi.graph_destroy(graph)
+        check(capi, oracle, batch, buckets)
+    print("[gpu] the same batch, per-precision launches", flush=True)
+    capi.mixed_chains(buckets)
+    capi.lib().lf_stream_synchronize(None)
+    check(capi, oracle, batch, buckets)
+    print("[gpu] lanes tuner on the batch", flush=True)
+    lanes, ms = capi.autotune_mixed_lanes(buckets)
+    assert lanes[0] == 1 and lanes[1] == 1 and lanes[2] in (1, 2, 4) and lanes[3] in (1, 2, 4) and ms > 0
+    check(capi, oracle, batch, buckets)
+    for bad in ([2, 1, 1, 1], [1, 4, 1, 1], [1, 1, 3, 1]):
+        with pytest.raises(ValueError, match="lanes per value"):
+            capi.mixed_graph(buckets, lanes=bad)
+
+
 def test_invalid_shapes(capi):
     rng = np.random.default_rng(5)
     batch = make_batch(rng, [64], 64)
