@@ -4,6 +4,7 @@
 #include "limbforge_hip.h"
 #include "lf_internal.hpp"
 #include "host_run.hpp"
+#include "host_copy.hpp"
 
 #include <chrono>
 #include <cstdio>
@@ -561,13 +562,13 @@ int lf_cast_host(int from_bits, int to_bits, int complex_values, const void* in,
     LF_HIP(st.reserve(4, lf_planes_bytes(to_bits, complex_values, count)), "hipMalloc (staging)");
     hipStream_t s = st.stream;
     lf_operand I{st.dev[1], pitch}, O{st.dev[4], pitch};
-    LF_HIP(hipMemcpyAsync(st.dev[0], in, count * Ein, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    LF_HIP(lf::copy_to_device(st.dev[0], in, count * Ein, s), "upload");
     if (int r = lf_aos_to_planes(from_bits, complex_values, st.dev[0], &I, count, s)) return r;
     LF_HIP(hipEventRecord(st.ev[0], s), "hipEventRecord");
     if (int r = lf_cast(from_bits, to_bits, complex_values, &I, &O, count, s)) return r;
     LF_HIP(hipEventRecord(st.ev[1], s), "hipEventRecord");
     if (int r = lf_planes_to_aos(to_bits, complex_values, &O, st.dev[0], count, s)) return r;
-    LF_HIP(hipMemcpyAsync(out, st.dev[0], count * Eout, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    LF_HIP(lf::copy_to_host(out, st.dev[0], count * Eout, s), "download");
     LF_HIP(hipStreamSynchronize(s), "device execution");
     float ms = 0;
     LF_HIP(hipEventElapsedTime(&ms, st.ev[0], st.ev[1]), "hipEventElapsedTime");
@@ -631,27 +632,30 @@ int plan_host_run(int bits, int op, const void* a, const void* b, const void* c,
 }
 
 int stage_and_run(const HostRun& R, const void* a, const void* b, const void* c, void* out, void* const* dev, hipStream_t s,
-                  const hipEvent_t* ev)
+                  const hipEvent_t* ev, bool ring)
 {
+    auto up = [&](void* d, const void* h, std::size_t n) {
+        return ring ? lf::copy_to_device(d, h, n, s) : hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s);
+    };
     const int bits = R.bits, cplx = R.cplx;
     const uint64_t count = R.count, nb = R.nb, nc = R.nc, E = R.E;
     lf_operand A{dev[1], lf_plane_pitch(count)}, B{dev[2], lf_plane_pitch(nb)}, C{dev[3], lf_plane_pitch(nc)}, O{dev[4], lf_plane_pitch(count)};
     // stage in: AoS bytes to the device, then transposed into planes
-    LF_HIP(hipMemcpyAsync(dev[0], a, count * E, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    LF_HIP(up(dev[0], a, count * E), "upload");
     if (int r = lf_aos_to_planes(bits, cplx, dev[0], &A, count, s)) return r;
     if (nb) {
-        LF_HIP(hipMemcpyAsync(dev[0], b, nb * E, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+        LF_HIP(up(dev[0], b, nb * E), "upload");
         if (int r = lf_aos_to_planes(bits, cplx, dev[0], &B, nb, s)) return r;
     }
     if (nc) {
-        LF_HIP(hipMemcpyAsync(dev[0], c, nc * E, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+        LF_HIP(up(dev[0], c, nc * E), "upload");
         if (int r = lf_aos_to_planes(bits, cplx, dev[0], &C, nc, s)) return r;
     }
     if (ev) LF_HIP(hipEventRecord(ev[0], s), "hipEventRecord");
     if (int r = lf_elementwise(bits, R.op, &A, nb ? &B : &A, nc ? &C : &A, &O, count, R.b_index, R.c_index, s)) return r;
     if (ev) LF_HIP(hipEventRecord(ev[1], s), "hipEventRecord");
     if (int r = lf_planes_to_aos(bits, cplx, &O, dev[0], count, s)) return r;
-    LF_HIP(hipMemcpyAsync(out, dev[0], count * E, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    LF_HIP(ring ? lf::copy_to_host(out, dev[0], count * E, s) : hipMemcpyAsync(out, dev[0], count * E, hipMemcpyDeviceToHost, s), "download");
     LF_HIP(hipStreamSynchronize(s), "device execution");
     return LF_OK;
 }

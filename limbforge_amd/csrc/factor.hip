@@ -15,6 +15,7 @@
 // off-block updates as products of the update GEMM.
 #include "factor_launch.hpp"
 #include "gemm_plan.hpp"
+#include "host_copy.hpp"
 #include "limbforge_hip.h"
 
 #include <algorithm>
@@ -489,12 +490,12 @@ int lf_cholesky_host(int bits, const void* A, uint64_t n, void* L, const lf_fact
     FACTOR_HIP(aos.alloc(count * E), "hipMalloc (staging)");
     FACTOR_HIP(planes.alloc(lf_planes_bytes(bits, 0, count)), "hipMalloc (work matrix)");
     lf_operand W{planes.p, lf_plane_pitch(count)};
-    FACTOR_HIP(hipMemcpyAsync(aos.p, A, count * E, hipMemcpyHostToDevice, st.s), "upload A");
+    FACTOR_HIP(lf::copy_to_device(aos.p, A, count * E, st.s), "upload A");
     if (int r = lf_aos_to_planes(bits, 0, aos.p, &W, count, st.s)) return r;
     lf_cholesky_info ci;
     if (int r = lf_cholesky(bits, &W, n, &W, &fo, &lo, &ci, st.s)) return r;
     if (int r = lf_planes_to_aos(bits, 0, &W, aos.p, count, st.s)) return r;
-    FACTOR_HIP(hipMemcpyAsync(L, aos.p, count * E, hipMemcpyDeviceToHost, st.s), "download L");
+    FACTOR_HIP(lf::copy_to_host(L, aos.p, count * E, st.s), "download L");
     FACTOR_HIP(hipStreamSynchronize(st.s), "device execution (cholesky)");
     ci.timing_wall_seconds = since(start);
     if (info) *info = ci;
@@ -521,15 +522,15 @@ int lf_solve_host(int bits, int passes, const void* L, uint64_t n, const void* B
     FACTOR_HIP(fplanes.alloc(lf_planes_bytes(bits, 0, fc)), "hipMalloc (factor)");
     FACTOR_HIP(xplanes.alloc(lf_planes_bytes(bits, 0, xc)), "hipMalloc (right-hand sides)");
     lf_operand F{fplanes.p, lf_plane_pitch(fc)}, Xo{xplanes.p, lf_plane_pitch(xc)};
-    FACTOR_HIP(hipMemcpyAsync(aos.p, L, fc * E, hipMemcpyHostToDevice, st.s), "upload L");
+    FACTOR_HIP(lf::copy_to_device(aos.p, L, fc * E, st.s), "upload L");
     if (int r = lf_aos_to_planes(bits, 0, aos.p, &F, fc, st.s)) return r;
     FACTOR_HIP(hipStreamSynchronize(st.s), "upload L");
-    FACTOR_HIP(hipMemcpyAsync(aos.p, B, xc * E, hipMemcpyHostToDevice, st.s), "upload B");
+    FACTOR_HIP(lf::copy_to_device(aos.p, B, xc * E, st.s), "upload B");
     if (int r = lf_aos_to_planes(bits, 0, aos.p, &Xo, xc, st.s)) return r;
     double g = 0;
     if (int r = lf_solve(bits, passes, &F, n, &Xo, nrhs, &Xo, &fo, &lo, &g, nullptr, st.s)) return r;
     if (int r = lf_planes_to_aos(bits, 0, &Xo, aos.p, xc, st.s)) return r;
-    FACTOR_HIP(hipMemcpyAsync(X, aos.p, xc * E, hipMemcpyDeviceToHost, st.s), "download X");
+    FACTOR_HIP(lf::copy_to_host(X, aos.p, xc * E, st.s), "download X");
     FACTOR_HIP(hipStreamSynchronize(st.s), "device execution (solve)");
     if (gpu_seconds) *gpu_seconds = g;
     if (wall_seconds) *wall_seconds = since(start);

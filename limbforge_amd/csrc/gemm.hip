@@ -13,6 +13,7 @@
 #include "gemm_launch.hpp"
 #include "gemm_plan.hpp"
 #include "residue_reduce.hpp"
+#include "host_copy.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -965,39 +966,19 @@ namespace {
 using lf::check_linalg_options;
 
 // Staging of the host-array products, kept per host thread and device and grown on demand, so repeated
-// calls allocate nothing after the first: a stream, pinned host memory the host arrays are copied through
-// (in chunks, the copy of one chunk overlapping the DMA of the previous one), events that pace the chunked
-// download, and the device AoS / limb-plane buffers (planner pool slots).
+// calls allocate nothing after the first: a stream and the device AoS / limb-plane buffers (planner pool
+// slots); the host arrays pass through host_copy's pinned chunk ring.
 struct HostStage {
     int device = -1;
     hipStream_t stream = nullptr;
-    char* pinned = nullptr;
-    std::size_t pinned_bytes = 0;
-    hipEvent_t ring[8] = {};
     ~HostStage()
     {
-        if (pinned) (void)hipHostFree(pinned);
-        for (auto e : ring)
-            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
     hipError_t ready()
     {
         if (!stream)
             if (hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) return e;
-        for (auto& e : ring)
-            if (!e)
-                if (hipError_t x = hipEventCreateWithFlags(&e, hipEventDisableTiming)) return x;
-        return hipSuccess;
-    }
-    hipError_t pin(std::size_t bytes)
-    {
-        if (bytes <= pinned_bytes) return hipSuccess;
-        if (pinned) (void)hipHostFree(pinned);
-        pinned = nullptr;
-        pinned_bytes = 0;
-        if (hipError_t e = hipHostMalloc((void**)&pinned, bytes, hipHostMallocDefault)) return e;
-        pinned_bytes = bytes;
         return hipSuccess;
     }
 };
@@ -1007,39 +988,6 @@ HostStage& host_stage()
     int device = 0;
     (void)hipGetDevice(&device);
     return stages[device & 15];
-}
-constexpr std::size_t STAGE_CHUNK = std::size_t(4) << 20;
-
-// host -> device through pinned bytes [at, at + bytes): no wait for the DMA (the region is not reused
-// within the call)
-hipError_t stage_upload(HostStage& st, std::size_t at, void* dev, const void* host, std::size_t bytes)
-{
-    for (std::size_t off = 0; off < bytes; off += STAGE_CHUNK) {
-        const std::size_t len = std::min(STAGE_CHUNK, bytes - off);
-        std::memcpy(st.pinned + at + off, static_cast<const char*>(host) + off, len);
-        if (hipError_t e = hipMemcpyAsync(static_cast<char*>(dev) + off, st.pinned + at + off, len, hipMemcpyHostToDevice, st.stream)) return e;
-    }
-    return hipSuccess;
-}
-// device -> host through pinned bytes [0, bytes): every chunk's DMA is queued, then each chunk is copied out
-// as soon as its event completes (the copy of one chunk overlaps the DMA of the next)
-hipError_t stage_download(HostStage& st, void* host, const void* dev, std::size_t bytes)
-{
-    const std::size_t chunks = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
-    std::size_t queued = 0, done = 0;
-    while (done < chunks) {
-        while (queued < chunks && queued < done + 8) {
-            const std::size_t off = queued * STAGE_CHUNK, len = std::min(STAGE_CHUNK, bytes - off);
-            if (hipError_t e = hipMemcpyAsync(st.pinned + off, static_cast<const char*>(dev) + off, len, hipMemcpyDeviceToHost, st.stream)) return e;
-            if (hipError_t e = hipEventRecord(st.ring[queued % 8], st.stream)) return e;
-            ++queued;
-        }
-        if (hipError_t e = hipEventSynchronize(st.ring[done % 8])) return e;
-        const std::size_t off = done * STAGE_CHUNK, len = std::min(STAGE_CHUNK, bytes - off);
-        std::memcpy(static_cast<char*>(host) + off, st.pinned + off, len);
-        ++done;
-    }
-    return hipSuccess;
 }
 
 } // namespace
@@ -1090,20 +1038,19 @@ int lf_gemm_host(int bits, int transpose_a, const void* A, const void* B, uint64
     GEMM_HIP(pa.alloc(lf_planes_bytes(bits, 0, std::max<uint64_t>(na, 1))), "hipMalloc (A)");
     GEMM_HIP(pb.alloc(lf_planes_bytes(bits, 0, std::max<uint64_t>(nb, 1))), "hipMalloc (B)");
     GEMM_HIP(pc.alloc(lf_planes_bytes(bits, 0, nc)), "hipMalloc (C)");
-    GEMM_HIP(st.pin((na + nb + nc) * E), "hipHostMalloc (staging)");
     char* daos = static_cast<char*>(aos.p);
     lf_operand OA{pa.p, lf_plane_pitch(std::max<uint64_t>(na, 1))}, OB{pb.p, lf_plane_pitch(std::max<uint64_t>(nb, 1))},
         OC{pc.p, lf_plane_pitch(nc)};
     if (na) {
-        GEMM_HIP(stage_upload(st, 0, daos, A, na * E), "upload A");
+        GEMM_HIP(lf::copy_to_device(daos, A, na * E, st.stream), "upload A");
         if (int r = lf_aos_to_planes(bits, 0, daos, &OA, na, st.stream)) return r;
     }
     if (nb) {
-        GEMM_HIP(stage_upload(st, na * E, daos + na * E, B, nb * E), "upload B");
+        GEMM_HIP(lf::copy_to_device(daos + na * E, B, nb * E, st.stream), "upload B");
         if (int r = lf_aos_to_planes(bits, 0, daos + na * E, &OB, nb, st.stream)) return r;
     }
     if (subtract) {
-        GEMM_HIP(stage_upload(st, (na + nb) * E, daos + (na + nb) * E, C, nc * E), "upload C");
+        GEMM_HIP(lf::copy_to_device(daos + (na + nb) * E, C, nc * E, st.stream), "upload C");
         if (int r = lf_aos_to_planes(bits, 0, daos + (na + nb) * E, &OC, nc, st.stream)) return r;
     }
     Product P{};
@@ -1123,7 +1070,7 @@ int lf_gemm_host(int bits, int transpose_a, const void* A, const void* B, uint64
     float ms = 0;
     if (int r = run_product(P, report, &ms)) return r;
     if (int r = lf_planes_to_aos(bits, 0, &OC, daos, nc, st.stream)) return r;
-    GEMM_HIP(stage_download(st, C, daos, nc * E), "download C");
+    GEMM_HIP(lf::copy_to_host(C, daos, nc * E, st.stream), "download C");
     if (gpu_seconds) *gpu_seconds = ms * 1e-3;
     if (wall_seconds) *wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     return LF_OK;
@@ -1267,15 +1214,14 @@ int lf_syrk_host(int bits, const void* A, uint64_t rows, uint64_t cols, void* C,
     GEMM_HIP(aos.alloc((na + nc) * E), "hipMalloc (staging)");
     GEMM_HIP(pa.alloc(lf_planes_bytes(bits, 0, std::max<uint64_t>(na, 1))), "hipMalloc (A)");
     GEMM_HIP(pc.alloc(lf_planes_bytes(bits, 0, nc)), "hipMalloc (C)");
-    GEMM_HIP(st.pin((na + nc) * E), "hipHostMalloc (staging)");
     char* daos = static_cast<char*>(aos.p);
     lf_operand OA{pa.p, lf_plane_pitch(std::max<uint64_t>(na, 1))}, OC{pc.p, lf_plane_pitch(nc)};
     if (na) {
-        GEMM_HIP(stage_upload(st, 0, daos, A, na * E), "upload A");
+        GEMM_HIP(lf::copy_to_device(daos, A, na * E, st.stream), "upload A");
         if (int r = lf_aos_to_planes(bits, 0, daos, &OA, na, st.stream)) return r;
     }
     if (subtract || lower_only) { // untouched entries keep their old values
-        GEMM_HIP(stage_upload(st, na * E, daos + na * E, C, nc * E), "upload C");
+        GEMM_HIP(lf::copy_to_device(daos + na * E, C, nc * E, st.stream), "upload C");
         if (int r = lf_aos_to_planes(bits, 0, daos + na * E, &OC, nc, st.stream)) return r;
     }
     Product P{};
@@ -1295,7 +1241,7 @@ int lf_syrk_host(int bits, const void* A, uint64_t rows, uint64_t cols, void* C,
     float ms = 0;
     if (int r = run_product(P, report, &ms)) return r;
     if (int r = lf_planes_to_aos(bits, 0, &OC, daos, nc, st.stream)) return r;
-    GEMM_HIP(stage_download(st, C, daos, nc * E), "download C");
+    GEMM_HIP(lf::copy_to_host(C, daos, nc * E, st.stream), "download C");
     if (gpu_seconds) *gpu_seconds = ms * 1e-3;
     if (wall_seconds) *wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     return LF_OK;

@@ -23,9 +23,11 @@ int plan_host_run(int bits, int op, const void* a, const void* b, const void* c,
                   const lf_broadcast* c_index, HostRun& R);
 // Copies the host operands into device staging dev[0..4] (sized by plan_host_run), transposes them into
 // limb planes, runs the operation and brings the result back; returns after stream s has finished.
-// ev (may be null): two events recorded around the operation's kernel.
+// ev (may be null): two events recorded around the operation's kernel.  ring: the host arrays pass through
+// the calling thread's pinned chunk ring (host_copy.hpp, allocated on its first use); false: plain
+// hipMemcpyAsync from the host arrays, for callers that must not allocate.
 int stage_and_run(const HostRun& R, const void* a, const void* b, const void* c, void* out, void* const* dev, hipStream_t s,
-                  const hipEvent_t* ev);
+                  const hipEvent_t* ev, bool ring = true);
 // Records `message` as this host thread's lf_last_error and returns `code`.
 int set_error(int code, const char* message);
 } // namespace lf

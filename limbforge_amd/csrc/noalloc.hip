@@ -58,7 +58,7 @@ int lf_run_host_noalloc(int bits, int op, const void* a, const void* b, const vo
         dev[k] = base;
         base += rounded(R.bytes[k]);
     }
-    return lf::stage_and_run(R, a, b, c, out, dev, (hipStream_t)stream, nullptr);
+    return lf::stage_and_run(R, a, b, c, out, dev, (hipStream_t)stream, nullptr, false);
 }
 
 int lf_noalloc_scratch_bytes(uint64_t* bytes)

@@ -2,6 +2,7 @@
 // CommandBatch::segmented_dot) and the adjacent-pair tree sum (CommandBatch::tree_sum); argument checks with
 // the reference's error classes and the width dispatch to reduce_w.hip.
 #include "host_run.hpp"
+#include "host_copy.hpp"
 #include "limbforge_hip.h"
 #include "reduce_launch.hpp"
 
@@ -124,9 +125,9 @@ int lf_segmented_dot_host(int bits, int complex_values, uint64_t segments, uint6
     RED_HIP(hipMalloc(&po.p, lf_planes_bytes(bits, cplx, segments)), "hipMalloc (out)");
     lf_operand A{pa.p, lf_plane_pitch(na ? na : 1)}, B{pb.p, lf_plane_pitch(nb ? nb : 1)}, O{po.p, lf_plane_pitch(segments)};
     if (na) {
-        RED_HIP(hipMemcpyAsync(stage.p, a, na * E, hipMemcpyHostToDevice, s), "upload a");
+        RED_HIP(lf::copy_to_device(stage.p, a, na * E, s), "upload a");
         if (int r = lf_aos_to_planes(bits, cplx, stage.p, &A, na, s)) return r;
-        RED_HIP(hipMemcpyAsync(stage.p, b, nb * E, hipMemcpyHostToDevice, s), "upload b");
+        RED_HIP(lf::copy_to_device(stage.p, b, nb * E, s), "upload b");
         if (int r = lf_aos_to_planes(bits, cplx, stage.p, &B, nb, s)) return r;
     }
     hipEvent_t e0, e1;
@@ -144,7 +145,7 @@ int lf_segmented_dot_host(int bits, int complex_values, uint64_t segments, uint6
     if (int r = lf_segmented_dot(bits, cplx, segments, length, shared_right, &A, &B, &O, s)) return r;
     RED_HIP(hipEventRecord(e1, s), "hipEventRecord");
     if (int r = lf_planes_to_aos(bits, cplx, &O, stage.p, segments, s)) return r;
-    RED_HIP(hipMemcpyAsync(out, stage.p, segments * E, hipMemcpyDeviceToHost, s), "download out");
+    RED_HIP(lf::copy_to_host(out, stage.p, segments * E, s), "download out");
     RED_HIP(hipStreamSynchronize(s), "device execution (segmented dot)");
     float ms = 0;
     RED_HIP(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
