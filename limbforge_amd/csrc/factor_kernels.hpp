@@ -509,11 +509,13 @@ template <int N> __device__ __forceinline__ void store_recip(u32* r, u64 rpitch,
     for (int k = 0; k <= N; ++k) r[(u64)k * rpitch + i] = v[k];
 }
 
-// Waves of the diagonal-block workgroup: 12 up to 12 limbs (three waves per SIMD leave 170 VGPRs a
-// lane, enough for the exact dot there), 8 above; fewer when their LDS rows would pass 144 KB.
+// Waves of the diagonal-block workgroup: 12 up to 4 limbs (three waves per SIMD leave 168 VGPRs a lane,
+// enough there), 8 above (at 168 VGPRs the kernel spilled from 5 limbs on, 350 bytes a lane at 8 limbs;
+// at 256 bits 8 waves without those spills ran the factorization's blocks 5% faster than 12, with four
+// numerator waves fewer); fewer when their LDS rows would pass 144 KB.
 template <int N> constexpr int diag_waves()
 {
-    constexpr int by_regs = N <= 12 ? 12 : 8;
+    constexpr int by_regs = N <= 4 ? 12 : 8;
     constexpr int by_lds = (int)(147456 / sizeof(WaveLds<N>));
     return by_lds < 1 ? 1 : (by_lds < by_regs ? by_lds : by_regs);
 }
