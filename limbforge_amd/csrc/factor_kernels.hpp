@@ -673,6 +673,113 @@ template <int N> __global__ void __launch_bounds__(64) solve_backward_kernel(Sol
     }
 }
 
+// ---- in-block solves with the block staged in LDS
+// The in-block solves run one wavefront per right-hand side, a few waves on the whole GPU, so each row's
+// global loads (its terms, b_i, the reciprocal, l_ii, and the x_k just written) stand on the chain's
+// critical path.  These forms copy the block's part of L (the lower triangle, packed so that each row's
+// terms are contiguous: by rows for the forward pass, by columns for the backward pass), its x column and
+// its reciprocals into LDS once, then read only LDS in the recurrence; x_i goes to LDS for the later rows
+// and to global memory as the result.  Same arithmetic and order as the kernels above.
+constexpr u32 STAGED_ROWS = 32; // largest block the staged forms take
+
+// LDS words of the staged block of B rows: the packed triangle (N+3 planes of B(B+1)/2 entries), the x
+// column (N+3 planes of B entries) and the reciprocals (N+1 planes of B entries).
+template <int N> __host__ __device__ constexpr u32 staged_tri_pitch(u32 B) { return B * (B + 1) / 2; }
+template <int N> __host__ __device__ constexpr u64 staged_words(u32 B) { return (u64)(N + 3) * staged_tri_pitch<N>(B) + (u64)(N + 3) * B + (u64)(N + 1) * B; }
+
+// first index of column i of a column-packed lower triangle of B rows: sum over c < i of (B - c)
+__device__ __forceinline__ u32 tri_col_start(u32 B, u32 i) { return i * B - i * (i - 1) / 2; }
+
+template <int N> __device__ __forceinline__ void stage_copy(u32* dst, u64 dpitch, u64 di, const u32* src, u64 spitch, u64 si)
+{
+#pragma unroll
+    for (int w = 0; w < N + 3; ++w) dst[(u64)w * dpitch + di] = src[(u64)w * spitch + si];
+}
+
+// Copies the block's L triangle (by rows or by columns), x column and reciprocals into LDS.
+template <int N>
+__device__ __forceinline__ void stage_solve_block(const SolveArgs& a, u32 c, bool by_columns, u32* tri, u32* xs, u32* rs, u32 lane)
+{
+    const u32 B = a.r1 - a.r0, TP = staged_tri_pitch<N>(B);
+    for (u32 i = 0; i < B; ++i) {
+        if (by_columns) // column i: rows k = i .. B-1
+            for (u32 k = i + lane; k < B; k += WAVE)
+                stage_copy<N>(tri, TP, tri_col_start(B, i) + (k - i), a.f, a.fpitch, (u64)(a.r0 + k) * a.n + (a.r0 + i));
+        else // row i: columns k = 0 .. i
+            for (u32 k = lane; k <= i; k += WAVE)
+                stage_copy<N>(tri, TP, i * (i + 1) / 2 + k, a.f, a.fpitch, (u64)(a.r0 + i) * a.n + (a.r0 + k));
+    }
+    for (u32 i = lane; i < B; i += WAVE) {
+        stage_copy<N>(xs, B, i, a.x, a.xpitch, (u64)(a.r0 + i) * a.nrhs + c);
+#pragma unroll
+        for (int w = 0; w <= N; ++w) rs[(u64)w * B + i] = a.recip[(u64)w * a.rpitch + a.r0 + i];
+    }
+}
+
+template <int N> __device__ __forceinline__ void load_recip_lds(const u32* rs, u32 B, u32 i, u32 (&v)[N + 1])
+{
+#pragma unroll
+    for (int k = 0; k <= N; ++k) v[k] = rs[(u64)k * B + i];
+}
+
+// Forward block [r0, r1) of L X = B, r1 - r0 <= STAGED_ROWS, block staged in LDS (dynamic shared memory of
+// staged_words<N>(r1 - r0) words); one wavefront per right-hand side.
+template <int N> __global__ void __launch_bounds__(64) solve_forward_staged_kernel(SolveArgs a)
+{
+    __shared__ WaveLds<N> L;
+    extern __shared__ u32 stage[];
+    const u32 c = blockIdx.x, lane = threadIdx.x, B = a.r1 - a.r0, TP = staged_tri_pitch<N>(B);
+    u32* tri = stage;
+    u32* xs = tri + (u64)(N + 3) * TP;
+    u32* rs = xs + (u64)(N + 3) * B;
+    stage_solve_block<N>(a, c, false, tri, xs, rs, lane);
+    __syncthreads();
+    const DotScratch sc{a.scratch + (u64)c * 2 * a.swords, a.swords, a.error};
+    const VecView xv{xs, B, 0, 1};
+    for (u32 i = 0; i < B; ++i) {
+        const VecView li{tri, TP, (i64)(i * (i + 1) / 2), 1};
+        const Num<N> v = wave_dot_sub<N>(load_out<N>(xs, B, i), li, xv, i, lane, L, sc);
+        if (lane == 0) {
+            u32 rv[N + 1];
+            load_recip_lds<N>(rs, B, i, rv);
+            const Num<N> x = div_by_reciprocal<N>(v, load_out<N>(tri, TP, i * (i + 1) / 2 + i), rv);
+            store_out<N>(xs, B, 1, i, 0, false, x);
+            store_out<N>(a.x, a.xpitch, a.nrhs, a.r0 + i, c, false, x);
+        }
+        __syncthreads(); // x_i is a term of the next row's dot
+    }
+}
+
+// Backward block [r0, r1) of L^T X = B, staged as the forward form (the triangle packed by columns).
+template <int N> __global__ void __launch_bounds__(64) solve_backward_staged_kernel(SolveArgs a)
+{
+    __shared__ WaveLds<N> L;
+    extern __shared__ u32 stage[];
+    const u32 c = blockIdx.x, lane = threadIdx.x, B = a.r1 - a.r0, TP = staged_tri_pitch<N>(B);
+    u32* tri = stage;
+    u32* xs = tri + (u64)(N + 3) * TP;
+    u32* rs = xs + (u64)(N + 3) * B;
+    stage_solve_block<N>(a, c, true, tri, xs, rs, lane);
+    __syncthreads();
+    const DotScratch sc{a.scratch + (u64)c * 2 * a.swords, a.swords, a.error};
+    for (u32 i = B; i-- > 0;) {
+        Num<N> v = load_out<N>(xs, B, i);
+        if (i + 1 < B) {
+            const VecView col{tri, TP, (i64)tri_col_start(B, i) + 1, 1};
+            const VecView xv{xs, B, (i64)i + 1, 1};
+            v = wave_dot_sub<N>(v, col, xv, B - i - 1, lane, L, sc);
+        }
+        if (lane == 0) {
+            u32 rv[N + 1];
+            load_recip_lds<N>(rs, B, i, rv);
+            const Num<N> x = div_by_reciprocal<N>(v, load_out<N>(tri, TP, tri_col_start(B, i)), rv);
+            store_out<N>(xs, B, 1, i, 0, false, x);
+            store_out<N>(a.x, a.xpitch, a.nrhs, a.r0 + i, c, false, x);
+        }
+        __syncthreads();
+    }
+}
+
 // Off-block update of a solve by the block [r0, r1), for updates too small to pay for the update GEMM's
 // planning: x_ic = D(x_ic; l, x_c; [r0, r1)) with l = row i of L (forward, rows i in [r1, n)) or column i of L
 // (backward, rows i in [0, r0)), one wavefront per entry, grid-stride over the rows x nrhs entries; wavefront

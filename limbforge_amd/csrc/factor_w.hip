@@ -53,6 +53,15 @@ template <int N> hipError_t launch_solve(int backward, const SolveArgs& a, hipSt
 {
     if (!a.nrhs || a.r1 <= a.r0) return hipSuccess;
     const dim3 grid(a.nrhs), block(WAVE);
+    // blocks of up to STAGED_ROWS rows whose staged copy fits 64 KB of LDS beside the exact-dot rows run
+    // from LDS; larger blocks and wider values read global memory
+    const uint32_t B = a.r1 - a.r0;
+    const uint64_t dyn = staged_words<N>(B) * 4;
+    if (B <= STAGED_ROWS && dyn + sizeof(WaveLds<N>) <= (uint64_t(64) << 10)) {
+        if (backward) hipLaunchKernelGGL((solve_backward_staged_kernel<N>), grid, block, (unsigned)dyn, s, a);
+        else hipLaunchKernelGGL((solve_forward_staged_kernel<N>), grid, block, (unsigned)dyn, s, a);
+        return hipGetLastError();
+    }
     if (backward) hipLaunchKernelGGL((solve_backward_kernel<N>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((solve_forward_kernel<N>), grid, block, 0, s, a);
     return hipGetLastError();
