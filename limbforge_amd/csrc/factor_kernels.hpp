@@ -235,6 +235,23 @@ __device__ __forceinline__ int wave_or(int v)
     return v;
 }
 
+// Maximum (MAX) or minimum of a 32-bit value over the 64 lanes of the wave, in every lane: DPP row shifts by
+// 1, 2, 4 and 8 leave each row's reduction in its lane 15, the row broadcasts of lanes 15 and 31 carry it to
+// lane 63, which is then read (ALU latency per step, where a cross-lane shuffle waits on LDS).  Every lane
+// of the wave must be active.
+template <bool MAX> __device__ __forceinline__ int wave_extreme32(int v)
+{
+    constexpr int ID = MAX ? (int)0x80000000 : 0x7fffffff;
+    auto pick = [](int a, int b) { return MAX ? (a > b ? a : b) : (a < b ? a : b); };
+    v = pick(v, __builtin_amdgcn_update_dpp(ID, v, 0x111, 0xF, 0xF, false)); // row_shr:1
+    v = pick(v, __builtin_amdgcn_update_dpp(ID, v, 0x112, 0xF, 0xF, false)); // row_shr:2
+    v = pick(v, __builtin_amdgcn_update_dpp(ID, v, 0x114, 0xF, 0xF, false)); // row_shr:4
+    v = pick(v, __builtin_amdgcn_update_dpp(ID, v, 0x118, 0xF, 0xF, false)); // row_shr:8
+    v = pick(v, __builtin_amdgcn_update_dpp(ID, v, 0x142, 0xA, 0xF, false)); // row_bcast:15 into rows 1, 3
+    v = pick(v, __builtin_amdgcn_update_dpp(ID, v, 0x143, 0xC, 0xF, false)); // row_bcast:31 into rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
 // Row r = sg * src (S words) * 2^rel in the (cleared) rows of a wave, as the S+1 words of the shifted term
 // at word offset o = rel / 32, two's complement within those words for a negative term; the rest of a
 // negative term's W-word two's complement (ones from word o+S+1 up) is left out and counted in neg[o+S+1]
@@ -397,9 +414,13 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
     constexpr int CPL = (W + WAVE - 1) / WAVE; // window columns per lane
     constexpr i64 NONE = -((i64)1 << 62);
     // pass 1: statuses, scale range and count of the nonzero products.  With one chunk of terms (K <= 32)
-    // each lane loads its term once, here, and keeps it for pass 2.
+    // each lane loads its term once, here, and forms its exact product at once (it depends only on the
+    // loads, so it overlaps the cross-lane reductions that follow); the status OR and the count come from
+    // ballots and the scale range from DPP reductions over 32-bit scales (valid exponents keep them in
+    // range; a lane whose scale is not takes the 64-bit shuffle reductions).
     const bool one_chunk = K <= (u32)DOT_ROWS;
     Num<N> ta = zero<N>(ST_OK), tb = zero<N>(ST_OK);
+    u32 pt[2 * N];
     u32 st = 0;
     i64 hi = NONE, lo = -NONE;
     int count = 0;
@@ -407,13 +428,21 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
         if (lane < K) {
             ta = vec_load<N>(x, lane);
             tb = vec_load<N>(y, lane);
-            st = ta.st | tb.st;
-            if (ta.s && tb.s) {
-                hi = lo = (i64)ta.e + tb.e - 2 * (32 * N - 1);
-                count = 1;
-            }
         }
-    } else
+        mantissa_product<N>(ta.w, tb.w, pt);
+        const u32 stl = ta.st | tb.st;
+        const bool nz = ta.s && tb.s;
+        const i64 s64 = nz ? (i64)ta.e + tb.e - 2 * (32 * N - 1) : 0;
+        st = __ballot(stl != 0) ? (u32)wave_or((int)stl) : 0u;
+        count = __popcll(__ballot(nz));
+        if (__ballot(nz && (s64 < -((i64)1 << 30) || s64 > ((i64)1 << 30)))) {
+            hi = wave_max(nz ? s64 : NONE);
+            lo = wave_min(nz ? s64 : -NONE);
+        } else if (count) {
+            hi = wave_extreme32<true>(nz ? (int)s64 : (int)0x80000000);
+            lo = wave_extreme32<false>(nz ? (int)s64 : 0x7fffffff);
+        }
+    } else {
         for (u32 k = lane; k < K; k += WAVE) {
             int ex, sx, ey, sy;
             u32 tx, ty;
@@ -427,12 +456,14 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
                 ++count;
             }
         }
-    st = (u32)wave_or((int)st) | c.st;
+        st = (u32)wave_or((int)st);
+        count = wave_sum(count);
+        hi = wave_max(hi);
+        lo = wave_min(lo);
+    }
+    st |= c.st;
     if (st) return zero<N>(st);
-    count = wave_sum(count);
     if (!count) return c;
-    hi = wave_max(hi);
-    lo = wave_min(lo);
     if (c.s) {
         const i64 s = (i64)c.e - (32 * N - 1);
         hi = s > hi ? s : hi;
@@ -459,11 +490,14 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
         clear_rows<N>(L, lane);
         wave_sync();
         if (lane < rows) {
-            const u32 k = kb + lane;
-            const Num<N> a = one_chunk ? ta : vec_load<N>(x, k), b = one_chunk ? tb : vec_load<N>(y, k);
-            u32 p[2 * N];
-            mantissa_product<N>(a.w, b.w, p);
-            put_row<N, 2 * N>(L, (int)lane, p, (a.s && b.s) ? (i64)a.e + b.e - 2 * (32 * N - 1) - lo : 0, -a.s * b.s);
+            if (one_chunk) put_row<N, 2 * N>(L, (int)lane, pt, (ta.s && tb.s) ? (i64)ta.e + tb.e - 2 * (32 * N - 1) - lo : 0, -ta.s * tb.s);
+            else {
+                const u32 k = kb + lane;
+                const Num<N> a = vec_load<N>(x, k), b = vec_load<N>(y, k);
+                u32 p[2 * N];
+                mantissa_product<N>(a.w, b.w, p);
+                put_row<N, 2 * N>(L, (int)lane, p, (a.s && b.s) ? (i64)a.e + b.e - 2 * (32 * N - 1) - lo : 0, -a.s * b.s);
+            }
         }
         const bool with_c = kb == 0 && c.s != 0;
         if (with_c && lane == WAVE - 1) put_row<N, N>(L, DOT_ROWS, c.w, (i64)c.e - (32 * N - 1) - lo, c.s);
