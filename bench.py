@@ -602,6 +602,9 @@ def cholesky_section(oracle_check: bool):
                                "sample": "the whole workload: the oracle's restatement of the reference's fixed-order host path "
                                          "(panel rows, trailing-update rows and right-hand sides on the host threads)"}
         log(f"[bench] cholesky oracle check: {res['oracle_check']}, {res['cpu_baseline']}")
+    # the per-row steps the factorization and the solves chain, timed on one wavefront (lf_factor_step_cycles)
+    res["step_cycles_one_wave"] = capi.factor_step_cycles(bits, 32)
+    log(f"[bench] cholesky per-row step cycles: {res['step_cycles_one_wave']}")
     res["workload"] = ("256-bit SYRK (A 2048 x 512, single band) + blocked Cholesky n = 512 (block 32) + cholesky_solve "
                        "with 64 right-hand sides (config 4)")
     return res

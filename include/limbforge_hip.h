@@ -326,6 +326,11 @@ int lf_calibrate_mad64(uint64_t blocks, int iters, double* products_per_second);
  * bytes written.  Profilers' FETCH_SIZE / WRITE_SIZE of this kernel against its known size scale those
  * counters for the traffic of the other kernels. */
 int lf_calibrate_copy(uint64_t bytes, double* bytes_per_second);
+/* Cycles (s_memtime ticks) of the factorization's per-row steps at `bits` on one wavefront with the GPU
+ * otherwise idle, each averaged over `reps` dependent repetitions: cycles[0] an exact dot of 16 terms,
+ * [1] of 31 terms, [2] a division by a precomputed reciprocal, [3] a Knuth division, [4] a square root,
+ * [5] a reciprocal (synthetic normalized operands). */
+int lf_factor_step_cycles(int bits, int reps, uint64_t* cycles);
 
 #ifdef __cplusplus
 }

@@ -86,6 +86,7 @@ def lib() -> C.CDLL:
         "lf_autotune_chain": (i32, [i32, u64, i32, P(C.c_uint), P(C.c_float)]),
         "lf_calibrate_mad64": (i32, [u64, i32, P(C.c_double)]),
         "lf_calibrate_copy": (i32, [u64, P(C.c_double)]),
+        "lf_factor_step_cycles": (i32, [i32, i32, P(C.c_uint64)]),
         "lf_cast": (i32, [i32, i32, i32, P(Operand), P(Operand), u64, vp]),
         "lf_cast_host": (i32, [i32, i32, i32, vp, vp, u64, P(C.c_double), P(C.c_double)]),
         "lf_run_host": (i32, [i32, i32, vp, vp, vp, vp, u64, P(Broadcast), P(Broadcast), P(C.c_double), P(C.c_double)]),
@@ -232,6 +233,14 @@ def autotune_chain(bits: int, count: int, steps: int = 64):
     ms = (C.c_float * 3)()
     check(lib().lf_autotune_chain(bits, int(count), int(steps), C.byref(lanes), ms))
     return lanes.value, list(ms)
+
+
+def factor_step_cycles(bits: int, reps: int = 32) -> dict:
+    """lf_factor_step_cycles: s_memtime cycles of the factorization's per-row steps on one wavefront."""
+    out = (C.c_uint64 * 6)()
+    check(lib().lf_factor_step_cycles(bits, int(reps), out))
+    keys = ("exact_dot_16_terms", "exact_dot_31_terms", "div_by_reciprocal", "knuth_div", "sqrt", "reciprocal")
+    return dict(zip(keys, [int(v) for v in out]))
 
 
 def broadcast_elements(count: int, index) -> int:

@@ -70,6 +70,7 @@ typedef hipError_t (*invalid_fn)(uint32_t*, uint64_t, uint32_t, uint32_t, hipStr
 typedef hipError_t (*solve_fn)(int, const lf::SolveArgs&, hipStream_t);
 typedef hipError_t (*solve_update_fn)(int, const lf::SolveArgs&, uint32_t, hipStream_t);
 typedef hipError_t (*recip_fn)(const uint32_t*, uint64_t, uint32_t, uint32_t*, uint64_t, hipStream_t);
+typedef hipError_t (*steps_fn)(const uint32_t*, uint64_t, uint32_t*, uint64_t, int32_t*, int, uint64_t*, uint32_t*, hipStream_t);
 struct FactorTables;
 template <int N> void fill_factor(FactorTables& t);
 struct FactorTables {
@@ -81,6 +82,7 @@ struct FactorTables {
     solve_fn solve[33] = {};
     solve_update_fn solve_update[33] = {};
     recip_fn reciprocals[33] = {};
+    steps_fn steps[33] = {};
     FactorTables() { fill_factor<2>(*this); }
 };
 template <int N> void fill_factor(FactorTables& t)
@@ -93,6 +95,7 @@ template <int N> void fill_factor(FactorTables& t)
     t.solve[N] = lf::launch_solve<N>;
     t.solve_update[N] = lf::launch_solve_update<N>;
     t.reciprocals[N] = lf::launch_reciprocals<N>;
+    t.steps[N] = lf::launch_factor_steps<N>;
     if constexpr (N < 32) fill_factor<N + 1>(t);
 }
 const FactorTables& tables()
@@ -534,6 +537,46 @@ int lf_solve_host(int bits, int passes, const void* L, uint64_t n, const void* B
     FACTOR_HIP(hipStreamSynchronize(st.s), "device execution (solve)");
     if (gpu_seconds) *gpu_seconds = g;
     if (wall_seconds) *wall_seconds = since(start);
+    return LF_OK;
+}
+
+int lf_factor_step_cycles(int bits, int reps, uint64_t* cycles)
+{
+    if (bits < 64 || bits > 1024 || bits % 32) return fail(LF_INVALID_ARGUMENT, "bits must be a multiple of 32 in [64,1024]");
+    if (reps < 1 || reps > 1024) return fail(LF_INVALID_ARGUMENT, "reps must be in [1, 1024]");
+    if (!cycles) return fail(LF_INVALID_ARGUMENT, "null pointer");
+    const int N = bits / 32;
+    constexpr uint64_t pitch = 128; // 67 values: terms 0-30 and 32-62, addend 64, divisor 65, dividend 66
+    std::vector<uint32_t> h((N + 3) * pitch, 0);
+    uint64_t state = 0x9E3779B97F4A7C15ull + (uint64_t)bits;
+    auto next = [&] {
+        state ^= state << 13;
+        state ^= state >> 7;
+        state ^= state << 17;
+        return state;
+    };
+    for (uint64_t i = 0; i < 67; ++i) {
+        for (int k = 0; k < N; ++k) h[k * pitch + i] = (uint32_t)next();
+        h[(N - 1) * pitch + i] |= 0x80000000u;
+        h[N * pitch + i] = (uint32_t)((int)(next() % 41) - 20);
+        h[(N + 1) * pitch + i] = (i >= 64 || (next() & 1)) ? 1u : (uint32_t)-1;
+    }
+    const uint64_t swords = lf::factor_scratch_words(N, 32);
+    DevBuf dv, ds, derr, dout, dsink;
+    FACTOR_HIP(dv.alloc(h.size() * 4), "hipMalloc");
+    FACTOR_HIP(ds.alloc(2 * swords * 4), "hipMalloc");
+    FACTOR_HIP(derr.alloc(4), "hipMalloc");
+    FACTOR_HIP(dout.alloc(6 * 8), "hipMalloc");
+    FACTOR_HIP(dsink.alloc(64 * 4), "hipMalloc");
+    OwnedStream st;
+    FACTOR_HIP(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking), "hipStreamCreate");
+    FACTOR_HIP(hipMemcpyAsync(dv.p, h.data(), h.size() * 4, hipMemcpyHostToDevice, st.s), "upload");
+    FACTOR_HIP(hipMemsetAsync(derr.p, 0, 4, st.s), "hipMemsetAsync");
+    FACTOR_HIP(tables().steps[N]((const uint32_t*)dv.p, pitch, (uint32_t*)ds.p, swords, (int32_t*)derr.p, reps, (uint64_t*)dout.p,
+                                 (uint32_t*)dsink.p, st.s),
+               "step timing launch");
+    FACTOR_HIP(hipMemcpyAsync(cycles, dout.p, 6 * 8, hipMemcpyDeviceToHost, st.s), "download");
+    FACTOR_HIP(hipStreamSynchronize(st.s), "step timing");
     return LF_OK;
 }
 

@@ -464,15 +464,20 @@ __device__ Num<N> wave_dot_sub(const Num<N>& c, const VecView& x, const VecView&
     st |= c.st;
     if (st) return zero<N>(st);
     if (!count) return c;
+    // the window spans from the lowest term's last bit to the highest top bit: a product's 64N bits above
+    // its scale, c's 32N above its own (c, a 32N-bit value, sits about 32N bits above products of its
+    // magnitude, so charging it a product's width would send ordinary dots past about 21 limbs to the
+    // cluster path)
+    i64 top = hi + 64 * N;
     if (c.s) {
         const i64 s = (i64)c.e - (32 * N - 1);
-        hi = s > hi ? s : hi;
+        top = s + 32 * N > top ? s + 32 * N : top;
         lo = s < lo ? s : lo;
         ++count;
     }
     int logc = 0;
     while ((1 << logc) < count) ++logc;
-    if (hi - lo + 64 * N + logc + 1 > 32 * W) {
+    if (top - lo + logc + 1 > 32 * W) {
         Num<N> r = zero<N>(ST_OK);
         if (lane == 0) r = dot_sub_clusters<N>(c, x, y, K, sc);
         return r;
@@ -832,6 +837,70 @@ template <int N> __global__ void __launch_bounds__(64) solve_update_kernel(Solve
         const Num<N> v = wave_dot_sub<N>(load_out<N>(a.x, a.xpitch, (u64)i * a.nrhs + c), l, xc, a.r1 - a.r0, lane, L, sc);
         if (lane == 0) store_out<N>(a.x, a.xpitch, a.nrhs, i, c, false, v);
     }
+}
+
+// Measurement support: cycles (s_memtime) of the factorization's per-row steps on one wavefront alone,
+// each the average over `reps` dependent repetitions: out[0] an exact dot of 16 terms, out[1] of 31 terms,
+// out[2] a division by a reciprocal, out[3] a Knuth division, out[4] a square root, out[5] a reciprocal.
+// v holds 67 normalized values in N+3 planes of pitch `pitch` (terms 0-30 and 32-62, addend 64, divisor 65,
+// dividend 66); sink keeps the results live.
+template <int N>
+__global__ void __launch_bounds__(64) factor_steps_kernel(const u32* v, u64 pitch, u32* scratch, u64 swords, int32_t* err, int reps, u64* out,
+                                                         u32* sink)
+{
+    __shared__ WaveLds<N> L;
+    const u32 lane = threadIdx.x;
+    const DotScratch sc{scratch, swords, err};
+    const VecView x{v, pitch, 0, 1}, y{v, pitch, 32, 1};
+    Num<N> acc = load_out<N>(v, pitch, 64);
+    u32 chk = 0;
+    for (int t = 0; t < 2; ++t) {
+        const u32 K = t ? 31u : 16u;
+        const u64 t0 = __builtin_amdgcn_s_memtime();
+        for (int r = 0; r < reps; ++r) {
+            const Num<N> d = wave_dot_sub<N>(acc, x, y, K, lane, L, sc);
+            acc.w[0] ^= d.w[0] & 1u; // each dot depends on the one before
+            chk += d.w[N - 1];
+        }
+        const u64 t1 = __builtin_amdgcn_s_memtime();
+        if (lane == 0) out[t] = (t1 - t0) / (u64)reps;
+    }
+    if (lane == 0) {
+        const Num<N> d = load_out<N>(v, pitch, 65);
+        Num<N> a = load_out<N>(v, pitch, 66);
+        a.s = 1;
+        u32 rv[N + 1];
+        reciprocal<N>(d.w, rv);
+        u64 t[5];
+        t[0] = __builtin_amdgcn_s_memtime();
+        for (int r = 0; r < reps; ++r) {
+            const Num<N> q = div_by_reciprocal<N>(a, d, rv);
+            a.w[0] ^= q.w[0] & 1u;
+            chk += q.w[N - 1];
+        }
+        t[1] = __builtin_amdgcn_s_memtime();
+        for (int r = 0; r < reps; ++r) {
+            const Num<N> q = div<N>(a, d);
+            a.w[0] ^= q.w[0] & 1u;
+            chk += q.w[N - 1];
+        }
+        t[2] = __builtin_amdgcn_s_memtime();
+        for (int r = 0; r < reps; ++r) {
+            const Num<N> q = sqrt<N>(a);
+            a.w[0] ^= q.w[0] & 1u;
+            chk += q.w[N - 1];
+        }
+        t[3] = __builtin_amdgcn_s_memtime();
+        for (int r = 0; r < reps; ++r) {
+            u32 w[N + 1];
+            reciprocal<N>(a.w, w);
+            a.w[0] ^= w[0] & 1u;
+            chk += w[N];
+        }
+        t[4] = __builtin_amdgcn_s_memtime();
+        for (int k = 0; k < 4; ++k) out[2 + k] = (t[k + 1] - t[k]) / (u64)reps;
+    }
+    sink[lane] = chk;
 }
 
 // Reciprocals of the n diagonal entries of the factor, one thread each (for the solves' divisions).

@@ -52,6 +52,10 @@ inline uint64_t factor_scratch_words(int N, uint64_t K)
 }
 
 template <int N> hipError_t launch_chol_diag(const FactorArgs& f, uint32_t threads, hipStream_t s);
+// one wavefront timing the per-row steps (factor_steps_kernel)
+template <int N>
+hipError_t launch_factor_steps(const uint32_t* v, uint64_t pitch, uint32_t* scratch, uint64_t swords, int32_t* err, int reps, uint64_t* out,
+                               uint32_t* sink, hipStream_t s);
 template <int N> hipError_t launch_chol_panel(const FactorArgs& f, hipStream_t s);
 // entries (i, j), j <= i, of columns [k1, k2) and rows [k1, n) updated by the block [k0, k1); at most
 // `waves` wavefronts (scratch slots)

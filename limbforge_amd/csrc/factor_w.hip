@@ -84,7 +84,16 @@ template <int N> hipError_t launch_reciprocals(const uint32_t* w, uint64_t pitch
     return hipGetLastError();
 }
 
+template <int N>
+hipError_t launch_factor_steps(const uint32_t* v, uint64_t pitch, uint32_t* scratch, uint64_t swords, int32_t* err, int reps, uint64_t* out,
+                               uint32_t* sink, hipStream_t s)
+{
+    hipLaunchKernelGGL((factor_steps_kernel<N>), dim3(1), dim3(WAVE), 0, s, v, pitch, scratch, swords, err, reps, out, sink);
+    return hipGetLastError();
+}
+
 template hipError_t launch_chol_diag<LF_WORDS>(const FactorArgs&, uint32_t, hipStream_t);
+template hipError_t launch_factor_steps<LF_WORDS>(const uint32_t*, uint64_t, uint32_t*, uint64_t, int32_t*, int, uint64_t*, uint32_t*, hipStream_t);
 template hipError_t launch_chol_panel<LF_WORDS>(const FactorArgs&, hipStream_t);
 template hipError_t launch_chol_lookahead<LF_WORDS>(const FactorArgs&, uint32_t, uint32_t, hipStream_t);
 template hipError_t launch_chol_invalid<LF_WORDS>(uint32_t*, uint64_t, uint32_t, uint32_t, hipStream_t);

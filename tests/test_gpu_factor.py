@@ -63,6 +63,17 @@ def test_spd_kinds(capi, oracle, bits, n, kind):
     check_factor(capi, oracle, rng, bits, A, n, kind, runs(n))
 
 
+@pytest.mark.parametrize("bits", [640, 704, 768, 1024])
+@pytest.mark.parametrize("n,kind", [(33, "qsc"), (40, "spread"), (48, "banded")])
+def test_wide_values(capi, oracle, bits, n, kind):
+    """640-1024 bits: the exact dots' window holds products and the addend together at these widths (the
+    addend sits about 32N bits above products of its magnitude), so the register window, not only the
+    cluster path, runs here; spread inputs take both."""
+    rng = np.random.default_rng(bits * 7 + n)
+    A = spd(oracle, rng, bits, n, kind)
+    check_factor(capi, oracle, rng, bits, A, n, f"{kind} {bits}-bit", [("default", 32), ("block 16", 16)])
+
+
 @pytest.mark.parametrize("bits", [256, 224])
 def test_rank_deficient(capi, oracle, bits):
     rng = np.random.default_rng(bits + 33)
