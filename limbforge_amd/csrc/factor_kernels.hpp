@@ -235,6 +235,10 @@ __device__ __forceinline__ int wave_or(int v)
     return v;
 }
 
+// The value of lane l - 1 in lane l (0 in lane 0): a DPP wave shift right by one lane (wave_shr:1), at ALU
+// latency where a shuffle goes through LDS.  Every lane of the wave must be active.
+__device__ __forceinline__ u32 wave_from_previous_lane(u32 v) { return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false); }
+
 // Maximum (MAX) or minimum of a 32-bit value over the 64 lanes of the wave, in every lane: DPP row shifts by
 // 1, 2, 4 and 8 leave each row's reduction in its lane 15, the row broadcasts of lanes 15 and 31 carry it to
 // lane 63, which is then read (ALU latency per step, where a cross-lane shuffle waits on LDS).  Every lane
@@ -332,7 +336,7 @@ __device__ __forceinline__ Num<N> wave_round(const u64 (&S)[(factor_window(N) + 
         for (int t = 0; t < CPL; ++t) {
             const bool in = (int)lane + WAVE * t < W;
             const u32 h = (u32)(S[t] >> 32); // at most the row count: a single carry bit into the next word
-            u32 hprev = __shfl_up(h, 1u);
+            u32 hprev = wave_from_previous_lane(h);
             hprev = lane == 0 ? hlast : hprev;
             hlast = __builtin_amdgcn_readlane(h, WAVE - 1);
             const u64 ts = (u64)(u32)S[t] + hprev;
@@ -382,7 +386,7 @@ __device__ __forceinline__ Num<N> wave_round(const u64 (&S)[(factor_window(N) + 
         u32 last = 0;
 #pragma unroll
         for (int t = 0; t < CPL; ++t) {
-            u32 prev = __shfl_up(word[t], 1u);
+            u32 prev = wave_from_previous_lane(word[t]);
             prev = lane == 0 ? last : prev;
             last = __builtin_amdgcn_readlane(word[t], WAVE - 1);
             nw[t] = lz ? (word[t] << lz) | (prev >> (32 - lz)) : word[t];
