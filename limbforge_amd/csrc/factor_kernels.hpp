@@ -843,66 +843,57 @@ template <int N> __global__ void __launch_bounds__(64) solve_update_kernel(Solve
     }
 }
 
-// Measurement support: cycles (s_memtime) of the factorization's per-row steps on one wavefront alone,
-// each the average over `reps` dependent repetitions: out[0] an exact dot of 16 terms, out[1] of 31 terms,
-// out[2] a division by a reciprocal, out[3] a Knuth division, out[4] a square root, out[5] a reciprocal.
-// v holds 67 normalized values in N+3 planes of pitch `pitch` (terms 0-30 and 32-62, addend 64, divisor 65,
-// dividend 66); sink keeps the results live.
-template <int N>
+// Measurement support: cycles (s_memtime) of one of the factorization's per-row steps on one wavefront
+// alone, averaged over `reps` dependent repetitions, one launch per step so that each kernel holds only
+// the code it times: STEP 0 exact dots of 16 then 31 terms (out[0], out[1]), 1 a division by a reciprocal
+// (out[2]), 2 a Knuth division (out[3]), 3 a square root (out[4]), 4 a reciprocal (out[5]).  v holds 67
+// normalized values in N+3 planes of pitch `pitch` (terms 0-30 and 32-62, addend 64, divisor 65, dividend
+// 66); sink keeps the results live.
+template <int N, int STEP>
 __global__ void __launch_bounds__(64) factor_steps_kernel(const u32* v, u64 pitch, u32* scratch, u64 swords, int32_t* err, int reps, u64* out,
                                                          u32* sink)
 {
     __shared__ WaveLds<N> L;
     const u32 lane = threadIdx.x;
-    const DotScratch sc{scratch, swords, err};
-    const VecView x{v, pitch, 0, 1}, y{v, pitch, 32, 1};
-    Num<N> acc = load_out<N>(v, pitch, 64);
     u32 chk = 0;
-    for (int t = 0; t < 2; ++t) {
-        const u32 K = t ? 31u : 16u;
-        const u64 t0 = __builtin_amdgcn_s_memtime();
-        for (int r = 0; r < reps; ++r) {
-            const Num<N> d = wave_dot_sub<N>(acc, x, y, K, lane, L, sc);
-            acc.w[0] ^= d.w[0] & 1u; // each dot depends on the one before
-            chk += d.w[N - 1];
+    if constexpr (STEP == 0) {
+        const DotScratch sc{scratch, swords, err};
+        const VecView x{v, pitch, 0, 1}, y{v, pitch, 32, 1};
+        Num<N> acc = load_out<N>(v, pitch, 64);
+        for (int t = 0; t < 2; ++t) {
+            const u32 K = t ? 31u : 16u;
+            const u64 t0 = __builtin_amdgcn_s_memtime();
+            for (int r = 0; r < reps; ++r) {
+                const Num<N> d = wave_dot_sub<N>(acc, x, y, K, lane, L, sc);
+                acc.w[0] ^= d.w[0] & 1u; // each dot depends on the one before
+                chk += d.w[N - 1];
+            }
+            const u64 t1 = __builtin_amdgcn_s_memtime();
+            if (lane == 0) out[t] = (t1 - t0) / (u64)reps;
         }
-        const u64 t1 = __builtin_amdgcn_s_memtime();
-        if (lane == 0) out[t] = (t1 - t0) / (u64)reps;
-    }
-    if (lane == 0) {
+    } else if (lane == 0) {
         const Num<N> d = load_out<N>(v, pitch, 65);
         Num<N> a = load_out<N>(v, pitch, 66);
         a.s = 1;
         u32 rv[N + 1];
-        reciprocal<N>(d.w, rv);
-        u64 t[5];
-        t[0] = __builtin_amdgcn_s_memtime();
+        if constexpr (STEP == 1) reciprocal<N>(d.w, rv);
+        const u64 t0 = __builtin_amdgcn_s_memtime();
         for (int r = 0; r < reps; ++r) {
-            const Num<N> q = div_by_reciprocal<N>(a, d, rv);
-            a.w[0] ^= q.w[0] & 1u;
-            chk += q.w[N - 1];
+            if constexpr (STEP == 4) {
+                u32 w[N + 1];
+                reciprocal<N>(a.w, w);
+#pragma unroll
+                for (int k = 0; k < N; ++k) a.w[k] ^= w[k] & 1u; // every word of the reciprocal feeds the next
+                chk += w[0];
+            } else {
+                const Num<N> q = STEP == 1 ? div_by_reciprocal<N>(a, d, rv) : STEP == 2 ? div<N>(a, d) : sqrt<N>(a);
+#pragma unroll
+                for (int k = 0; k < N - 1; ++k) a.w[k] ^= q.w[k] & 1u;
+                chk += q.w[N - 1];
+            }
         }
-        t[1] = __builtin_amdgcn_s_memtime();
-        for (int r = 0; r < reps; ++r) {
-            const Num<N> q = div<N>(a, d);
-            a.w[0] ^= q.w[0] & 1u;
-            chk += q.w[N - 1];
-        }
-        t[2] = __builtin_amdgcn_s_memtime();
-        for (int r = 0; r < reps; ++r) {
-            const Num<N> q = sqrt<N>(a);
-            a.w[0] ^= q.w[0] & 1u;
-            chk += q.w[N - 1];
-        }
-        t[3] = __builtin_amdgcn_s_memtime();
-        for (int r = 0; r < reps; ++r) {
-            u32 w[N + 1];
-            reciprocal<N>(a.w, w);
-            a.w[0] ^= w[0] & 1u;
-            chk += w[N];
-        }
-        t[4] = __builtin_amdgcn_s_memtime();
-        for (int k = 0; k < 4; ++k) out[2 + k] = (t[k + 1] - t[k]) / (u64)reps;
+        const u64 t1 = __builtin_amdgcn_s_memtime();
+        out[STEP + 1] = (t1 - t0) / (u64)reps;
     }
     sink[lane] = chk;
 }

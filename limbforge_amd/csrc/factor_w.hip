@@ -88,7 +88,11 @@ template <int N>
 hipError_t launch_factor_steps(const uint32_t* v, uint64_t pitch, uint32_t* scratch, uint64_t swords, int32_t* err, int reps, uint64_t* out,
                                uint32_t* sink, hipStream_t s)
 {
-    hipLaunchKernelGGL((factor_steps_kernel<N>), dim3(1), dim3(WAVE), 0, s, v, pitch, scratch, swords, err, reps, out, sink);
+    hipLaunchKernelGGL((factor_steps_kernel<N, 0>), dim3(1), dim3(WAVE), 0, s, v, pitch, scratch, swords, err, reps, out, sink);
+    hipLaunchKernelGGL((factor_steps_kernel<N, 1>), dim3(1), dim3(WAVE), 0, s, v, pitch, scratch, swords, err, reps, out, sink);
+    hipLaunchKernelGGL((factor_steps_kernel<N, 2>), dim3(1), dim3(WAVE), 0, s, v, pitch, scratch, swords, err, reps, out, sink);
+    hipLaunchKernelGGL((factor_steps_kernel<N, 3>), dim3(1), dim3(WAVE), 0, s, v, pitch, scratch, swords, err, reps, out, sink);
+    hipLaunchKernelGGL((factor_steps_kernel<N, 4>), dim3(1), dim3(WAVE), 0, s, v, pitch, scratch, swords, err, reps, out, sink);
     return hipGetLastError();
 }
 

@@ -652,16 +652,17 @@ template <int N> __device__ __forceinline__ u32 knuth_step(u32 (&R)[N + 1], cons
         --q;
         rem += d;
     }
-    u64 borrow = 0;
+    // R -= q B: the product words from a multiply-add chain (each word's high half carried in the 64-bit
+    // accumulator), subtracted by a borrow chain (one v_mad_u64_u32 and one subtract pair per word)
+    u64 p = 0;
+    u32 bw = 0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        u64 p = q * b[i] + borrow;
-        u32 lo = (u32)p, old = R[i];
-        R[i] = old - lo;
-        borrow = (p >> 32) + (old < lo ? 1u : 0u);
+        p = (u64)(u32)q * b[i] + (p >> 32);
+        R[i] = sub_borrow(R[i], (u32)p, bw);
     }
-    const bool under = (u64)R[N] < borrow;
-    R[N] = (u32)((u64)R[N] - borrow);
+    R[N] = sub_borrow(R[N], (u32)(p >> 32), bw);
+    const bool under = bw != 0;
     if (under) {
         --q;
         u32 c = 0;
@@ -734,13 +735,11 @@ template <int N> __device__ __forceinline__ Num<N> div(const Num<N>& a, const Nu
     u32 next = lt ? 0u : (a.w[0] << 31);
     const u32 d = b.w[N - 1];
     const u32 inv = (u32)(~(u64)0 / d - ((u64)1 << 32));
-#pragma unroll 2
-    for (int j = N - 1; j >= 0; --j) {
-        const u32 q = knuth_step<N>(R, b.w, next, inv);
-        next = 0;
+    // fully unrolled: the remainder's word shifts and the quotient words' places are register choices
 #pragma unroll
-        for (int i = N - 1; i > 0; --i) Q[i] = Q[i - 1];
-        Q[0] = q;
+    for (int j = N - 1; j >= 0; --j) {
+        Q[j] = knuth_step<N>(R, b.w, next, inv);
+        next = 0;
     }
     return round_quotient<N>(Q, R, b.w, e, a.s * b.s);
 }
@@ -756,13 +755,8 @@ template <int N> __device__ __forceinline__ void reciprocal(const u32 (&b)[N], u
     R[N] = 0;
     v[N] = 1;
     const u32 inv = (u32)(~(u64)0 / b[N - 1] - ((u64)1 << 32));
-#pragma unroll 2
-    for (int j = N - 1; j >= 0; --j) {
-        const u32 q = knuth_step<N>(R, b, 0xffffffffu, inv);
 #pragma unroll
-        for (int i = 0; i < N; ++i)
-            if (i == j) v[i] = q;
-    }
+    for (int j = N - 1; j >= 0; --j) v[j] = knuth_step<N>(R, b, 0xffffffffu, inv);
 }
 
 // The low P words of the product of two digit strings (29-bit digits, DA and DB of them): columns below

@@ -350,6 +350,39 @@ template <int N> static void check_coop(uint64_t& state, int cases)
     if constexpr (N < 32) check_coop<N + 1>(state, cases);
 }
 
+// Divisions with words drawn from {0, 1, 2, 2^31 - 1, 2^31, 2^31 + 1, 2^32 - 2, 2^32 - 1} (three in four)
+// and random words: these take Knuth's add-back branch (the quotient word estimate one too large after
+// the two-word correction) tens of thousands of times per million cases; against the oracle.
+template <int N> static void check_div_structured(uint64_t& state, long cases)
+{
+    const uint32_t pool[] = {0u, 1u, 2u, 0x7fffffffu, 0x80000000u, 0x80000001u, 0xfffffffeu, 0xffffffffu};
+    auto next = [&] {
+        state ^= state << 13;
+        state ^= state >> 7;
+        state ^= state << 17;
+        return state;
+    };
+    for (long c = 0; c < cases; ++c) {
+        lfo_num a{}, b{};
+        for (int i = 0; i < N; ++i) {
+            a.limb[i] = (next() & 3) ? pool[next() & 7] : (uint32_t)next();
+            b.limb[i] = (next() & 3) ? pool[next() & 7] : (uint32_t)next();
+        }
+        a.limb[N - 1] |= 0x80000000u;
+        b.limb[N - 1] |= 0x80000000u;
+        a.sign = (next() & 1) ? 1 : -1;
+        b.sign = 1;
+        a.exponent = (int)(next() % 7) - 3;
+        b.exponent = (int)(next() % 7) - 3;
+        report<N>("div structured", (int)c, lf::div<N>(to_dev<N>(a), to_dev<N>(b)), lfo_div(N, a, b));
+        if (c % 8 == 0) { // the same quotients through the reciprocal
+            uint32_t v[N + 1];
+            lf::reciprocal<N>(to_dev<N>(b).w, v);
+            report<N>("div_by_reciprocal structured", (int)c, lf::div_by_reciprocal<N>(to_dev<N>(a), to_dev<N>(b), v), lfo_div(N, a, b));
+        }
+    }
+}
+
 // RN to N words (ties to even) of ps * P * 2^pl + c, exactly, with GMP: the reference for fma_tail on
 // arbitrary 2N-word P (P need not be a product of two N-word mantissas).
 template <int N> static lf::Num<N> exact_round_sum(const uint32_t (&p)[2 * N], long pl, int ps, const lf::Num<N>& c)
@@ -499,6 +532,12 @@ int main(int argc, char** argv)
     check_coop<8>(coop_state, 200);
     uint64_t tail_state = 0x9E3779B97F4A7C15ull;
     check_fma_tail_sticky<4>(tail_state, cases);
+    uint64_t div_state = 0xD1B54A32D192ED03ull;
+    check_div_structured<2>(div_state, 400L * cases);
+    check_div_structured<3>(div_state, 200L * cases);
+    check_div_structured<4>(div_state, 100L * cases);
+    check_div_structured<8>(div_state, 20L * cases);
+    check_div_structured<32>(div_state, 2L * cases);
     std::printf("device core (host build): %ld checks, %ld failures\n", checks, failures);
     return failures ? 1 : 0;
 }
