@@ -36,17 +36,21 @@ constexpr int PT = 128;       // output tile per workgroup (rows and columns)
 constexpr int PK = 32;        // K per MFMA step
 constexpr u32 XCDS = 8;       // MI355X: 8 XCDs, workgroups dealt to them round-robin by linear id
 
-// One 32x32 block of residues: (65536 G11 + 256 Gmid + G00) mod m, written as uint16, in 32-bit integer
-// and FP32 arithmetic (residue_combine).
+// One 32x32 block of residues times the CRT factor y = (M/m)^-1 mod m: (65536 G11 + 256 Gmid + G00) y mod m,
+// written as uint16, in 32-bit integer and FP32 arithmetic (residue_fold; the factor rides on the digit
+// weights, so the CRT kernel reads y_q itself).
+struct FoldWeights {
+    int k11, kmid, k00;
+};
 __device__ __forceinline__ void store_block(uint16_t* out, u32 Np, u32 row, u32 col, u32 lane, const v16i& c11, const v16i& c00,
-                                            const v16i& cmid, int m, float inv, int c32)
+                                            const v16i& cmid, const FoldWeights& k, int m, float inv, int c32)
 {
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         // 32x32 accumulator layout: lane l, element i -> row 8(i/4) + 4(l/32) + i%4, column l%32
         const u32 rr = row + 8u * (u32)(i / 4) + 4u * (lane >> 5) + (u32)(i % 4);
         const u32 cc = col + (lane & 31u);
-        out[(u64)rr * Np + cc] = (uint16_t)residue_combine(c11[i], cmid[i], c00[i], m, inv, c32);
+        out[(u64)rr * Np + cc] = (uint16_t)residue_fold(c11[i], cmid[i], c00[i], k.k11, k.kmid, k.k00, m, inv, c32);
     }
 }
 
@@ -133,15 +137,16 @@ __global__ void __launch_bounds__(256, 2) product_kernel(ProductArgs g)
     const int m = (int)g.moduli[q];
     const float inv = 1.0f / (float)m;
     const int c32u = (int)(((u64)1 << 32) % (u64)m), c32 = c32u > m / 2 ? c32u - m : c32u; // balanced 2^32 mod m
+    const FoldWeights k{g.fold[3 * q], g.fold[3 * q + 1], g.fold[3 * q + 2]};
     uint16_t* out = g.res + (u64)q * g.Mp * g.Np;
     // one block at a time (the scheduling barriers keep the accumulators in AGPRs until their turn)
-    store_block(out, g.Np, row0 + wr, col0 + wc, lane, c11[0][0], c00[0][0], cmid[0][0], m, inv, c32);
+    store_block(out, g.Np, row0 + wr, col0 + wc, lane, c11[0][0], c00[0][0], cmid[0][0], k, m, inv, c32);
     __builtin_amdgcn_sched_barrier(0);
-    store_block(out, g.Np, row0 + wr, col0 + wc + 32u, lane, c11[0][1], c00[0][1], cmid[0][1], m, inv, c32);
+    store_block(out, g.Np, row0 + wr, col0 + wc + 32u, lane, c11[0][1], c00[0][1], cmid[0][1], k, m, inv, c32);
     __builtin_amdgcn_sched_barrier(0);
-    store_block(out, g.Np, row0 + wr + 32u, col0 + wc, lane, c11[1][0], c00[1][0], cmid[1][0], m, inv, c32);
+    store_block(out, g.Np, row0 + wr + 32u, col0 + wc, lane, c11[1][0], c00[1][0], cmid[1][0], k, m, inv, c32);
     __builtin_amdgcn_sched_barrier(0);
-    store_block(out, g.Np, row0 + wr + 32u, col0 + wc + 32u, lane, c11[1][1], c00[1][1], cmid[1][1], m, inv, c32);
+    store_block(out, g.Np, row0 + wr + 32u, col0 + wc + 32u, lane, c11[1][1], c00[1][1], cmid[1][1], k, m, inv, c32);
 }
 
 hipError_t launch_product(const ProductArgs& g, hipStream_t s)
@@ -691,9 +696,9 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     hipEvent_t e0 = ev[0], e1 = ev[4];
     const int WX = lf::rebuild_words(N, wx_class);
     const int G = P.opt.band_bits;
-    // constants: moduli | recip | chunk_pow[L][64] | shift_pow[L][G+1] | cinv | crt[L][WX] | bound[2][WX]
+    // constants: moduli | recip | chunk_pow[L][64] | shift_pow[L][G+1] | fold[L][3] | crt[L][WX] | bound[2][WX]
     std::vector<uint32_t> cst;
-    std::size_t o_mod = 0, o_rec = 0, o_chunk = 0, o_shift = 0, o_cinv = 0, o_crt = 0, o_bound = 0;
+    std::size_t o_mod = 0, o_rec = 0, o_chunk = 0, o_shift = 0, o_fold = 0, o_crt = 0, o_bound = 0;
     int tshift = 0;
     if (any_pair) {
         o_mod = cst.size();
@@ -718,13 +723,18 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
                 p = (p << 1) % m;
             }
         }
-        o_cinv = cst.size();
+        o_fold = cst.size();
         tshift = 32 * (WX - 1) - mbits;
         std::vector<std::vector<uint32_t>> crt(L);
         for (unsigned q = 0; q < L; ++q) {
             const uint32_t m = primes()[q];
             Big Mq = big_div_small(M, m);
-            cst.push_back(inverse_mod(big_mod_small(Mq, m), m));
+            // the product kernel's digit weights 65536 y, 256 y, y (mod m, balanced), y = (M/m)^-1 mod m
+            const uint64_t y = inverse_mod(big_mod_small(Mq, m), m);
+            for (uint64_t w : {uint64_t(65536), uint64_t(256), uint64_t(1)}) {
+                const int64_t k = int64_t(w * y % m);
+                cst.push_back(uint32_t(int32_t(k > int64_t(m / 2) ? k - int64_t(m) : k)));
+            }
             crt[q] = big_shifted(Mq, tshift, WX);
         }
         o_crt = cst.size();
@@ -806,7 +816,7 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
         pa.Kp = uint32_t(Kp);
         pa.L = L;
         pa.moduli = C32 + o_mod;
-        pa.recip = C32 + o_rec;
+        pa.fold = (const int32_t*)(C32 + o_fold);
         pa.lower = P.syrk;
         GEMM_HIP(lf::launch_product(pa, P.stream), "product launch");
     } else
@@ -818,9 +828,6 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
         r.Mp = uint32_t(Mp);
         r.Np = uint32_t(Np);
         r.L = any_pair ? L : 0;
-        r.moduli = C32 + o_mod;
-        r.recip = C32 + o_rec;
-        r.cinv = C32 + o_cinv;
         r.crt = C32 + o_crt;
         r.bound = C32 + o_bound;
         r.t = tshift;

@@ -56,17 +56,14 @@ struct ProductArgs {
     uint16_t* res;     // [L][Mp][Np]
     uint32_t Mp, Np, Kp, L;
     const uint32_t* moduli;
-    const uint32_t* recip;
+    const int32_t* fold; // per modulus k11, kmid, k00: 65536 y, 256 y, y mod m (balanced), y = (M/m)^-1 mod m
     uint32_t lower;    // SYRK: skip tiles strictly above the diagonal
 };
 
 // Output-centric reconstruction: one lane per output C[i][j].
 struct RebuildArgs {
     const uint16_t* res;
-    uint32_t Mp, Np, L;
-    const uint32_t* moduli;
-    const uint32_t* recip;
-    const uint32_t* cinv;     // (M/m_q)^-1 mod m_q
+    uint32_t Mp, Np, L;       // res holds y_q = r_q (M/m_q)^-1 mod m_q (the product kernel's folded residues)
     const uint32_t* crt;      // [L][WX]: (M/m_q) * 2^t, words
     const uint32_t* bound;    // [2][WX]: M' = M * 2^t, then floor(M'/2)
     int32_t t;                // normalising shift of M' (the exact integer is X' / 2^t)

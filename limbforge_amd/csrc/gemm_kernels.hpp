@@ -4,12 +4,13 @@
 //   digits_kernel   each entry's exact band integer X = m * 2^(e-lo) reduced modulo every prime (16-bit
 //                   chunk x (2^16h mod m) dot products, then the band shift), balanced residue
 //                   v in (-m/2, m/2], split into int8 digits v = 256 d1 + d0 (linalg.metal:24-39)
-//   product_kernel  per modulus and 64x64 tile: G11 = D1 D1', G00 = D0 D0', Gmid = D1 D0' + D0 D1' on
+//   product_kernel  per modulus and 128x128 tile: G11 = D1 D1', G00 = D0 D0', Gmid = D1 D0' + D0 D1' on
 //                   v_mfma_i32_32x32x32_i8 with int32 accumulation from LDS-staged tiles, then
-//                   (65536 G11 + 256 Gmid + G00) mod m written as uint16 (linalg.metal:66-96)
-//   rebuild_kernel  one lane per output: for each band pair, the exact integer by direct CRT
-//                   X' = sum_q y_q (M/m_q) 2^t with y_q = r_q (M/m_q)^-1 mod m_q (independent 64-bit
-//                   column sums, no carry chains), reduced mod M' and mapped to the symmetric range,
+//                   y = (65536 G11 + 256 Gmid + G00) (M/m)^-1 mod m written as uint16 (linalg.metal:66-96
+//                   writes the residue itself; the CRT factor rides on the digit weights here)
+//   crt_pairs_kernel, rebuild_kernel  one lane per output: for each band pair, the exact integer by direct
+//                   CRT X' = sum_q y_q (M/m_q) 2^t (independent 64-bit column sums, no carry chains),
+//                   reduced mod M' and mapped to the symmetric range,
 //                   then one rounding, or exact accumulation over the band pairs then one rounding
 //                   (the reference's Garner `reconstruct` + `finish`, linalg.metal:109-152; the same integer)
 // Exact dot kernels (plain limb GEMM and the lines the residue path does not take, linalg.hpp:268-318):
@@ -141,20 +142,19 @@ template <int N> __global__ void __launch_bounds__(256) digits_kernel(DigitsArgs
 }
 
 // ---------------------------------------------------------------- CRT rebuild
-// X' = |sum| * 2^t from the L residues at (row, col); returns the sign (+1/-1) of the symmetric value.
-// The CRT words and modulus constants come from LDS (staged by the kernel, rows padded to whole 16-byte
-// groups, read 16 bytes at a time) for wide integers, from scalar loads otherwise.  Measured on the
-// 512-bit 512^3 products: 67-word rows 300 us scalar against 173 us staged; 39-word rows 500 us scalar
+// X' = |sum| * 2^t from the L values y_q = r_q (M/m_q)^-1 mod m_q at (row, col) (the product kernel writes
+// the residues already multiplied by that factor); returns the sign (+1/-1) of the symmetric value.  The
+// CRT words come from LDS (staged by the kernel, rows padded to whole 16-byte groups, read 16 bytes at a
+// time) for wide integers, from scalar loads otherwise.  Measured on the 512-bit 512^3 products (with the
+// factor still applied here): 67-word rows 300 us scalar against 173 us staged; 39-word rows 500 us scalar
 // against 531 us staged.
 constexpr int CRT_LDS_MIN_WX = 48;
 template <int WX>
 __device__ __forceinline__ int crt_rebuild(const RebuildArgs& g, const u32* tab, u32 row, u32 col, u32 (&X)[WX])
 {
-    // tab (LDS, when WX >= CRT_LDS_MIN_WX): the L rows of WX CRT words, then per modulus m,
-    // floor(2^32/m), (M/m)^-1 mod m
+    // tab (LDS, when WX >= CRT_LDS_MIN_WX): the L rows of WX CRT words
     constexpr bool STAGED = WX >= CRT_LDS_MIN_WX;
     constexpr int WXP = (WX + 3) & ~3;
-    const u32* mods = tab + g.L * WXP;
     u64 colsum[WX];
 #pragma unroll
     for (int w = 0; w < WX; ++w) colsum[w] = 0;
@@ -171,9 +171,12 @@ __device__ __forceinline__ int crt_rebuild(const RebuildArgs& g, const u32* tab,
         for (u32 u = 0; u < RB; ++u) {
             const u32 q = q0 + u;
             if (q >= g.L) break;
+            u32 y = rr[u];
+            // y < 2^16 is known to the compiler from the 16-bit load; knowing it, the compiler splits the
+            // 64-bit column sums over unaligned register pairs and copies every one around its
+            // multiply-add (one extra move per v_mad_u64_u32).  The empty asm hides the range.
+            asm("" : "+v"(y));
             if constexpr (STAGED) {
-                const u32 m = mods[3 * q], cm = mods[3 * q + 1];
-                const u32 y = mulmod16(rr[u], mods[3 * q + 2], m, cm);
                 const uint4* C4 = reinterpret_cast<const uint4*>(tab + q * WXP);
 #pragma unroll
                 for (int w4 = 0; w4 < WXP / 4; ++w4) {
@@ -184,8 +187,6 @@ __device__ __forceinline__ int crt_rebuild(const RebuildArgs& g, const u32* tab,
                         if (4 * w4 + k < WX) colsum[4 * w4 + k] += (u64)y * cw[k];
                 }
             } else {
-                const u32 m = g.moduli[q], cm = g.recip[q];
-                const u32 y = mulmod16(rr[u], g.cinv[q], m, cm);
                 const u32* C = g.crt + q * WX;
 #pragma unroll
                 for (int w = 0; w < WX; ++w) colsum[w] += (u64)y * C[w];
@@ -277,9 +278,9 @@ template <int N> __device__ __forceinline__ Num<N> load_out(const u32* base, u64
 // Stage 1 of the rebuild: one lane per (band pair, output) of rows [row0, row0 + rows) reconstructs the
 // pair's exact integer by CRT, reduced to the symmetric range, into the stage (WX words and a sign word).
 // Only this stage touches the residues; the register budget holds the column sums and nothing else.
-// Wide CRT integers stage their words and modulus constants in LDS once per workgroup (dynamic shared
-// memory of crt_table_bytes(L, WX)).
-constexpr u64 crt_table_bytes(u32 L, int WX) { return WX >= CRT_LDS_MIN_WX ? (u64)L * (((WX + 3) & ~3) + 3) * 4 : 0; }
+// Wide CRT integers stage their words in LDS once per workgroup (dynamic shared memory of
+// crt_table_bytes(L, WX)).
+constexpr u64 crt_table_bytes(u32 L, int WX) { return WX >= CRT_LDS_MIN_WX ? (u64)L * ((WX + 3) & ~3) * 4 : 0; }
 template <int WX> __global__ void __launch_bounds__(256) crt_pairs_kernel(RebuildArgs g)
 {
     constexpr int WXP = (WX + 3) & ~3;
@@ -288,11 +289,6 @@ template <int WX> __global__ void __launch_bounds__(256) crt_pairs_kernel(Rebuil
         for (u32 k = threadIdx.x; k < g.L * WXP; k += 256u) {
             const u32 q = k / WXP, w = k % WXP;
             tab[k] = w < (u32)WX ? g.crt[q * WX + w] : 0u;
-        }
-        for (u32 q = threadIdx.x; q < g.L; q += 256u) {
-            tab[g.L * WXP + 3 * q] = g.moduli[q];
-            tab[g.L * WXP + 3 * q + 1] = g.recip[q];
-            tab[g.L * WXP + 3 * q + 2] = g.cinv[q];
         }
         __syncthreads();
     }

@@ -240,9 +240,11 @@ template <int N> void run_all(int lo, int hi, int cases)
     if constexpr (N < 32) run_all<N + 1>(lo, hi, cases);
 }
 
-// The residue GEMM's epilogue, (65536 G11 + 256 Gmid + G00) mod m, against exact 64-bit remainders over
-// every modulus the planner can use (the 1200 largest primes <= 65279), at the accumulators' extremes
-// (|G11|, |G00| <= 65472 * 2^14, |Gmid| <= 2 * 65472 * 2^14), near multiples of m and random values.
+// The residue GEMM's epilogue, (65536 G11 + 256 Gmid + G00) mod m (residue_combine) and the same times a
+// CRT factor y mod m through balanced digit weights (residue_fold, with y = 1, 2, (m+1)/2, m-1 and a random
+// value per modulus), against exact 64-bit remainders over every modulus the planner can use (the 1200
+// largest primes <= 65279), at the accumulators' extremes (|G11|, |G00| <= 65472 * 2^14,
+// |Gmid| <= 2 * 65472 * 2^14), near multiples of m and random values.
 static void check_residue_combine()
 {
     std::vector<int> primes;
@@ -265,6 +267,7 @@ static void check_residue_combine()
     const int64_t B = 65472LL * 16384, BM = 2 * B;
     const int64_t ends[] = {0, 1, -1, B, -B, B - 1, -B + 1, 1 << 29, -(1 << 29)};
     const int64_t mids[] = {0, 1, -1, BM, -BM, BM - 1, -BM + 1, 1 << 30, -(1 << 30)};
+    int64_t ys[5] = {};
     auto check = [&](int64_t a, int64_t b, int64_t c, int m) {
         ++checks;
         const int c32 = int((uint64_t(1) << 32) % uint64_t(m)), cb = c32 > m / 2 ? c32 - m : c32;
@@ -273,8 +276,23 @@ static void check_residue_combine()
         if ((int64_t)r != want && ++failures <= 30)
             std::printf("MISMATCH residue_combine g11=%lld gmid=%lld g00=%lld m=%d: %u, want %lld\n", (long long)a, (long long)b, (long long)c, m, r,
                         (long long)want);
+        for (int64_t y : ys) {
+            ++checks;
+            auto bal = [&](int64_t w) {
+                const int64_t k = w * y % m;
+                return int(k > m / 2 ? k - m : k);
+            };
+            const uint32_t f = lf::residue_fold((int)a, (int)b, (int)c, bal(65536), bal(256), bal(1), m, 1.0f / (float)m, cb);
+            const int64_t wf = want * y % m;
+            if ((int64_t)f != wf && ++failures <= 30)
+                std::printf("MISMATCH residue_fold g11=%lld gmid=%lld g00=%lld m=%d y=%lld: %u, want %lld\n", (long long)a, (long long)b, (long long)c, m,
+                            (long long)y, f, (long long)wf);
+        }
     };
     for (int m : primes) {
+        const int64_t fixed[4] = {1, 2, (m + 1) / 2, m - 1};
+        for (int k = 0; k < 4; ++k) ys[k] = fixed[k];
+        ys[4] = 1 + int64_t(next() % uint64_t(m - 1));
         for (int64_t a : ends)
             for (int64_t b : mids)
                 for (int64_t c : ends) check(a, b, c, m);
