@@ -522,7 +522,10 @@ def cholesky_section(oracle_check: bool):
 
     A = records(rows * n, np.round(8 * rng.standard_normal(rows * n)))
     B = records(n * nrhs, rng.integers(-10, 11, size=n * nrhs))
-    S, srep, syrk_g, syrk_w = capi.syrk(bits, A, rows, n, lower_only=False)
+    # one untimed call (the first launch of each kernel loads its code), then the median of three
+    capi.syrk(bits, A, rows, n, lower_only=False)
+    syrks = sorted((capi.syrk(bits, A, rows, n, lower_only=False) for _ in range(3)), key=lambda x: x[2])
+    S, srep, syrk_g, syrk_w = syrks[1]
     S0 = S.copy()
     one = np.zeros((n, w + 3), np.uint32)
     one[:, w - 1] = 0x80000000

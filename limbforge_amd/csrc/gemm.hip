@@ -696,7 +696,7 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
     hipEvent_t e0 = ev[0], e1 = ev[4];
     const int WX = lf::rebuild_words(N, wx_class);
     const int G = P.opt.band_bits;
-    // constants: moduli | recip | chunk_pow[L][64] | shift_pow[L][G+1] | fold[L][3] | crt[L][WX] | bound[2][WX]
+    // constants: moduli | reduce[L][2] | chunk_pow[L][64] | shift_pow[L][G+1] | fold[L][3] | crt[L][WX] | bound[2][WX]
     std::vector<uint32_t> cst;
     std::size_t o_mod = 0, o_rec = 0, o_chunk = 0, o_shift = 0, o_fold = 0, o_crt = 0, o_bound = 0;
     int tshift = 0;
@@ -704,7 +704,15 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
         o_mod = cst.size();
         for (unsigned q = 0; q < L; ++q) cst.push_back(primes()[q]);
         o_rec = cst.size();
-        for (unsigned q = 0; q < L; ++q) cst.push_back(uint32_t((uint64_t(1) << 32) / primes()[q]));
+        for (unsigned q = 0; q < L; ++q) {
+            const uint32_t m = primes()[q];
+            const float inv = 1.0f / float(m);
+            uint32_t bits;
+            std::memcpy(&bits, &inv, 4);
+            const int64_t c32 = int64_t((uint64_t(1) << 32) % m);
+            cst.push_back(bits);
+            cst.push_back(uint32_t(int32_t(c32 > int64_t(m / 2) ? c32 - int64_t(m) : c32)));
+        }
         o_chunk = cst.size();
         for (unsigned q = 0; q < L; ++q) {
             const uint32_t m = primes()[q];
@@ -784,7 +792,7 @@ int lf::run_product(const Product& P, lf_linalg_report* rep, float* device_ms)
         d.Kp = uint32_t(Kp);
         d.K = uint32_t(P.K);
         d.moduli = C32 + o_mod;
-        d.recip = C32 + o_rec;
+        d.reduce = C32 + o_rec;
         d.chunk_pow = C32 + o_chunk;
         d.shift_pow = C32 + o_shift;
         d.shift_max = uint32_t(G);

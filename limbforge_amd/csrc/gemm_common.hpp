@@ -40,7 +40,7 @@ struct DigitsArgs {
     uint32_t nlines, K, Kp, rows_p; // rows_p: extended rows (padded) of this side
     uint32_t L;
     const uint32_t* moduli;   // [L]
-    const uint32_t* recip;    // floor(2^32 / m) [L]
+    const uint32_t* reduce;   // [L][2]: the bits of 1.0f / m (correctly rounded), balanced 2^32 mod m
     const uint32_t* chunk_pow; // [L][64]: 2^(16h) mod m for the 16-bit mantissa chunks
     const uint32_t* shift_pow; // [L][G+1]: 2^s mod m
     uint32_t shift_max;        // G

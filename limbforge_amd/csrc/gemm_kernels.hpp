@@ -19,16 +19,12 @@
 #pragma once
 #include "gemm_common.hpp"
 #include "limb_core.hpp"
+#include "residue_reduce.hpp"
 
 namespace lf {
 
-__device__ __forceinline__ u32 reduce32(u32 x, u32 m, u32 c)
-{
-    u32 y = x - __umulhi(x, c) * m;
-    return y >= m ? y - m : y;
-}
-
-__device__ __forceinline__ u32 mulmod16(u32 a, u32 b, u32 m, u32 c) { return reduce32(a * b, m, c); }
+// a global table read through the constant address space (uniform loads become scalar loads)
+typedef __attribute__((address_space(4))) const u32 CU32;
 
 // ---------------------------------------------------------------- digits
 // One workgroup per (line, 256 consecutive k of [0, Kp)): thread t finds the band of entry k0 + t and its
@@ -89,37 +85,41 @@ template <int N> __global__ void __launch_bounds__(256) digits_kernel(DigitsArgs
     }
     bandof[t] = (uint8_t)band;
     const u64 plane = (u64)g.rows_p * g.Kp;
+    // the modulus tables are read through the constant address space: uniform loads from it are scalar
+    // loads (through a generic pointer the compiler must assume the digit stores below may alias them, and
+    // loads every table row into every lane)
+    const CU32* cpow = (const CU32*)g.chunk_pow;
+    const CU32* mods = (const CU32*)g.moduli;
+    const CU32* red = (const CU32*)g.reduce;
     for (u32 qb = 0; qb < g.L; qb += DIG_QB) {
         const u32 nq = g.L - qb < DIG_QB ? g.L - qb : DIG_QB;
         // every lane runs the same modulus loop (an entry outside every band has all chunks zero and
-        // yields zero digits); the next modulus's table row is loaded while this one is reduced, so the
-        // loads are not waited for one 16-byte piece at a time
-        u32 cp[2 * N], cpn[2 * N], m = g.moduli[qb], cm = g.recip[qb], sp = g.shift_pow[qb * (g.shift_max + 1) + (u32)shift];
+        // yields zero digits); the next modulus's table row is loaded while this one is reduced
+        u32 cp[2 * N], cpn[2 * N], sp = g.shift_pow[qb * (g.shift_max + 1) + (u32)shift];
 #pragma unroll
-        for (int h = 0; h < 2 * N; ++h) cp[h] = g.chunk_pow[qb * 64u + h];
+        for (int h = 0; h < 2 * N; ++h) cp[h] = cpow[qb * 64u + h];
         for (u32 qi = 0; qi < nq; ++qi) {
             const u32 q = qb + qi, qn = qi + 1 < nq ? q + 1 : q;
 #pragma unroll
-            for (int h = 0; h < 2 * N; ++h) cpn[h] = g.chunk_pow[qn * 64u + h];
-            const u32 mn = g.moduli[qn], cmn = g.recip[qn], spn = g.shift_pow[qn * (g.shift_max + 1) + (u32)shift];
+            for (int h = 0; h < 2 * N; ++h) cpn[h] = cpow[qn * 64u + h];
+            const u32 spn = g.shift_pow[qn * (g.shift_max + 1) + (u32)shift];
+            const int m = (int)mods[q], c32 = (int)red[2 * q + 1];
+            const float inv = __uint_as_float(red[2 * q]);
             u64 acc = 0;
 #pragma unroll
             for (int h = 0; h < 2 * N; ++h) acc += (u64)c[h] * cp[h];
-            // acc < 2N * 2^32: fold the high part with 2^32 mod m = cp[2]
-            u32 r = reduce32((u32)acc, m, cm) + (u32)(acc >> 32) * cp[2];
-            r = reduce32(r, m, cm);
-            r = mulmod16(r, sp, m, cm);
+            // acc < 2N 2^32 and r sp < 2^32: both within the 2^47 of residue_of (residue_reduce.hpp)
+            u32 r = residue_of((int64_t)acc, m, inv, c32);
+            r = residue_of((int64_t)(u32)__umul24(r, sp), m, inv, c32); // (__umul24 is declared int)
             int v = (int)r;
-            if (s < 0) v = v ? (int)m - v : 0;
-            if (v > (int)(m >> 1)) v -= (int)m;
+            if (s < 0) v = v ? m - v : 0;
+            if (v > (m >> 1)) v -= m;
             const int d0 = ((v + 128) & 255) - 128;
             const int d1 = (v - d0) / 256;
             stage[qi][0][t] = (int8_t)d1;
             stage[qi][1][t] = (int8_t)d0;
 #pragma unroll
             for (int h = 0; h < 2 * N; ++h) cp[h] = cpn[h];
-            m = mn;
-            cm = cmn;
             sp = spn;
         }
         __syncthreads();
