@@ -50,7 +50,8 @@ struct LinalgReport {
 // Blocked Cholesky and triangular solves (the reference's linalg.hpp:70-87).  The result depends on
 // `block` (the documented rounding sequence) and on nothing else; host_macs, solve_host_macs and gpu
 // choose the reference's host/GPU placement of updates, which never changes a bit; here every step runs
-// on the GPU.
+// on the GPU, and solve_host_macs picks direct exact dots (below it or up to 2^22 terms) or the update
+// GEMM (above; every off-block solve update when negative) for the solves' off-block updates.
 struct FactorOptions {
     std::size_t block = 32;       // column block nb; 0 selects a single block
     double host_macs = 5e4;

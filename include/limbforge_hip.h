@@ -241,7 +241,10 @@ int lf_syrk(int bits, const lf_operand* A, uint64_t rows, uint64_t cols, const l
  * The rounding sequence is fixed by `block` alone (docs/numerics.md:687-727): every entry takes
  * blk(j) + 2 roundings.  FactorOptions' host_macs, solve_host_macs and gpu choose the reference's
  * GPU/host placement of each update; placement never changes a bit, and here every step runs on the
- * GPU, so they are accepted and not used. */
+ * GPU.  host_macs and gpu are accepted and not used; solve_host_macs picks the form of an off-block
+ * solve update: direct exact dots below it or up to 2^22 multiply-add terms, else the update GEMM
+ * (a negative value sends every such update to the update GEMM).  Both forms round the same exact
+ * value once. */
 typedef struct lf_factor_options {
     uint64_t block;         /* column block nb (default 32); 0 = one block */
     double host_macs;       /* default 5e4 */

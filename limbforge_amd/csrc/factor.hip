@@ -178,6 +178,11 @@ double since(Clock::time_point t) { return std::chrono::duration<double>(Clock::
 
 // wavefronts (and scratch slots) of a direct off-block solve update
 constexpr uint64_t SOLVE_UPDATE_WAVES = 2048;
+// Off-block solve updates of up to this many multiply-add terms (rows x right-hand sides x block) also run
+// as direct exact dots: on 2048 wavefronts 2^22 terms of 256 bits take about 50 us, less than one update
+// GEMM's planning round trip (band analysis, line-table download, constants, about 0.4 ms of host time per
+// update at n = 512 with 64 right-hand sides).  Both forms round the same exact value once.
+constexpr double SOLVE_DIRECT_TERMS = 4194304.0;
 
 } // namespace
 
@@ -422,9 +427,13 @@ int lf_solve(int bits, int passes, const lf_operand* L, uint64_t n, const lf_ope
     };
     float ms = 0;
     // An off-block update of fewer multiply-adds than solve_host_macs (the reference's threshold for taking
-    // such an update off the GPU, linalg.hpp:73-78) runs as direct exact dots on the stream, without the
-    // update GEMM's planning and its synchronisation; larger ones are products of the update GEMM.
-    auto direct = [&](uint64_t rows, uint64_t K) { return double(rows) * double(nrhs) * double(K) < fo.solve_host_macs; };
+    // such an update off the GPU, linalg.hpp:73-78), or of at most SOLVE_DIRECT_TERMS, runs as direct exact
+    // dots on the stream, without the update GEMM's planning and its synchronisation; larger ones are
+    // products of the update GEMM.  A negative solve_host_macs sends every update to the update GEMM.
+    auto direct = [&](uint64_t rows, uint64_t K) {
+        const double terms = double(rows) * double(nrhs) * double(K);
+        return fo.solve_host_macs >= 0 && (terms < fo.solve_host_macs || terms <= SOLVE_DIRECT_TERMS);
+    };
     auto dot_update = [&](int backward, uint64_t r0, uint64_t r1) {
         lf::SolveArgs a{F, fpitch, Xp, xpitch, uint32_t(n), uint32_t(nrhs), uint32_t(r0), uint32_t(r1), (uint32_t*)dscratch.p, swords,
                         (int32_t*)derr.p, (const uint32_t*)drecip.p, rpitch};

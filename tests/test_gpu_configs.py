@@ -106,7 +106,11 @@ def test_syrk_cholesky_solve_256bit_n512(capi, oracle):
     same(L, WL, "Cholesky n=512 vs oracle")
     print("[gpu] cholesky_solve with 64 right-hand sides", flush=True)
     X, _, _ = capi.solve(bits, 3, L, n, B, nrhs, block=block)
-    same(X, oracle.solve(bits, WL, n, B, nrhs, block, 3), "solve 64 RHS vs oracle")
+    WX = oracle.solve(bits, WL, n, B, nrhs, block, 3)
+    same(X, WX, "solve 64 RHS vs oracle")
+    print("[gpu] cholesky_solve with 64 right-hand sides, every off-block update through the update GEMM", flush=True)
+    Xg, _, _ = capi.solve(bits, 3, L, n, B, nrhs, options=capi.FactorOptions.default(block=block, solve_host_macs=-1.0))
+    same(Xg, WX, "solve 64 RHS (update GEMM) vs oracle")
 
 
 def test_mixed_graph_all_widths(capi, oracle):
