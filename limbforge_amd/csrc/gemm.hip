@@ -105,11 +105,8 @@ __global__ void __launch_bounds__(256, 2) product_kernel(ProductArgs g)
     for (u32 st = 0; st + 1 < STAGES; ++st)
         if (st < steps) issue(st);
     for (u32 s = 0; s < steps; ++s) {
-        // stages issued after s (four loads each per wave): min(steps - 1 - s, STAGES - 2)
-        const u32 left = steps - 1 - s, ahead = left < STAGES - 2 ? left : STAGES - 2;
-        static_assert(STAGES >= 3 && STAGES <= 5, "vmcnt cases");
-        if (ahead == 3) wait_vmcnt<12>();
-        else if (ahead == 2) wait_vmcnt<8>();
+        const u32 ahead = steps - 1 - s; // stages issued after s, capped at STAGES - 2
+        if (ahead >= STAGES - 2) wait_vmcnt<4 * (STAGES - 2)>();
         else if (ahead == 1) wait_vmcnt<4>();
         else wait_vmcnt<0>();
         // a bare barrier: __syncthreads' fence would wait for every load in flight (vmcnt 0); the loads
