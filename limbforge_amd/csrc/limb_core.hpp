@@ -1207,16 +1207,22 @@ template <int W> __device__ __forceinline__ void accumulate_unit(u32 (&acc)[W], 
 //                                         below it (fma_tail), rounded once.
 // The cases after P is formed: p = the 2N-word product of two nonzero status-free operands whose
 // exponents sum to eab, ps = the product's sign.
+// Whether c is zero or |c| < 2^(p_lsb-1), half a unit of the last bit of the product P (2N words, exponent
+// sum eab): the exact sum is then P + t * delta, t the sign of c relative to P (0 for a zero c),
+// 0 < delta < that half unit, which is P's rounding with the tail nudged by t.
+template <int N> __device__ __forceinline__ bool fma_addend_below(i64 eab, const Num<N>& c)
+{
+    return !c.s || (i64)c.e < eab - 2 * (32 * N - 1) - 1;
+}
+
 template <int N> __device__ __forceinline__ Num<N> fma_tail(const u32 (&p)[2 * N], i64 eab, int ps, const Num<N>& c)
 {
-    if (!c.s) return round_product<N>(p, eab, ps, ST_OK);
+    // one branch for a zero and a far addend, so a wavefront whose lanes mix plain products (the
+    // multiplication steps of a chain) and far addends runs one rounding of P, not two
+    if (fma_addend_below<N>(eab, c)) return round_product<N>(p, eab, ps, ST_OK, false, ps * c.s);
     const bool top = (p[2 * N - 1] >> 31) != 0;
     const i64 p_msb = eab + (top ? 1 : 0);
-    const i64 p_lsb = eab - 2 * (32 * N - 1);
     const i64 c_msb = c.e, c_lsb = (i64)c.e - (32 * N - 1);
-    // |c| < 2^(p_lsb-1), half a unit of P's last bit: the exact sum is P + t * delta, t the sign of c
-    // relative to P, 0 < delta < that half unit, which is P's rounding with the tail nudged by t
-    if (c_msb < p_lsb - 1) return round_product<N>(p, eab, ps, ST_OK, false, ps * c.s);
     if (p_msb < c_lsb - 2) return c; // |p| < 2^(c_lsb-2)
     // Near case: a window of W = 2N+3 words with the larger term's top bit (top bit of P or of c, the one
     // with the higher exponent) at window bit 32W-2; the smaller term, aligned to it, may reach below bit 0
@@ -1292,10 +1298,14 @@ __device__ __forceinline__ Num<N> fma_by_digits(const Num<N>& a, const Num<N>& b
     u32 ad[digit_count<N>()], p[2 * N];
     to_digits<N>(a.w, ad);
     rounding_product<N>(ad, bd, p, c.s != 0, kill); // with c = 0 only the rounding of P is needed
-    // c = 0 (a chain's mul step) and status operands: the product's rounding, cleared when kill
-    if (!c.s || st) return round_product<N>(p, (i64)a.e + b.e, ps, st, kill);
+    // c = 0 (a chain's mul step) and status operands: the product's rounding, cleared when kill; an addend
+    // below half a unit of P's last bit: the same rounding with the tail nudged (fma_tail's first case),
+    // taken here so that the two share one copy of the rounding
+    const i64 eab = (i64)a.e + b.e;
+    const bool plain = !c.s || st;
+    if (plain || (ps && fma_addend_below<N>(eab, c))) return round_product<N>(p, eab, ps, st, kill, plain ? 0 : ps * c.s);
     if (!ps) return c; // RN(c) = c
-    return fma_tail<N>(p, (i64)a.e + b.e, ps, c);
+    return fma_tail<N>(p, eab, ps, c);
 }
 
 // Position of the highest set bit of x, or -1 (a scan over compile-time indices).
