@@ -1392,9 +1392,15 @@ template <int N> __device__ __forceinline__ Cpx<N> cmul(const Cpx<N>& a, const C
 template <int N> __device__ __forceinline__ Cpx<N> cdiv(const Cpx<N>& a, const Cpx<N>& b)
 {
     Num<N> d = add<N>(mul<N>(b.re, b.re), mul<N>(b.im, b.im));
-    Num<N> re = div<N>(add<N>(mul<N>(a.re, b.re), mul<N>(a.im, b.im)), d);
-    Num<N> im = div<N>(sub<N>(mul<N>(a.im, b.re), mul<N>(a.re, b.im)), d);
-    return {re, im};
+    const Num<N> nre = add<N>(mul<N>(a.re, b.re), mul<N>(a.im, b.im));
+    const Num<N> nim = sub<N>(mul<N>(a.im, b.re), mul<N>(a.re, b.im));
+    // both parts divide by d: its reciprocal once, then two divisions by it (each RN(n / d), as div)
+    u32 v[N + 1];
+    if (d.s && !d.st) reciprocal<N>(d.w, v);
+    else
+#pragma unroll
+        for (int k = 0; k <= N; ++k) v[k] = 0; // not read: a zero or status divisor is decided first
+    return {div_by_reciprocal<N>(nre, d, v), div_by_reciprocal<N>(nim, d, v)};
 }
 
 // ---------------------------------------------------------------- casts
