@@ -584,8 +584,11 @@ int lf_factor_step_cycles(int bits, int reps, uint64_t* cycles)
     FACTOR_HIP(tables().steps[N]((const uint32_t*)dv.p, pitch, (uint32_t*)ds.p, swords, (int32_t*)derr.p, reps, (uint64_t*)dout.p,
                                  (uint32_t*)dsink.p, st.s),
                "step timing launch");
+    int32_t err = 0;
     FACTOR_HIP(hipMemcpyAsync(cycles, dout.p, 6 * 8, hipMemcpyDeviceToHost, st.s), "download");
+    FACTOR_HIP(hipMemcpyAsync(&err, derr.p, 4, hipMemcpyDeviceToHost, st.s), "download status");
     FACTOR_HIP(hipStreamSynchronize(st.s), "step timing");
+    if (err) return fail(LF_LOGIC_ERROR, "step timing: exact-dot scratch overflow");
     return LF_OK;
 }
 
